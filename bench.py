@@ -1,0 +1,168 @@
+"""Benchmark: PCISPH particle-updates/s of the 1M-particle dam break on one MI355X.
+
+Workload (the metric's configuration): 128 x 64 x 128 = 1,048,576 fluid
+particles (spacing 0.01 m, jitter +-0.001 spacing, seed 0) in the corner of a
+2.56 x 0.64 x 1.28 m box with one layer of wall particles (~0.18M), PCISPH
+with min 3 / max 50 iterations and a 1 % density-error threshold, Morton sort
+every step, dt = 1 ms, FP32.  One particle-update = one fluid particle
+advanced one full step including all K iterations.
+
+Prints ONE JSON line.  `roofline` is for the kernel class with the largest
+device time (HIP events recorded by the library on its own stream around every
+launch during the timed steps); `cpu_baseline` times the CPU oracle (OpenMP
+port of the reference CPU path) on a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
+
+
+def step_bytes(n, nf, ncells, k):
+    """Algorithmic bytes of one step at K iterations (compulsory traffic:
+    each SoA field a pass touches once per pass; neighbour re-reads on chip).
+    key 20N, sort 12N, reorder 76N, cell table 4N + 8C, external forces
+    40N + 36N_f, per iteration predict 80N_f + density 16N + 12N_f + force
+    20N + 16N_f, integrate 96N_f."""
+    return 152 * n + 132 * nf + 8 * ncells + k * (36 * n + 108 * nf)
+
+
+# Algorithmic bytes per launch of each kernel class, from the same model.
+def class_bytes(name, n, nf, ncells):
+    table = {
+        "radix_sort": 12 * n + 20 * n,          # key read + (key, perm) write, keys from integration
+        "gather": 76 * n,
+        "cell_table": 4 * n + 8 * ncells,
+        "neighbours_external_force": 40 * n + 36 * nf + 80 * nf,  # + the fused first prediction
+        "predicted_density": 16 * n + 12 * nf,
+        "pressure_force_predict": 20 * n + 16 * nf + 80 * nf,     # + the fused next prediction
+        "integrate": 96 * nf,
+    }
+    return table.get(name)
+
+
+def run_gpu(args):
+    import unifiedparticleframeworkcuda_amd as upf
+    nx, ny, nz = args.lattice
+    p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz)
+    n = len(typ)
+    sim = upf.Simulation(p, n)
+    sim.upload(pos, typ)
+    sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
+    L = upf.lib()
+    L.upf_profile_enable.argtypes = [C.c_void_p, C.c_int]
+    L.upf_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.upf_profile_name.restype = C.c_char_p
+    L.upf_profile_reset.argtypes = [C.c_void_p]
+    L.upf_profile_enable(sim.h, 1)
+    L.upf_profile_reset(sim.h)
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sim.step(args.steps, upf.UPF_PCISPH_STATIC)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    K, rmax = sim.stats()
+    Kt = K[-args.steps:]
+    rt = rmax[-args.steps:]
+    keys, ids, cells = sim.sorted_state()
+    ncells = len(cells)
+    prof = {}
+    for c in range(L.upf_profile_classes()):
+        ms, cnt = C.c_double(), C.c_longlong()
+        L.upf_profile_read(sim.h, c, C.byref(ms), C.byref(cnt))
+        if cnt.value:
+            prof[L.upf_profile_name(c).decode()] = (ms.value, cnt.value)
+    sim.close()
+    return dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof)
+
+
+def cpu_baseline(args, budget_s):
+    """The oracle (OpenMP C++ port of the reference CPU path) on the same
+    workload: steps until the time budget is used (at least one)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    import unifiedparticleframeworkcuda_amd as upf
+    nx, ny, nz = args.lattice
+    p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz)
+    o = ol.OracleSim(p, pos, None, typ, omp=True)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step(1, upf.UPF_PCISPH_STATIC)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= args.cpu_max_steps:
+            break
+    threads = ol.oracle(True).orc_num_threads()
+    return dict(value=nf * steps / el, unit="particle-updates/s", cores=int(threads), kind="port",
+                sample="first %d step(s) of the same 1M dam break from its initial state, %.1f s" % (steps, el))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--lattice", type=int, nargs=3, default=[128, 64, 128])
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-max-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the workload is one scene on one MI355X "
+              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+        sys.exit(2)
+    g = run_gpu(args)
+    n, nf, ncells = g["n"], g["nf"], g["ncells"]
+    sec = g["seconds"]
+    value = nf * args.steps / sec
+    total_bytes = sum(step_bytes(n, nf, ncells, int(k)) for k in g["K"])
+    # dominant kernel class
+    dom = max(g["prof"].items(), key=lambda kv: kv[1][0])
+    name, (ms, cnt) = dom
+    avg_s = ms / cnt / 1e3
+    b = class_bytes(name, n, nf, ncells)
+    roof = dict(bound="hbm", kernel=name, launches=cnt, avg_us=avg_s * 1e6,
+                achieved=(b / avg_s / 1e9) if b else None, peak=HBM_PEAK / 1e9, unit="GB/s",
+                frac=(b / avg_s / HBM_PEAK) if b else None, traffic=None)
+    out = {
+        "metric": "PCISPH particle-updates/s @1M; batched vs plain at <=1% density error",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pcisph_dam_break_1m", "fluid": nf, "boundary": n - nf,
+                   "lattice": list(args.lattice), "spacing_m": 0.01, "dt_s": 0.001,
+                   "min_max_iterations": [3, 50], "density_error_threshold_pct": 1.0, "path": "plain"},
+        "iterations": {"mean": float(np.mean(g["K"])), "max": int(np.max(g["K"])),
+                       "hist": {str(int(k)): int(c) for k, c in zip(*np.unique(g["K"], return_counts=True))}},
+        "max_density_error_pct": float(max(0.0, 0.1 * float(np.max(g["rmax"])) - 100.0)),
+        "step_roofline": {"achieved": total_bytes / sec / 1e9, "unit": "GB/s", "frac": total_bytes / sec / HBM_PEAK,
+                          "bytes_per_update": total_bytes / (nf * args.steps)},
+        "kernel_ms_per_step": {k: v[0] / args.steps for k, v in g["prof"].items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

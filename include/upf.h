@@ -1,0 +1,299 @@
+/*
+ * upf.h -- C ABI of the MI355X (gfx950) per-step particle pipeline.
+ *
+ * The reference exposes its GPU path as an `extern "C"` block of functions
+ * that take a caller-owned `dataPointers&` plus ~35 global constant setters
+ * (src/gpu/particles.cuh:430-586).  This header replaces that block with a
+ * handle-based ABI: every function takes plain pointers and sizes, returns an
+ * int status (never exit()/abort()), and keeps no process-global state, so
+ * several simulations may live in one process.  Each entry point names the
+ * reference function(s) it replaces.
+ *
+ * Units and layouts
+ *   positions / velocities : float4 per particle (x, y, z, w), w ignored on
+ *                            input and 0 on output, SoA like the reference's
+ *                            d_pos_zindex / d_vel (src/gpu/particles.cuh:124-126)
+ *   type                   : int per particle, the reference's particle types
+ *                            (src/cpu/UnifiedParticle.h:14-23):
+ *                            0 rigid, 2 liquid, 6 frozen (static boundary)
+ *   keys                   : 30-bit z-index (Morton) codes, 10 bits per axis
+ *                            (src/cpu/zIndex.cpp:10-24)
+ */
+#ifndef UPF_H_
+#define UPF_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+/* The reference aborts or exits on these conditions; here they are codes.  */
+#define UPF_OK                  0
+#define UPF_ERR_HIP             1  /* HIP runtime error (CUDA_SAFE_CALL -> exit, src/cpu/cutil.h:761-769) */
+#define UPF_ERR_ARG             2  /* invalid argument / unsupported configuration */
+#define UPF_ERR_RANGE           3  /* a position maps outside the 1024^3 z-index grid (z_table_ overflow, src/cpu/zIndex.h:53) */
+#define UPF_ERR_NAN             4  /* NaN predicted position (abort at src/cpu/UnifiedPhysics.cpp:5446-5452) */
+#define UPF_ERR_NBR_OVERFLOW    5  /* a particle has more neighbours than the list capacity */
+#define UPF_ERR_DELTA           6  /* density error factor divisor is zero (exit at src/cpu/UnifiedPhysics.cpp:5677-5681) */
+#define UPF_ERR_NO_DEVICE       7  /* no HIP device */
+#define UPF_ERR_ISOLATED        8  /* boundary particle without boundary neighbours (abort at src/cpu/UnifiedPhysics.cpp:2739-2740) */
+#define UPF_ERR_STATE           9  /* call order violated (e.g. step before upload) */
+
+/* ------------------------------------------------------ particle types */
+#define UPF_RIGID   0
+#define UPF_LIQUID  2
+#define UPF_FROZEN  6
+
+/* ----------------------------------------------------------- methods */
+/* Step drivers; each replaces one CPU driver and its GPU counterpart.     */
+#define UPF_PCISPH_STATIC  0  /* CpuParticlePhysicsStaticBoundariesPCISPH src/cpu/UnifiedPhysics.cpp:4220 /
+                                 ParticlePhysicsOnDeviceFluidStaticBoundariesPCISPH src/gpu/particles.cu:2603 */
+#define UPF_WCSPH_STATIC   1  /* CpuParticlePhysicsStaticBoundariesSPH src/cpu/UnifiedPhysics.cpp:4180 (physicsType 'w') /
+                                 ParticlePhysicsOnDeviceSPH src/gpu/particles.cu:1281 */
+#define UPF_PCISPH_RIGID   2  /* CpuParticlePhysicsVersatileCouplingPCISPH src/cpu/UnifiedPhysics.cpp:4991 /
+                                 ParticlePhysicsOnDeviceFluidRigidCouplingPCISPH src/gpu/particles.cu:2515 */
+
+/* Boundary-pressure mode (Akinci boundary pressure force on liquid from
+ * boundary particles).  The reference CPU path passes an integer LUT index to
+ * kernelPressureGradLut(float) (src/cpu/UnifiedPhysics.cpp:4245), which makes
+ * the force identically zero; its GPU path indexes the table directly
+ * (src/gpu/particles_kernel.cuh:791-802).                                    */
+#define UPF_BOUNDARY_CPU_REF   0  /* as the CPU path computes it (zero force) */
+#define UPF_BOUNDARY_AKINCI    1  /* table indexed by the LUT bin, Psi of the boundary neighbour */
+
+/* ------------------------------------------------------------ params */
+/* Scalar part of UnifiedConstants (src/sceneConfigure/UnifiedConstants.h:13-219)
+ * and of dataPointers (src/gpu/particles.cuh:49-121) that the step reads.
+ * Filled by upf_params_init + upf_params_set_domain; fields may be edited
+ * afterwards (dt, stiffness, loop limits, coupling coefficients).          */
+typedef struct upf_params {
+    /* UnifiedConstants::setParticleSpacing (UnifiedConstants.cpp:168-191) */
+    float spacing;                 /* particleSpacing (delta)                      */
+    float particle_radius;         /* particleRadius = 0.5 * spacing               */
+    float support_radius;          /* globalSupportRadius h = 2.00002 * spacing    */
+    float force_distance;          /* forceDistance = 0.25 h                       */
+    float initial_mass;            /* initialMass = spacing^3 * rest density       */
+    float rest_density;            /* fluidRestDensity                             */
+    float visc_const;              /* fluidViscConst                               */
+    float gravity;                 /* gravityConst                                 */
+    float dt;                      /* deltaT of the step                           */
+    float gas_const_wcsph;         /* fluidGasConstantWCSPH (Tait stiffness)       */
+    float gamma;                   /* Tait exponent (7)                            */
+    float max_boundary_force;      /* maxBoundaryForce                             */
+    int   add_boundary_force;      /* addBoundaryForce                             */
+    /* PCISPH (UnifiedConstants.cpp:89-103,152-155)                         */
+    float density_error_factor;    /* densityErrorFactor (delta, already scaled)   */
+    float density_error_factor_parameter;
+    float max_density_error_allowed; /* percent                                   */
+    int   min_loops;
+    int   max_loops;
+    float kernel_self;             /* KernelM4(0, h)                               */
+    /* coupling (UnifiedConstants.cpp:85-87,113-116,158)                    */
+    float surface_tension_gamma;
+    float surface_adhesion_beta;
+    float spring_coefficient;
+    float spring_coefficient_boundary;
+    float damping_coefficient;
+    float terminal_speed;
+    int   boundary_pressure_mode;  /* UPF_BOUNDARY_*                               */
+    int   renormalize_quaternion;  /* 0: CPU path (no renorm), 1: GPU path          */
+    int   rigid_predicted_current; /* 0: CPU path (rigid x* stays at its initial
+                                      value), 1: rigid x* = x                       */
+    /* z-index domain (UnifiedConstants::setBoundingBoxValues :308-382)     */
+    float vbox_min[3];             /* virtual bounding box (origin must be 0)      */
+    float vbox_max[3];
+    float scales[3];               /* GRID_RESOLUTION / extent                     */
+    uint32_t block_size;           /* ZIndex::block_size_ (power of two)          */
+    uint32_t block_shift;          /* log2(block_size)                             */
+    uint32_t grid_dim;             /* cells per axis = 1024 / block_size           */
+    float collision_min[3];        /* collisionBox                                 */
+    float collision_max[3];
+    /* neighbour search                                                        */
+    float support_radius_sq_limit; /* smallest float T with sqrtf(T) >= h: a pair
+                                      is a neighbour iff |x_i-x_j|^2 < T, which is
+                                      exactly `sqrtf(d2) < h` of the CPU path      */
+    int   max_neighbors;           /* neighbour list capacity per particle         */
+} upf_params;
+
+/* ------------------------------------------------------ host helpers */
+/* No device is touched by these.                                          */
+
+/* UnifiedConstants ctor + setParticleSpacing + computeGasConstAndTimeStep
+ * (src/sceneConfigure/UnifiedConstants.cpp:7-191,241-306), for a given
+ * spacing; dt defaults to deltaT_pcisph.  */
+int upf_params_init(float spacing, upf_params* out);
+
+/* setBoundingBoxValues (UnifiedConstants.cpp:308-382) with the virtual box and
+ * collision box given by the caller, plus the block-size rule of Scene::build /
+ * ZIndex::SetBlockSize (src/sceneConfigure/Scene.cpp:29-34, src/cpu/zIndex.cpp:27-40).
+ * Returns UPF_ERR_ARG when vbox_min is not the origin or when the block is
+ * smaller than the scaled support radius (cells must hold the h-neighbourhood). */
+int upf_params_set_domain(upf_params* p, const float vbox_max[3],
+                          const float collision_min[3], const float collision_max[3]);
+
+/* WeightingKernel::initLookupTables (src/cpu/WeightingKernel.cpp:303-320):
+ * five 301-entry tables (entry 300 = 0) in this order: M4, pressure gradient,
+ * viscosity laplacian, surface tension spline, adhesion spline.  out: 5*301 floats. */
+int upf_build_luts(const upf_params* p, float* out);
+
+/* ComputeDensityErrorFactor (src/cpu/UnifiedPhysics.cpp:5525-5689): fills
+ * p->density_error_factor from the prototype block.  *max_neighbors_out may be NULL. */
+int upf_compute_density_error_factor(upf_params* p, int* max_neighbors_out);
+
+/* Seeded dam-break scene (replaces Scene::build / createFluidObject /
+ * createBoxBoundaryFrozenParticles, src/sceneConfigure/Scene.cpp:23-138,597-645,890-1042):
+ * an nx*ny*nz fluid lattice at (i+1/2)*spacing with uniform jitter of
+ * +-jitter_rel*spacing per component in the corner of a box of extent box[3],
+ * walls sampled every wall_rel*spacing on the floor and four sides (open top).
+ * Call with pos4 == NULL to get the counts.  Also sets the domain of *p.     */
+typedef struct upf_dam_break_desc {
+    int   nx, ny, nz;
+    float box[3];          /* container extent at lattice_rel = 1 (scaled by lattice_rel) */
+    float jitter_rel;
+    float wall_rel;        /* wall sampling / spacing                                  */
+    float lattice_rel;     /* fluid lattice spacing / spacing (1.0025 puts the nearest
+                              neighbours mid-bin of the 300-entry kernel table, so the
+                              lattice starts at the table kernel's rest density)       */
+    uint64_t seed;
+} upf_dam_break_desc;
+int upf_scene_dam_break(const upf_dam_break_desc* d, upf_params* p,
+                        int* n_fluid, int* n_wall, float* pos4, int* type);
+
+/* Counts of the rigid-fluid coupling scene: a fluid block plus n_bodies seeded
+ * boxes and spheres sampled on their surface (see upf_scene_rigid).  */
+typedef struct upf_rigid_scene_desc {
+    int   nx, ny, nz;       /* fluid lattice                                 */
+    float box[3];           /* container extent                              */
+    int   n_bodies;
+    float size_min, size_max;       /* edge / diameter range (m)             */
+    float density_min, density_max; /* body density range (kg/m^3)           */
+    float jitter_rel;
+    float surface_rel;      /* surface sampling / spacing                    */
+    uint64_t seed;
+} upf_rigid_scene_desc;
+
+/* Rigid body state, one per body (replaces RigidBody_GPU, src/gpu/particles.cuh:27-44,
+ * without its fixed int[15000] index array).  Matrices are row-major 3x3.   */
+typedef struct upf_body {
+    float pos[3];               /* centre of mass                             */
+    float mass;
+    float vel[3];
+    float pad0;
+    float angular_velocity[3];
+    float pad1;
+    float linear_momentum[3];
+    float pad2;
+    float angular_momentum[3];
+    float pad3;
+    float quaternion[4];        /* (s, vx, vy, vz)                            */
+    float rotation[9];
+    float inv_inertia_local[9];
+    float inv_inertia_world[9];
+    float force[3];             /* last reduced force  (diagnostic)           */
+    float torque[3];            /* last reduced torque (diagnostic)           */
+} upf_body;
+
+/* Rigid scene.  Call with pos4 == NULL to get the counts.  body_id / rel
+ * (initial offset from the centre of mass, float4) are per particle, -1 / 0
+ * for non-rigid particles.  Also sets the domain of *p.                    */
+int upf_scene_rigid(const upf_rigid_scene_desc* d, upf_params* p,
+                    int* n_fluid, int* n_rigid, int* n_wall,
+                    float* pos4, int* type, int* body_id, float* rel4,
+                    upf_body* bodies);
+
+/* ------------------------------------------------------- simulation */
+typedef struct upf_sim upf_sim;
+
+/* MallocDeviceArraysParticles + SetDeviceConstants + CreateLut* (src/gpu/particles.cu:100-261).
+ * capacity = maximum particle count; the device is the current HIP device.  */
+int upf_create(const upf_params* p, int capacity, upf_sim** out);
+/* FreeDeviceArrays (src/gpu/particles.cu). */
+int upf_destroy(upf_sim* s);
+/* Replace the parameters (dt, stiffness, loop limits ...); the domain must not change. */
+int upf_set_params(upf_sim* s, const upf_params* p);
+
+/* CopyZIndicesKVPairHostToDevice + CopyParticleDataHostToDevice
+ * (src/gpu/particles.cu:349-501): host buffers are borrowed for the call.
+ * vel4 may be NULL (zero velocity).  The upload order defines particle ids
+ * 0..n-1.  Computes the z-index keys of the uploaded positions.  Boundary
+ * volumes (Psi, CalculateWeightedVolume src/cpu/UnifiedPhysics.cpp:2703-2746)
+ * are computed on the first step.                                        */
+int upf_upload_particles(upf_sim* s, int n, const float* pos4, const float* vel4,
+                         const int* type);
+/* CopyRigidBodyDataHostToDevice (src/gpu/particles.cu:563): bodies and the
+ * per-particle body id / initial relative position (upload order).        */
+int upf_upload_bodies(upf_sim* s, int n_bodies, const upf_body* bodies,
+                      const int* body_id, const float* rel4);
+
+/* One or more full time steps of `method` (UPF_*), equivalent to calling the
+ * reference's per-frame driver nsteps times.                              */
+int upf_step(upf_sim* s, int nsteps, int method);
+
+/* CopyParticlesCUDA (src/gpu/particles.cu:62-72) in upload (id) order.  Any
+ * pointer may be NULL.  pos4/vel4: float4; rho = last predicted density
+ * (PCISPH) or density (WCSPH); p = correction pressure (PCISPH) or pressure
+ * (WCSPH); force4 = external force of the last step; psi = boundary volume.  */
+int upf_download(upf_sim* s, float* pos4, float* vel4, float* rho, float* p,
+                 float* force4, float* psi, int* type);
+int upf_download_bodies(upf_sim* s, upf_body* bodies);
+
+/* Per-step PCISPH statistics of the last upf_step call (the reference prints
+ * them behind printDebuggingInfo, src/cpu/UnifiedPhysics.cpp:5391-5394):
+ * iterations K and max predicted density of the last iteration, per step. */
+int upf_get_step_stats(upf_sim* s, int max_steps, int* iterations, float* max_rho);
+
+/* ------------------------------------------- stage entry points (tests) */
+/* FillZindicesKernel / CalcIndex (src/gpu/particles_kernel.cu:3360,
+ * src/cpu/zIndex.h:40-45) on the current positions, then RadixSort +
+ * CopySortedParticleValues + BlockGeneration (src/gpu/particles.cu:31-60,904-964):
+ * sorts the particle state by key (stable) and builds the cell table.     */
+int upf_sort_and_bin(upf_sim* s);
+/* Sorted keys, ids (the permutation: sorted slot -> upload id), and the
+ * occupied-cell table (cell Morton code, first slot, end slot) in ascending
+ * cell order.  *n_cells receives the number of occupied cells; cells may be
+ * NULL to only query it.                                                   */
+int upf_download_sorted(upf_sim* s, uint32_t* keys, int32_t* ids,
+                        uint32_t* cells, int* n_cells);
+
+/* Device helpers used by the parity tests: keys of arbitrary positions and
+ * a stable sort of arbitrary (key) arrays, both fully on the device.       */
+int upf_keys(const upf_params* p, int n, const float* pos4, uint32_t* keys);
+int upf_radix_sort(int n, const uint32_t* keys_in, uint32_t* keys_out, int32_t* perm_out);
+
+/* ------------------------------------------------------ batched form */
+/* Many independent scenes stepped together (no reference counterpart; the
+ * reference holds one simulation per process).  Scenes share the params
+ * except for their domain, which comes from each scene's own params.      */
+typedef struct upf_batch upf_batch;
+int upf_batch_create(int n_scenes, const upf_params* scene_params,
+                     const int* counts, upf_batch** out);
+int upf_batch_destroy(upf_batch* b);
+/* Particles of all scenes concatenated scene by scene. */
+int upf_batch_upload(upf_batch* b, const float* pos4, const float* vel4, const int* type);
+/* nsteps PCISPH static-boundary steps of every scene.  use_graph: capture the
+ * PCISPH correction loop of the step in a hipGraph with the convergence test
+ * on the device.                                                           */
+int upf_batch_step(upf_batch* b, int nsteps, int use_graph);
+int upf_batch_download(upf_batch* b, float* pos4, float* vel4, float* rho);
+int upf_batch_get_iterations(upf_batch* b, int max_entries, int* iterations);
+
+/* ------------------------------------------------------ profiling */
+/* Per-kernel-class device time measured with HIP events recorded on the
+ * simulation's own stream around every launch (the reference's
+ * SPH_PROFILING_VERBOSE cudaEvent timers, src/gpu/particles.cu:1460-1690).
+ * Classes are numbered 0 .. upf_profile_classes()-1.                      */
+int upf_profile_enable(upf_sim* s, int on);
+int upf_profile_reset(upf_sim* s);
+int upf_profile_classes(void);
+const char* upf_profile_name(int cls);
+int upf_profile_read(upf_sim* s, int cls, double* total_ms, long long* launches);
+
+/* ------------------------------------------------------ version */
+const char* upf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UPF_H_ */

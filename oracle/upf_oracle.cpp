@@ -1,0 +1,1145 @@
+// upf_oracle.cpp -- CPU ORACLE (test infrastructure only).
+//
+// A C++ restatement of the reference's CPU per-step pipeline (src/cpu) used as
+// the checker of the HIP library.  Only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg may load it; the product never does.
+//
+// Every function cites the reference function it restates.  Float expressions
+// keep the reference's evaluation order (vmmlib Vector3 operators evaluate
+// left to right, operator/(float) multiplies by 1.0f/a, src/vmmlib/vector3.h:584-606);
+// the file must be compiled with -ffp-contract=off and without fast-math.
+//
+// Deviations from the reference CPU path (all documented in the divergence
+// ledger of the design notes):
+//   * stable sort by key instead of std::sort (ties keep the previous order);
+//   * the max predicted density is a true reduction, not a racy OpenMP update;
+//   * conditions on which the reference calls abort()/exit() set an error
+//     code instead;
+//   * uninitialised fields of the reference particle (predicted density and
+//     previous correction pressure before the first step) start at rho0 / 0.
+//
+// Parity status per function (see tests/test_oracle_*.py):
+//   params / LUTs / z-table / block size / BoxQuery ranges / rigid-body
+//   integration : pinned against the reference's own translation units compiled
+//   into oracle/_ref (when /root/reference is present) and closed forms;
+//   neighbour search : brute force; sort : std::stable_sort and invariants;
+//   density error factor : closed-form lattice sum; PCISPH / WCSPH step :
+//   conservation laws, hydrostatic rest, FP64 twin of the same restatement.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <utility>
+#include <vector>
+
+#include "../include/upf.h"
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+template <class T> struct V3 {
+    T x, y, z;
+    V3() : x(0), y(0), z(0) {}
+    V3(T a, T b, T c) : x(a), y(b), z(c) {}
+    V3 operator+(const V3& o) const { return V3(x + o.x, y + o.y, z + o.z); }
+    V3 operator-(const V3& o) const { return V3(x - o.x, y - o.y, z - o.z); }
+    V3 operator*(T a) const { return V3(x * a, y * a, z * a); }
+    V3 operator*(const V3& o) const { return V3(x * o.x, y * o.y, z * o.z); }
+    // vmmlib Vector3<float>::operator/(float): a = 1/a, multiply (vector3.h:598-604)
+    V3 div(T a) const { T r = T(1) / a; return V3(x * r, y * r, z * r); }
+    V3& operator+=(const V3& o) { x = x + o.x; y = y + o.y; z = z + o.z; return *this; }
+    V3& operator-=(const V3& o) { x = x - o.x; y = y - o.y; z = z - o.z; return *this; }
+    T& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+    T operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+    T length_sq() const { return x * x + y * y + z * z; }        // vector3.h:473-476
+    T dot(const V3& o) const { return x * o.x + y * o.y + z * o.z; }
+    V3 cross(const V3& o) const {
+        return V3(y * o.z - z * o.y, z * o.x - x * o.z, x * o.y - y * o.x);
+    }
+};
+
+template <class T> inline T vsqrt(T v);
+template <> inline float vsqrt<float>(float v) { return sqrtf(v); }
+template <> inline double vsqrt<double>(double v) { return std::sqrt(v); }
+
+template <class T> inline T dist3(const V3<T>& a, const V3<T>& b) {  // Vector3::distance (vector3.h:482-485)
+    return vsqrt<T>((a - b).length_sq());
+}
+
+// MAX / MIN macros of src/sceneConfigure/global.h:4-5 (keep their NaN behaviour)
+template <class T> inline T MAXr(T a, T b) { return a >= b ? a : b; }
+template <class T> inline T MINr(T a, T b) { return a <= b ? a : b; }
+
+// ----------------------------------------------------------------- z-index
+// ZIndex::ZIndex table (src/cpu/zIndex.cpp:10-24)
+struct ZTable {
+    uint32_t t[1024][3];
+    ZTable() {
+        for (int k = 0; k < 1024; ++k) {
+            t[k][0] = t[k][1] = t[k][2] = 0;
+            for (int i = 0; i < 10; ++i) {
+                t[k][0] |= ((((unsigned)k >> i) & 1u) << (3 * i));
+                t[k][1] |= ((((unsigned)k >> i) & 1u) << (3 * i + 1));
+                t[k][2] |= ((((unsigned)k >> i) & 1u) << (3 * i + 2));
+            }
+        }
+    }
+    uint32_t calc(unsigned x, unsigned y, unsigned z) const {  // CalcIndex (zIndex.h:40-45)
+        return t[x][0] | t[y][1] | t[z][2];
+    }
+};
+static const ZTable& ztab() { static ZTable z; return z; }
+
+static const uint32_t XMASK = 0xB6DB6DB6u, YMASK = 0x6DB6DB6Du, ZMASK = 0xDB6DB6DBu;  // zIndex.h:12-14
+static inline uint32_t xinc(uint32_t i) { return (((i | XMASK) + 1) & ~XMASK) | (i & XMASK); }
+static inline uint32_t yinc(uint32_t i) { return (((i | YMASK) + 1) & ~YMASK) | (i & YMASK); }
+static inline uint32_t zinc(uint32_t i) { return (((i | ZMASK) + 1) & ~ZMASK) | (i & ZMASK); }
+
+// float -> unsigned short conversion of vmmlib's converting ctor (vector3.h:300-304);
+// returns false when the coordinate is outside the z_table.
+template <class T> static inline bool to_grid(T v, unsigned& out) {
+    if (!(v >= T(0)) || v >= T(1024)) { out = v >= T(1024) ? 1023u : 0u; return false; }
+    out = (unsigned)(unsigned short)v;
+    return true;
+}
+
+// ZIndex::BoxQuery (zIndex.cpp:43-74): index ranges of the cells covering a box
+static void box_query(unsigned bs, const unsigned mn[3], const unsigned mx[3],
+                      std::vector<std::pair<uint32_t, uint32_t>>& ranges) {
+    const uint32_t offset = bs * bs * bs;
+    unsigned s[3] = {mn[0] / bs, mn[1] / bs, mn[2] / bs};
+    unsigned e[3] = {mx[0] / bs, mx[1] / bs, mx[2] / bs};
+    ranges.clear();
+    uint32_t xind = ztab().calc(s[0], s[1], s[2]);
+    for (unsigned i = s[0]; i <= e[0]; ++i) {
+        uint32_t yind = xind;
+        for (unsigned j = s[1]; j <= e[1]; ++j) {
+            uint32_t zind = yind;
+            for (unsigned k = s[2]; k <= e[2]; ++k) {
+                ranges.push_back(std::make_pair(offset * zind, offset * (zind + 1) - 1));
+                zind = zinc(zind);
+            }
+            yind = yinc(yind);
+        }
+        xind = xinc(xind);
+    }
+    std::sort(ranges.begin(), ranges.end());  // QuickSort of the ranges (UnifiedPhysics.cpp:2259)
+}
+
+// ------------------------------------------------------------- parameters
+// UnifiedConstants ctor / setParticleSpacing / computeGasConstAndTimeStep
+// (src/sceneConfigure/UnifiedConstants.cpp:7-191,241-306)
+static void params_init(float spacing, upf_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    const float sizeFactor = 1.0f;
+    p->rest_density = 1000;
+    p->gravity = 9.81f;
+    p->gamma = 7.0f;
+    // setParticleSpacing (:168-191)
+    p->spacing = spacing;
+    p->particle_radius = spacing * 0.5f;
+    p->support_radius = (float)((double)spacing * 2.0 * 1.00001);
+    p->force_distance = (float)(0.25 * (double)p->support_radius);
+    p->initial_mass = spacing * spacing * spacing * p->rest_density;
+    p->visc_const = (float)(1.0 * sizeFactor);
+    // computeGasConstAndTimeStep(maxDensityErrorAllowed = 1.0) (:246-265)
+    p->max_density_error_allowed = 1.0f;
+    const float deltaT_pcisph = 0.0012f;
+    p->gas_const_wcsph = 2000;
+    p->dt = deltaT_pcisph;
+    // densityErrorFactorParameter (:102-103)
+    const float factor = deltaT_pcisph / 0.0002f;
+    p->density_error_factor_parameter = (float)(0.05 * factor * factor);
+    p->spring_coefficient = 300.0f;
+    p->spring_coefficient_boundary = 300.0f;
+    p->damping_coefficient = 0.5f;
+    p->surface_tension_gamma = 0.3f;
+    p->surface_adhesion_beta = 0.0f;
+    p->add_boundary_force = 0;
+    p->max_boundary_force = sizeFactor * 64 * spacing * spacing;  // boundaryForceFactor = 64 (:130,143)
+    p->min_loops = 3;
+    p->max_loops = 50;
+    p->terminal_speed = 20.0f;
+    p->boundary_pressure_mode = UPF_BOUNDARY_CPU_REF;
+    p->max_neighbors = 160;
+}
+
+// KernelM4 (src/cpu/WeightingKernel.cpp:75-97)
+template <class T> static T kernel_m4(T dist, T sr) {
+    T s = dist / sr;
+    T result;
+    T factor = T(2.546479089470325472) / (sr * sr * sr);
+    if (dist < T(0) || dist >= sr) return T(0);
+    if (s < T(0.5)) result = (T)(T(1) - 6.0 * s * s + T(6) * s * s * s);
+    else { T tmp = T(1) - s; result = (T)(2.0 * tmp * tmp * tmp); }
+    return factor * result;
+}
+// kernelPressureGrad (:165-180)
+template <class T> static T kernel_pressure_grad(T dist, T sr) {
+    if (dist == 0) return T(0);
+    if (dist > sr) return T(0);
+    T c = T(-45) / ((T(3.1415926535897932384626) * sr * sr * sr * sr * sr * sr));
+    return c / dist * (sr - dist) * (sr - dist);
+}
+// kernelViscosityLaplacian (:190-200)
+template <class T> static T kernel_visc_lap(T dist, T sr) {
+    if (dist > sr) return T(0);
+    T c = T(45) / ((T(3.1415926535897932384626) * sr * sr * sr * sr * sr * sr));
+    return c * (sr - dist);
+}
+// kernelSplineSurfaceTension (:210-233)
+template <class T> static T kernel_tension(T dist, T sr) {
+    const T c = T(32) / ((T(3.1415926535897932384626) * sr * sr * sr * sr * sr * sr * sr * sr * sr));
+    if (dist > 0.5 * sr && dist <= sr)
+        return c * (sr - dist) * (sr - dist) * (sr - dist) * dist * dist * dist;
+    else if (dist > 0 && dist <= 0.5 * sr)
+        return c * (T(2) * (sr - dist) * (sr - dist) * (sr - dist) * dist * dist * dist - sr * sr * sr * sr * sr * sr / T(64));
+    return T(0);
+}
+template <class T> static T vpow(T a, T b);
+template <> float vpow<float>(float a, float b) { return powf(a, b); }
+template <> double vpow<double>(double a, double b) { return std::pow(a, b); }
+// kernelSplineSurfaceAdhesion (:243-263)
+template <class T> static T kernel_adhesion(T dist, T sr) {
+    const T c = T(0.007) / (vpow<T>(sr, T(3.25)));
+    if (dist > 0.5 * sr && dist <= sr) {
+        const T base = T(-4) * dist * dist / sr + T(6) * dist - T(2) * sr;
+        return c * vpow<T>(base, T(1.0 / 4.0));
+    }
+    return T(0);
+}
+
+template <class T> struct Luts {
+    T m4[301], pg[301], vl[301], st[301], ad[301];
+    T factor;  // static float factor = 1.0 / globalSupportRadius (e.g. :122)
+    // initLookupTables (:303-320)
+    void build(T sr) {
+        for (int i = 0; i < 300; ++i) {
+            T dist = sr * i / 300;
+            m4[i] = kernel_m4<T>(dist, sr);
+            pg[i] = kernel_pressure_grad<T>(dist, sr);
+            vl[i] = kernel_visc_lap<T>(dist, sr);
+            st[i] = kernel_tension<T>(dist, sr);
+            ad[i] = kernel_adhesion<T>(dist, sr);
+        }
+        m4[300] = pg[300] = vl[300] = st[300] = ad[300] = 0;
+        factor = (T)(1.0 / (double)sr);
+    }
+    // kernel*Lut(float dist): index = dist * factor * kLutSize (:122-127)
+    int idx(T dist) const { return (int)(dist * factor * T(300)); }
+    T w(T dist) const { int i = idx(dist); return i >= 300 ? T(0) : m4[i]; }
+    T grad(T dist) const { int i = idx(dist); return i >= 300 ? T(0) : pg[i]; }
+    T visc(T dist) const { int i = idx(dist); return i >= 300 ? T(0) : vl[i]; }
+};
+
+// ------------------------------------------------------------ particles
+// Fields of UnifiedParticle (src/cpu/UnifiedParticle.h:94-176) used by the step.
+template <class T> struct Particle {
+    V3<T> pos, vel, force, pos_pred, vel_pred, fp, sum_grad_w, rel0;
+    T density, pressure, corr_p, prev_corr_p, rho_pred, rho_err, psi, mass, sum_grad_w_dot;
+    int type, body, id;
+    uint32_t key;
+};
+
+struct Body {  // RigidBody (src/cpu/RigidBody.h) state, float like the reference
+    float pos[3], vel[3], ang_vel[3], lin_mom[3], ang_mom[3], q[4];
+    float R[9], Iinv_local[9], Iinv_world[9];
+    float force[3], torque[3];
+    float mass;
+    std::vector<int> idx;  // rigid_particle_indices_
+};
+
+template <class T> class Sim {
+  public:
+    upf_params P;
+    Luts<T> L;
+    T kernel_self;
+    std::vector<Particle<T>> ps;
+    std::vector<std::vector<int>> nbrs;
+    std::vector<Body> bodies;
+    std::vector<int> K_hist;
+    std::vector<float> maxrho_hist;
+    int err = UPF_OK;
+    T max_pred_rho = 0;
+    bool too_large = true;
+
+    void init(const upf_params& p) {
+        P = p;
+        L.build((T)P.support_radius);
+        kernel_self = kernel_m4<T>(T(0), (T)P.support_radius);
+    }
+
+    T h() const { return (T)P.support_radius; }
+
+    // CalcIndex(scales * position) (UnifiedPhysics.cpp:3967)
+    uint32_t key_of(const V3<T>& pos) {
+        unsigned c[3];
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) ok &= to_grid<T>((T)P.scales[a] * pos[a], c[a]);
+        if (!ok) err = UPF_ERR_RANGE;
+        return ztab().calc(c[0], c[1], c[2]);
+    }
+
+    // SortParticles (UnifiedPhysics.cpp:353-358), stable
+    void sort_particles() {
+        std::stable_sort(ps.begin(), ps.end(),
+                         [](const Particle<T>& a, const Particle<T>& b) { return a.key < b.key; });
+    }
+
+    // UnifiedPhysics::GetNeighbors (UnifiedPhysics.cpp:2242-2295)
+    void get_neighbors(int i, std::vector<std::pair<uint32_t, uint32_t>>& ranges) {
+        const Particle<T>& p = ps[i];
+        const T hh = h();
+        unsigned mn[3], mx[3];
+        for (int a = 0; a < 3; ++a) {
+            T lo = (T)P.scales[a] * (p.pos[a] - hh);
+            T hi = (T)P.scales[a] * (p.pos[a] + hh);
+            lo = MAXr<T>(lo, T(0));
+            hi = MINr<T>(hi, T(1024));
+            mn[a] = (unsigned)(unsigned short)lo;
+            mx[a] = (unsigned)(unsigned short)hi;
+        }
+        box_query(P.block_size, mn, mx, ranges);
+        std::vector<int>& out = nbrs[i];
+        out.clear();
+        const int n = (int)ps.size();
+        for (size_t k = 0; k < ranges.size(); ++k) {
+            // lower_bound over the key-sorted particles
+            int lo = 0, hi = n;
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (ps[mid].key < ranges[k].first) lo = mid + 1; else hi = mid;
+            }
+            for (int j = lo; j < n && ps[j].key <= ranges[k].second; ++j) {
+                if (j == i) continue;
+                T dist = dist3<T>(p.pos, ps[j].pos);
+                if (dist < hh) out.push_back(j);
+            }
+        }
+        if ((int)out.size() > P.max_neighbors && p.type != UPF_FROZEN) err = UPF_ERR_NBR_OVERFLOW;
+    }
+
+    void neighbor_search() {  // NeighborSearchOMP (:4467-4478)
+        nbrs.resize(ps.size());
+        const int n = (int)ps.size();
+#pragma omp parallel
+        {
+            std::vector<std::pair<uint32_t, uint32_t>> ranges;
+#pragma omp for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) get_neighbors(i, ranges);
+        }
+    }
+
+    // CalculateWeightedVolume (:2703-2746).  Akinci mode adds the fallback of
+    // the GPU path (V0 when no boundary neighbour) instead of aborting.
+    void weighted_volume(int i) {
+        Particle<T>& p = ps[i];
+        if (!(p.type == UPF_FROZEN || p.type == UPF_RIGID)) return;
+        T inv = 0;
+        const std::vector<int>& nb = nbrs[i];
+        for (size_t k = 0; k < nb.size(); ++k) {
+            const Particle<T>& q = ps[nb[k]];
+            T dist = dist3<T>(p.pos, q.pos);
+            if (dist < h()) {
+                T w = L.w(dist);
+                if (q.type == UPF_FROZEN || q.type == UPF_RIGID) inv += w;
+            }
+        }
+        if (inv != 0) p.psi = (T)(1.0 / (double)inv);
+        else err = UPF_ERR_ISOLATED;
+    }
+
+    // CalculateBoundaryFluidPressureForceHost (:4240-4250) and its GPU twin
+    // (src/gpu/particles_kernel.cuh:791-802) for the Akinci mode.
+    V3<T> boundary_pressure(uint32_t dist_lut, const V3<T>& ppos, const V3<T>& npos, T rho,
+                            T corr_p, T wvol) {
+        T p_inv_den = T(1) / rho;
+        T g;
+        if (P.boundary_pressure_mode == UPF_BOUNDARY_CPU_REF) g = L.grad((T)dist_lut);  // int index passed as distance
+        else g = L.pg[dist_lut];
+        V3<T> kg = (ppos - npos) * g;
+        T press_grad = corr_p * p_inv_den * p_inv_den;
+        return kg * press_grad * (T)P.initial_mass * (T)P.rest_density * wvol * T(-1);
+    }
+
+    uint32_t lut_bin(T dist) {  // dist * lut_size / h, clamped (:5212-5214)
+        uint32_t d = (uint32_t)(dist * T(300) / h());
+        return d > 300 ? 300 : d;
+    }
+
+    // CalculateExternalForcesStaticBoundariesPCISPH (:5198-5250)
+    void ext_forces_static(int i) {
+        Particle<T>& p = ps[i];
+        if (p.type == UPF_LIQUID) {
+            const std::vector<int>& nb = nbrs[i];
+            p.force = V3<T>();
+            const T restVolume = (T)P.initial_mass / (T)P.rest_density;
+            for (size_t k = 0; k < nb.size(); ++k) {
+                const Particle<T>& q = ps[nb[k]];
+                T dist = dist3<T>(p.pos, q.pos);
+                if (dist < h()) {
+                    uint32_t dl = lut_bin(dist);
+                    if (q.type == UPF_LIQUID) {
+                        T kv = L.visc(dist);
+                        V3<T> vij = p.vel - q.vel;
+                        p.force -= vij * restVolume * restVolume * (T)P.visc_const * kv;
+                    } else if (q.type == UPF_FROZEN) {
+                        p.force += boundary_pressure(dl, p.pos, q.pos, p.rho_pred, own_pressure(p), boundary_volume(p, q));
+                    }
+                }
+            }
+            p.force.y -= (T)P.initial_mass * (T)P.gravity;
+        }
+    }
+
+    // The reference resets the correction pressure at the end of each
+    // particle's external-force evaluation (:5246-5249, :5361-5364); neighbours
+    // read in the same parallel loop may or may not have been reset yet.  The
+    // restatement evaluates all forces first and resets afterwards.
+    void reset_pressure(int i) {
+        Particle<T>& p = ps[i];
+        p.prev_corr_p = p.corr_p;
+        p.corr_p = 0;
+        p.fp = V3<T>();
+    }
+
+    // Pressure of the liquid particle used by the boundary pressure force: the
+    // CPU path passes previous_correction_pressure_ before it is refreshed
+    // (the value of two steps back); the Akinci mode uses the last loop's value.
+    T own_pressure(const Particle<T>& p) const {
+        return P.boundary_pressure_mode == UPF_BOUNDARY_CPU_REF ? p.prev_corr_p : p.corr_p;
+    }
+    // Volume in the boundary pressure force: the CPU and GPU paths pass the
+    // liquid particle's own weighted_volume_ (m / rho0); Akinci uses Psi_b.
+    T boundary_volume(const Particle<T>& liquid, const Particle<T>& boundary) const {
+        return P.boundary_pressure_mode == UPF_BOUNDARY_CPU_REF ? liquid.psi : boundary.psi;
+    }
+
+    // CalculateBoundaryForcePerLiquidParticleHost (:2032-2069)
+    V3<T> box_boundary_force(const V3<T>& x) {
+        V3<T> f;
+        const T inv = T(1) / (T)P.force_distance;
+        const T fmax = (T)P.max_boundary_force;
+        const T fd = (T)P.force_distance;
+        for (int a = 0; a < 3; ++a) {
+            const T mn = (T)P.collision_min[a], mx = (T)P.collision_max[a];
+            if (x[a] < mn + fd) {
+                V3<T> e; e[a] = T(1);
+                f += e * (T)((mn + fd - x[a]) * inv * 2.0 * fmax);
+            }
+            if (x[a] > mx - fd) {
+                V3<T> e; e[a] = T(-1);
+                f += e * (T)((x[a] + fd - mx) * inv * 2.0 * fmax);
+            }
+        }
+        return f;
+    }
+
+    // CalculateBoundaryForcePerRigidParticleHost (:2071-2132)
+    V3<T> rigid_wall_force(const V3<T>& x, const V3<T>& v) {
+        V3<T> f;
+        bool hit = false;
+        const T r = (T)P.particle_radius, k = (T)P.spring_coefficient_boundary, c = (T)P.damping_coefficient;
+        // order of the reference: x-, x+, y-, y+, z-, z+
+        for (int a = 0; a < 3; ++a) {
+            const T mn = (T)P.collision_min[a], mx = (T)P.collision_max[a];
+            if (x[a] - r < mn) { hit = true; V3<T> e; e[a] = T(1); f += e * (mn + r - x[a]) * k; }
+            if (x[a] + r > mx) { hit = true; V3<T> e; e[a] = T(-1); f += e * (x[a] + r - mx) * k; }
+        }
+        if (hit) { f.x -= c * v.x; f.y -= c * v.y; f.z -= c * v.z; }
+        return f;
+    }
+
+    // CalculateExternalForcesVersatilCouplingPCISPH (:5252-5365)
+    void ext_forces_coupling(int i) {
+        Particle<T>& p = ps[i];
+        if (p.type == UPF_LIQUID || p.type == UPF_RIGID) {
+            const std::vector<int>& nb = nbrs[i];
+            p.force = V3<T>();
+            const T restVolume = (T)P.initial_mass / (T)P.rest_density;
+            const T m = (T)P.initial_mass;
+            for (size_t k = 0; k < nb.size(); ++k) {
+                const Particle<T>& q = ps[nb[k]];
+                T dist = dist3<T>(p.pos, q.pos);
+                if (!(dist < h())) continue;
+                uint32_t dl = lut_bin(dist);
+                if (p.type == UPF_LIQUID) {
+                    if (q.type == UPF_LIQUID) {
+                        T kv = L.visc(dist);
+                        V3<T> vij = p.vel - q.vel;
+                        p.force -= vij * restVolume * restVolume * (T)P.visc_const * kv;
+                        if (dist >= h() * T(0.25)) {  // CalculateSurfaceTensionForcePCISPHHost (:1990-1998)
+                            T sc = L.st[dl];
+                            p.force -= (p.pos - q.pos) * (T)P.surface_tension_gamma * m * m * sc * (T(1) / dist);
+                        }
+                    } else if (q.type == UPF_FROZEN || q.type == UPF_RIGID) {
+                        p.force += boundary_pressure(dl, p.pos, q.pos, p.rho_pred, own_pressure(p), boundary_volume(p, q));
+                        if (dist >= h() * T(0.25)) {  // CalculateSurfaceCohesionForcePCISPHHost (:2000-2008)
+                            T sc = L.ad[dl];
+                            p.force -= (p.pos - q.pos) * (T)P.surface_adhesion_beta * m * (T)P.rest_density * q.psi * sc * (T(1) / dist);
+                        }
+                    }
+                } else {  // rigid
+                    if (q.type == UPF_RIGID) {
+                        const T overlap = T(2) * (T)P.particle_radius - dist;
+                        if (overlap > T(0.00001)) {
+                            V3<T> rij = q.pos - p.pos;  // CalculateSpringForceHost (:2010-2019)
+                            V3<T> fs(T(-1) * (T)P.spring_coefficient * overlap * rij.x / dist,
+                                     T(-1) * (T)P.spring_coefficient * overlap * rij.y / dist,
+                                     T(-1) * (T)P.spring_coefficient * overlap * rij.z / dist);
+                            V3<T> vij = q.vel - p.vel;  // CalculateDampingForceHost (:2021-2030)
+                            V3<T> fd((T)P.damping_coefficient * vij.x, (T)P.damping_coefficient * vij.y,
+                                     (T)P.damping_coefficient * vij.z);
+                            p.force += fs + fd;
+                        }
+                    } else if (q.type == UPF_LIQUID) {
+                        // reaction (:5316-5321); the CPU path reads the liquid's
+                        // correction_pressure_ of the previous loop
+                        p.force -= boundary_pressure(dl, q.pos, p.pos, q.rho_pred, q.corr_p, boundary_volume(q, p));
+                        if (dist >= h() * T(0.25)) {
+                            T sc = L.ad[dl];
+                            p.force += (q.pos - p.pos) * (T)P.surface_adhesion_beta * m * (T)P.rest_density * q.psi * sc * (T(1) / dist);
+                        }
+                    }
+                }
+            }
+            if (p.type == UPF_LIQUID) {
+                p.force.y -= (T)P.initial_mass * (T)P.gravity;
+                p.force += box_boundary_force(p.pos);
+            } else {
+                p.force += rigid_wall_force(p.pos, p.vel);
+            }
+        }
+    }
+
+    // PredictPositionAndVelocity + CollisionHandling (:5426-5454, :5739-5757)
+    void predict(int i) {
+        Particle<T>& p = ps[i];
+        if (p.type != UPF_LIQUID) return;
+        p.vel_pred = p.vel + ((p.force + p.fp) * (T)P.dt).div((T)P.initial_mass);
+        p.pos_pred = p.pos + p.vel_pred * (T)P.dt;
+        for (int a = 0; a < 3; ++a) {
+            if (p.pos_pred[a] < (T)P.collision_min[a]) { p.pos_pred[a] = (T)P.collision_min[a]; p.vel_pred[a] *= T(0); }
+            if (p.pos_pred[a] > (T)P.collision_max[a]) { p.pos_pred[a] = (T)P.collision_max[a]; p.vel_pred[a] *= T(0); }
+        }
+        p.rho_pred = 0;
+        if (!(p.pos_pred.x == p.pos_pred.x) || !(p.pos_pred.y == p.pos_pred.y) || !(p.pos_pred.z == p.pos_pred.z))
+            err = UPF_ERR_NAN;
+    }
+
+    // Mass a neighbour contributes to density and pressure sums.  The CPU path
+    // uses the fluid mass for every neighbour type; the Akinci mode uses the
+    // boundary "mass" rho0 * Psi_b of Akinci et al. 2012 (eq. 5) for boundary
+    // (frozen / rigid) neighbours.
+    T mass_of(const Particle<T>& q) const {
+        if (P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI && q.type != UPF_LIQUID) return (T)P.rest_density * q.psi;
+        return (T)P.initial_mass;
+    }
+
+    // ComputePredictedDensityAndPressure (:5456-5491); returns rho* (liquid) or 0
+    T pred_density(int i) {
+        Particle<T>& p = ps[i];
+        if (p.type != UPF_LIQUID) return T(0);
+        const std::vector<int>& nb = nbrs[i];
+        const T m = (T)P.initial_mass;
+        p.rho_pred = kernel_self * m;
+        for (size_t k = 0; k < nb.size(); ++k) {
+            const Particle<T>& q = ps[nb[k]];
+            T dist = dist3<T>(p.pos_pred, q.pos_pred);
+            if (dist < h()) {
+                T w = L.w(dist);
+                p.rho_pred += w * mass_of(q);
+            }
+        }
+        p.rho_err = MAXr<T>(p.rho_pred - (T)P.rest_density, T(0));
+        p.corr_p += MAXr<T>(p.rho_err * (T)P.density_error_factor, T(0));
+        return p.rho_pred;
+    }
+
+    // ComputeCorrectivePressureForce (:5493-5522)
+    void corr_force(int i) {
+        Particle<T>& p = ps[i];
+        if (p.type != UPF_LIQUID) return;
+        const std::vector<int>& nb = nbrs[i];
+        p.fp = V3<T>();
+        const T rho0 = (T)P.rest_density;
+        const T densSq = rho0 * rho0;
+        const T pi = p.corr_p;
+        const T m = (T)P.initial_mass;
+        for (size_t k = 0; k < nb.size(); ++k) {
+            const Particle<T>& q = ps[nb[k]];
+            T dist = dist3<T>(p.pos, q.pos);
+            if (dist < h()) {
+                T pj = q.corr_p;
+                T g = L.grad(dist);
+                V3<T> kg = (p.pos - q.pos) * g;
+                T grad = pi / densSq + pj / (rho0 * rho0);
+                p.fp -= kg * grad * m * mass_of(q);
+            }
+        }
+    }
+
+    // PredictionCorrectionStep (:5367-5424)
+    void pcisph_loop() {
+        const int n = (int)ps.size();
+        too_large = true;
+        int it = 0;
+        float last_max = 0;
+        while (it < P.min_loops || (too_large && it < P.max_loops)) {
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) predict(i);
+            T mx = 0;
+#pragma omp parallel for schedule(dynamic, 100) reduction(max : mx)
+            for (int i = 0; i < n; ++i) { T r = pred_density(i); mx = MAXr<T>(mx, r); }
+            max_pred_rho = mx;
+            last_max = (float)mx;
+            T pct = MAXr<T>(T(0.1) * max_pred_rho - T(100), T(0));
+            if (pct < (T)P.max_density_error_allowed) too_large = false;
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) corr_force(i);
+            ++it;
+            if (err == UPF_ERR_NAN) break;
+        }
+        K_hist.push_back(it);
+        maxrho_hist.push_back(last_max);
+    }
+
+    // UpdateLiquidParticlePosVelSphCPU (:3705-3742)
+    void integrate(int i, bool pcisph) {
+        Particle<T>& p = ps[i];
+        if (p.type != UPF_LIQUID) return;
+        if (pcisph) p.force += p.fp;
+        const T invMass = (T)(1.0 / (double)P.initial_mass);
+        const T dt = (T)P.dt;
+        p.vel += p.force * invMass * dt;
+        p.pos += p.vel * dt;
+    }
+
+    // BoundaryHandlingBoxPerNonRigidParticle (:4000-4044)
+    void box_clamp(Particle<T>& p) {
+        const T damping = T(-0.5);
+        const int order[3] = {1, 0, 2};
+        for (int o = 0; o < 3; ++o) {
+            int a = order[o];
+            if (p.pos[a] < (T)P.collision_min[a]) { p.pos[a] = (T)P.collision_min[a]; p.vel[a] *= damping; }
+            if (p.pos[a] > (T)P.collision_max[a]) { p.pos[a] = (T)P.collision_max[a]; p.vel[a] *= damping; }
+        }
+    }
+
+    // BoundaryHandlingBoxPerParticleOMP (:3940-3970) / BoundaryHandlingPureFluidOMP (:3972-3998)
+    void boundary_handling(bool pure_fluid) {
+        const int n = (int)ps.size();
+#pragma omp parallel for schedule(dynamic, 100)
+        for (int i = 0; i < n; ++i) {
+            Particle<T>& p = ps[i];
+            bool clamp = pure_fluid ? (p.type == UPF_LIQUID) : (p.type != UPF_FROZEN && p.body < 0);
+            if (clamp) box_clamp(p);
+            p.key = key_of(p.pos);
+        }
+    }
+
+    // CalculateDensityPressure (:2339-2406) with physicsType 'w', wall weight off
+    void density_pressure_wcsph(int i) {
+        Particle<T>& p = ps[i];
+        if (p.type != UPF_LIQUID) return;
+        const std::vector<int>& nb = nbrs[i];
+        const T m = (T)P.initial_mass;
+        p.density = kernel_self * m;
+        for (size_t k = 0; k < nb.size(); ++k) {
+            T dist = dist3<T>(p.pos, ps[nb[k]].pos);
+            if (dist < h()) { T w = L.w(dist); p.density += w * m; }
+        }
+        // CalculatePressureWCSPH (:2491-2501)
+        const T gamma = T(7.0);
+        const T b_i = (T)P.rest_density * (T)P.gas_const_wcsph / gamma;
+        const T frac = p.density / (T)P.rest_density;
+        const T pg = vpow<T>(frac, gamma);
+        p.pressure = MAXr<T>(T(0), b_i * (pg - T(1)));
+    }
+
+    // CalculateForcesPerParticleStep1 -> CalculateLiquidParticleForces (:2809-2834, :3009-3083)
+    void forces_wcsph(int i) {
+        Particle<T>& p = ps[i];
+        if (p.type != UPF_LIQUID) return;
+        const std::vector<int>& nb = nbrs[i];
+        const T m = (T)P.initial_mass;
+        const T invDensity = (T)(1.0 / (double)p.density);
+        const T pVol = m * invDensity;
+        V3<T> force;
+        for (size_t k = 0; k < nb.size(); ++k) {
+            const Particle<T>& q = ps[nb[k]];
+            T dist = dist3<T>(p.pos, q.pos);
+            if (dist < h()) {
+                T kv = L.visc(dist);
+                T nVol = m / q.density;
+                const V3<T> dv = p.pos - q.pos;
+                if (q.type == UPF_LIQUID) {
+                    V3<T> kg = dv * L.grad(dist);
+                    T grad = p.pressure * invDensity * invDensity + q.pressure / (q.density * q.density);
+                    V3<T> tp = kg * grad * m * m;
+                    force -= tp;
+                    V3<T> vij = p.vel - q.vel;
+                    force -= vij * pVol * nVol * (T)P.visc_const * kv;
+                }
+            }
+        }
+        if (P.add_boundary_force) force += box_boundary_force(p.pos);
+        p.force = force;
+        p.force.y -= (T)P.initial_mass * (T)P.gravity;
+    }
+
+    // ---- rigid bodies (RigidBody.cpp, UnifiedPhysics.cpp:3620-3703,3744-3792)
+    static void matmul(const float* A, const float* B, float* C) {  // Matrix3x3::operator*
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c)
+                C[r * 3 + c] = A[r * 3 + 0] * B[0 * 3 + c] + A[r * 3 + 1] * B[1 * 3 + c] + A[r * 3 + 2] * B[2 * 3 + c];
+    }
+    static void matvec(const float* A, const float* v, float* o) {
+        for (int r = 0; r < 3; ++r) o[r] = A[r * 3 + 0] * v[0] + A[r * 3 + 1] * v[1] + A[r * 3 + 2] * v[2];
+    }
+    void rigid_integrate(Body& b) {
+        const float dt = P.dt;
+        for (size_t j = 0; j < b.idx.size(); ++j) {
+            const Particle<T>& p = ps[b.idx[j]];
+            float f[3] = {(float)p.force.x, (float)p.force.y, (float)p.force.z};
+            for (int a = 0; a < 3; ++a) b.force[a] = b.force[a] + f[a];           // ApplyForce
+            float r[3] = {(float)p.pos.x - b.pos[0], (float)p.pos.y - b.pos[1], (float)p.pos.z - b.pos[2]};
+            float c[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2], r[0] * f[1] - r[1] * f[0]};
+            for (int a = 0; a < 3; ++a) b.torque[a] = b.torque[a] + c[a];         // ApplyTorque (RigidBody.cpp:16-19)
+            float g[3] = {0.0f, -(float)p.mass * P.gravity, 0.0f};
+            for (int a = 0; a < 3; ++a) b.force[a] = b.force[a] + g[a];
+        }
+        const float terminal = b.mass * P.terminal_speed;
+        // UpdateMomenta (RigidBody.cpp:146-160)
+        for (int a = 0; a < 3; ++a) b.lin_mom[a] = b.lin_mom[a] + b.force[a] * dt;
+        for (int a = 0; a < 3; ++a) {
+            if (b.lin_mom[a] > 0.0f) b.lin_mom[a] = MINr<float>(b.lin_mom[a], terminal);
+            else b.lin_mom[a] = MAXr<float>(b.lin_mom[a], -terminal);
+        }
+        for (int a = 0; a < 3; ++a) b.ang_mom[a] = b.ang_mom[a] + b.torque[a] * dt;
+        // PerformStep (:138-143): UpdateInverseInertiaTensor, linear, angular
+        float Rt[9], tmp[9];
+        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) Rt[r * 3 + c] = b.R[c * 3 + r];
+        matmul(b.R, b.Iinv_local, tmp);
+        matmul(tmp, Rt, b.Iinv_world);
+        {
+            float inv = 1.0f / b.mass;
+            for (int a = 0; a < 3; ++a) b.vel[a] = b.lin_mom[a] * inv;
+            for (int a = 0; a < 3; ++a) b.pos[a] = b.pos[a] + b.vel[a] * dt;
+        }
+        matvec(b.Iinv_world, b.ang_mom, b.ang_vel);
+        float sp = sqrtf(b.ang_vel[0] * b.ang_vel[0] + b.ang_vel[1] * b.ang_vel[1] + b.ang_vel[2] * b.ang_vel[2]);
+        if (sp > 0.0f) {
+            float l = sp;
+            float ll = (float)(1.0 / (double)l);  // Vector3::normalize (vector3.h:516-528)
+            float ax[3] = {b.ang_vel[0] * ll, b.ang_vel[1] * ll, b.ang_vel[2] * ll};
+            float ang = sp * dt;
+            float ds = cosf(ang / 2.0f);
+            float sn = sinf(ang / 2.0f);
+            float dvx = ax[0] * sn, dvy = ax[1] * sn, dvz = ax[2] * sn;
+            float s = b.q[0], vx = b.q[1], vy = b.q[2], vz = b.q[3];
+            b.q[0] = s * ds - vx * dvx - vy * dvy - vz * dvz;
+            b.q[1] = ds * vx + s * dvx + dvy * vz - dvz * vy;
+            b.q[2] = ds * vy + s * dvy + dvz * vx - dvx * vz;
+            b.q[3] = ds * vz + s * dvz + dvx * vy - dvy * vx;
+        }
+        // UpdateRotationMatrix (:92-128); the CPU drops getNormalized()'s result
+        if (P.renormalize_quaternion) {
+            float l = sqrtf(b.q[0] * b.q[0] + b.q[1] * b.q[1] + b.q[2] * b.q[2] + b.q[3] * b.q[3]);
+            if (l > 0.0f) { float il = 1.0f / l; for (int a = 0; a < 4; ++a) b.q[a] *= il; }
+        }
+        float w = b.q[0], x = b.q[1], y = b.q[2], z = b.q[3];
+        float xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z, wx = w * x, wy = w * y, wz = w * z;
+        b.R[0] = 1.0f - 2.0f * (yy + zz); b.R[1] = 2.0f * (xy - wz); b.R[2] = 2.0f * (xz + wy);
+        b.R[3] = 2.0f * (xy + wz); b.R[4] = 1.0f - 2.0f * (xx + zz); b.R[5] = 2.0f * (yz - wx);
+        b.R[6] = 2.0f * (xz - wy); b.R[7] = 2.0f * (yz + wx); b.R[8] = 1.0f - 2.0f * (xx + yy);
+    }
+
+    // UpdateRigidBodyParticleInformationOMP (:360-402): index lists in particle order
+    void rebuild_body_indices() {
+        for (auto& b : bodies) b.idx.clear();
+        for (int j = 0; j < (int)ps.size(); ++j)
+            if (ps[j].body >= 0) bodies[ps[j].body].idx.push_back(j);
+    }
+
+    // SynRigidParticlesPerRigidBody (:3744-3792)
+    void sync_rigid(Body& b) {
+        for (size_t j = 0; j < b.idx.size(); ++j) {
+            Particle<T>& p = ps[b.idx[j]];
+            float r0[3] = {(float)p.rel0.x, (float)p.rel0.y, (float)p.rel0.z}, r[3];
+            matvec(b.R, r0, r);
+            float w[3] = {b.ang_vel[0], b.ang_vel[1], b.ang_vel[2]};
+            float c[3] = {w[1] * r[2] - w[2] * r[1], w[2] * r[0] - w[0] * r[2], w[0] * r[1] - w[1] * r[0]};
+            p.vel = V3<T>((T)(b.vel[0] + c[0]), (T)(b.vel[1] + c[1]), (T)(b.vel[2] + c[2]));
+            p.pos = V3<T>((T)(b.pos[0] + r[0]), (T)(b.pos[1] + r[1]), (T)(b.pos[2] + r[2]));
+            if (P.rigid_predicted_current) p.pos_pred = p.pos;
+        }
+    }
+
+    // ---------------------------------------------------------- drivers
+    void step(int method) {
+        const int n = (int)ps.size();
+        if (method == UPF_PCISPH_STATIC) {  // CpuParticlePhysicsStaticBoundariesPCISPH (:4220-4238)
+            neighbor_search();
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) weighted_volume(i);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) ext_forces_static(i);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) reset_pressure(i);
+            pcisph_loop();
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) integrate(i, true);
+            boundary_handling(false);
+            sort_particles();
+        } else if (method == UPF_WCSPH_STATIC) {  // CpuParticlePhysicsStaticBoundariesSPH (:4180-4202)
+            neighbor_search();
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) density_pressure_wcsph(i);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) forces_wcsph(i);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) integrate(i, false);
+            boundary_handling(false);
+            sort_particles();
+        } else {  // CpuParticlePhysicsVersatileCouplingPCISPH (:4991-5024)
+            neighbor_search();
+            if (P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI) {
+#pragma omp parallel for schedule(dynamic, 100)
+                for (int i = 0; i < n; ++i) weighted_volume(i);
+            }
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) ext_forces_coupling(i);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) reset_pressure(i);
+            pcisph_loop();
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) integrate(i, true);
+            boundary_handling(true);
+            for (auto& b : bodies) {
+                for (int a = 0; a < 3; ++a) { b.force[a] = 0.0f; b.torque[a] = 0.0f; }
+                rigid_integrate(b);
+            }
+            sort_particles();
+            rebuild_body_indices();
+            for (auto& b : bodies) sync_rigid(b);
+        }
+    }
+};
+
+// ComputeDensityErrorFactor (:5525-5689) with CreatePreParticlesISPH (:5577-5606),
+// CreatePartilcesCPU (:2134-2205), ComputeGradWValues (:5546-5575), ComputeFactor (:5638-5689)
+static int density_error_factor(upf_params* P, int* maxn_out) {
+    const float spacing = P->spacing;
+    if (spacing == 0.0125f) return UPF_ERR_ARG;  // reference reads uninitialised bounds here
+    float scale = (float)((double)spacing / 0.0125);
+    V3<float> mn = V3<float>(-0.1f, 0.0f, -0.1f) * scale;
+    V3<float> mx = V3<float>(0.1f, 0.2f, 0.1f) * scale;
+    V3<float> delta = mx - mn;
+    int num[3];
+    for (int a = 0; a < 3; ++a) num[a] = (int)(delta[a] / spacing);
+    Sim<float> s;
+    upf_params Q = *P;
+    s.init(Q);
+    const int cnt = num[0] * num[1] * num[2];
+    s.ps.resize(cnt);
+    int c = 0;
+    for (int iy = 0; iy < num[1]; ++iy) {
+        float y = 0.0f + iy * spacing;
+        for (int ix = 0; ix < num[0]; ++ix) {
+            float x = 0.0f + ix * spacing;
+            for (int iz = 0; iz < num[2]; ++iz) {
+                float z = 0.0f + iz * spacing;
+                Particle<float>& p = s.ps[c];
+                p = Particle<float>();
+                p.pos = V3<float>(x, y, z);
+                p.type = UPF_LIQUID;
+                p.key = s.key_of(p.pos);
+                p.id = c;
+                ++c;
+            }
+        }
+    }
+    s.sort_particles();
+    s.neighbor_search();
+    const int n = (int)s.ps.size();
+    const float hh = P->support_radius;
+    for (int i = 0; i < n; ++i) {
+        Particle<float>& p = s.ps[i];
+        for (size_t k = 0; k < s.nbrs[i].size(); ++k) {
+            Particle<float>& q = s.ps[s.nbrs[i][k]];
+            float dist = dist3<float>(p.pos, q.pos);
+            if (dist < hh) {
+                V3<float> g = (p.pos - q.pos) * kernel_pressure_grad<float>(dist, hh);
+                p.sum_grad_w += g;
+                q.sum_grad_w -= g;
+                p.sum_grad_w_dot += g.dot(g);
+                V3<float> ng = g * -1.0f;
+                q.sum_grad_w_dot += ng.dot(ng);
+            }
+        }
+    }
+    int best = 0, maxn = 0;
+    for (int i = 0; i < n; ++i) {
+        int cntn = 0;
+        for (size_t k = 0; k < s.nbrs[i].size(); ++k)
+            if (dist3<float>(s.ps[i].pos, s.ps[s.nbrs[i][k]].pos) < hh) ++cntn;
+        if (cntn > maxn) { maxn = cntn; best = i; }
+    }
+    const Particle<float>& p = s.ps[best];
+    // densityErrorFactorParameter with deltaT_pcisph = dt (UnifiedConstants.cpp:102-103)
+    const float factor = P->dt / 0.0002f;
+    P->density_error_factor_parameter = (float)(0.05 * factor * factor);
+    float restVol = P->initial_mass / P->rest_density;
+    float preFactor = restVol * restVol * P->dt * P->dt;
+    float gradWTerm = ((p.sum_grad_w * -1.0f).dot(p.sum_grad_w) - p.sum_grad_w_dot);
+    float divisor = preFactor * gradWTerm;
+    if (maxn_out) *maxn_out = maxn;
+    if (divisor == 0) return UPF_ERR_DELTA;
+    float f = (float)(-1.0 / (double)divisor);
+    f *= P->density_error_factor_parameter;
+    P->density_error_factor = f;
+    return s.err;
+}
+
+static int set_domain(upf_params* p, const float vmax[3], const float cmin[3], const float cmax[3]) {
+    for (int a = 0; a < 3; ++a) {
+        p->vbox_min[a] = 0.0f;
+        p->vbox_max[a] = vmax[a];
+        p->scales[a] = 1024.0f / (vmax[a] - 0.0f);  // Vector3f(GRID_RESOLUTION) / delta (:315-316)
+        p->collision_min[a] = cmin[a];
+        p->collision_max[a] = cmax[a];
+    }
+    // Scene::build (Scene.cpp:29-34) -> ZIndex::SetBlockSize(int) (zIndex.cpp:27-40)
+    float bsz = p->scales[0] * p->support_radius;
+    bsz = MAXr<float>(bsz, p->scales[1] * p->support_radius);
+    bsz = MAXr<float>(bsz, p->scales[2] * p->support_radius);
+    int size = (int)bsz;
+    unsigned b;
+    for (b = 1; (int)b < size; b *= 2) {}
+    p->block_size = b;
+    unsigned sh = 0;
+    while ((1u << sh) < b) ++sh;
+    p->block_shift = sh;
+    p->grid_dim = 1024u / b;
+    return (float)b >= bsz ? UPF_OK : UPF_ERR_ARG;
+}
+
+}  // namespace orc
+
+// =================================================================== C ABI
+extern "C" {
+
+int orc_params_init(float spacing, upf_params* out) { orc::params_init(spacing, out); return 0; }
+int orc_params_set_domain(upf_params* p, const float* vmax, const float* cmin, const float* cmax) {
+    return orc::set_domain(p, vmax, cmin, cmax);
+}
+int orc_density_error_factor(upf_params* p, int* maxn) { return orc::density_error_factor(p, maxn); }
+
+int orc_luts(float h, float* out, float* kernel_self) {  // 5*301 floats
+    orc::Luts<float> L;
+    L.build(h);
+    for (int i = 0; i < 301; ++i) {
+        out[i] = L.m4[i]; out[301 + i] = L.pg[i]; out[602 + i] = L.vl[i];
+        out[903 + i] = L.st[i]; out[1204 + i] = L.ad[i];
+    }
+    if (kernel_self) *kernel_self = orc::kernel_m4<float>(0.0f, h);
+    return 0;
+}
+int orc_lut_lookup(float h, int n, const float* dist, int* idx_out, float* w, float* g, float* v) {
+    orc::Luts<float> L;
+    L.build(h);
+    for (int i = 0; i < n; ++i) {
+        idx_out[i] = L.idx(dist[i]);
+        w[i] = L.w(dist[i]); g[i] = L.grad(dist[i]); v[i] = L.visc(dist[i]);
+    }
+    return 0;
+}
+int orc_ztable(uint32_t* out) {
+    for (int k = 0; k < 1024; ++k) for (int a = 0; a < 3; ++a) out[k * 3 + a] = orc::ztab().t[k][a];
+    return 0;
+}
+int orc_keys(const upf_params* p, int n, const float* pos4, uint32_t* keys) {
+    orc::Sim<float> s;
+    s.P = *p;
+    for (int i = 0; i < n; ++i)
+        keys[i] = s.key_of(orc::V3<float>(pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2]));
+    return s.err;
+}
+int orc_box_query(unsigned bs, const unsigned* mn, const unsigned* mx, uint32_t* out, int cap) {
+    std::vector<std::pair<uint32_t, uint32_t>> r;
+    orc::box_query(bs, mn, mx, r);
+    int m = (int)r.size();
+    for (int i = 0; i < m && i < cap; ++i) { out[2 * i] = r[i].first; out[2 * i + 1] = r[i].second; }
+    return m;
+}
+int orc_stable_sort(int n, const uint32_t* keys, uint32_t* keys_out, int32_t* perm) {
+    std::vector<int32_t> idx(n);
+    for (int i = 0; i < n; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+    for (int i = 0; i < n; ++i) { perm[i] = idx[i]; keys_out[i] = keys[idx[i]]; }
+    return 0;
+}
+
+}  // extern "C"
+
+struct orc_sim {
+    int prec;
+    orc::Sim<float> f;
+    orc::Sim<double> d;
+};
+
+template <class T>
+static void load_particles(orc::Sim<T>& s, int n, const float* pos4, const float* vel4, const int* type) {
+    s.ps.resize(n);
+    const T m = (T)s.P.initial_mass;
+    for (int i = 0; i < n; ++i) {
+        orc::Particle<T>& p = s.ps[i];
+        p = orc::Particle<T>();
+        p.pos = orc::V3<T>(pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2]);
+        if (vel4) p.vel = orc::V3<T>(vel4[4 * i], vel4[4 * i + 1], vel4[4 * i + 2]);
+        p.pos_pred = p.pos;   // ctor: predicted_position_ = position_ (UnifiedParticle.cpp:16)
+        p.type = type[i];
+        p.mass = m;
+        p.psi = m / (T)s.P.rest_density;   // weighted_volume_ = mass / rest density (UnifiedParticle.cpp:24)
+        p.rho_pred = (T)s.P.rest_density;
+        p.body = -1;
+        p.id = i;
+        p.key = s.key_of(p.pos);
+    }
+    // Scene::addAllTo sorts before the first step (Scene.cpp:190-191)
+    s.sort_particles();
+}
+
+extern "C" orc_sim* orc_sim_create(const upf_params* p, int precision, int n, const float* pos4, const float* vel4,
+                        const int* type) {
+    orc_sim* s = new orc_sim();
+    s->prec = precision;
+    if (precision == 64) { s->d.init(*p); load_particles(s->d, n, pos4, vel4, type); }
+    else { s->f.init(*p); load_particles(s->f, n, pos4, vel4, type); }
+    return s;
+}
+extern "C" void orc_sim_destroy(orc_sim* s) { delete s; }
+
+template <class T>
+static void load_bodies(orc::Sim<T>& s, int nb, const upf_body* b, const int* body_id, const float* rel4,
+                        const float* pmass) {
+    s.bodies.resize(nb);
+    for (int k = 0; k < nb; ++k) {
+        orc::Body& B = s.bodies[k];
+        std::memset(B.pos, 0, sizeof(float) * 3);
+        for (int a = 0; a < 3; ++a) {
+            B.pos[a] = b[k].pos[a]; B.vel[a] = b[k].vel[a]; B.ang_vel[a] = b[k].angular_velocity[a];
+            B.lin_mom[a] = b[k].linear_momentum[a]; B.ang_mom[a] = b[k].angular_momentum[a];
+            B.force[a] = 0; B.torque[a] = 0;
+        }
+        for (int a = 0; a < 4; ++a) B.q[a] = b[k].quaternion[a];
+        for (int a = 0; a < 9; ++a) { B.R[a] = b[k].rotation[a]; B.Iinv_local[a] = b[k].inv_inertia_local[a]; B.Iinv_world[a] = b[k].inv_inertia_world[a]; }
+        B.mass = b[k].mass;
+    }
+    // particles are sorted already; ids are upload indices
+    for (auto& p : s.ps) {
+        int id = p.id;
+        p.body = body_id[id];
+        p.rel0 = orc::V3<T>(rel4[4 * id], rel4[4 * id + 1], rel4[4 * id + 2]);
+        if (pmass) {
+            p.mass = (T)pmass[id];
+            if (p.type == UPF_RIGID) p.psi = (T)pmass[id] / (T)s.P.rest_density;
+        }
+    }
+    s.rebuild_body_indices();
+}
+
+extern "C" int orc_sim_upload_bodies(orc_sim* s, int nb, const upf_body* b, const int* body_id, const float* rel4,
+                          const float* pmass) {
+    if (s->prec == 64) load_bodies(s->d, nb, b, body_id, rel4, pmass);
+    else load_bodies(s->f, nb, b, body_id, rel4, pmass);
+    return 0;
+}
+
+extern "C" int orc_sim_step(orc_sim* s, int nsteps, int method) {
+    for (int k = 0; k < nsteps; ++k) {
+        if (s->prec == 64) { s->d.step(method); if (s->d.err) return s->d.err; }
+        else { s->f.step(method); if (s->f.err) return s->f.err; }
+    }
+    return 0;
+}
+
+template <class T>
+static void download(orc::Sim<T>& s, float* pos4, float* vel4, float* rho, float* p, float* f4, float* psi,
+                     int* type, uint32_t* keys, int32_t* ids, int pcisph) {
+    const int n = (int)s.ps.size();
+    for (int i = 0; i < n; ++i) {
+        const orc::Particle<T>& q = s.ps[i];
+        const int id = q.id;
+        if (pos4) { pos4[4 * id] = (float)q.pos.x; pos4[4 * id + 1] = (float)q.pos.y; pos4[4 * id + 2] = (float)q.pos.z; pos4[4 * id + 3] = 0; }
+        if (vel4) { vel4[4 * id] = (float)q.vel.x; vel4[4 * id + 1] = (float)q.vel.y; vel4[4 * id + 2] = (float)q.vel.z; vel4[4 * id + 3] = 0; }
+        if (rho) rho[id] = (float)(pcisph ? q.rho_pred : q.density);
+        if (p) p[id] = (float)(pcisph ? q.corr_p : q.pressure);
+        if (f4) { f4[4 * id] = (float)q.force.x; f4[4 * id + 1] = (float)q.force.y; f4[4 * id + 2] = (float)q.force.z; f4[4 * id + 3] = 0; }
+        if (psi) psi[id] = (float)q.psi;
+        if (type) type[id] = q.type;
+        if (keys) keys[i] = q.key;
+        if (ids) ids[i] = id;
+    }
+}
+
+extern "C" int orc_sim_download(orc_sim* s, float* pos4, float* vel4, float* rho, float* p, float* f4, float* psi, int* type,
+                     uint32_t* keys_sorted, int32_t* ids_sorted, int pcisph) {
+    if (s->prec == 64) download(s->d, pos4, vel4, rho, p, f4, psi, type, keys_sorted, ids_sorted, pcisph);
+    else download(s->f, pos4, vel4, rho, p, f4, psi, type, keys_sorted, ids_sorted, pcisph);
+    return 0;
+}
+
+extern "C" int orc_sim_download_bodies(orc_sim* s, upf_body* out) {
+    std::vector<orc::Body>& B = s->prec == 64 ? s->d.bodies : s->f.bodies;
+    for (size_t k = 0; k < B.size(); ++k) {
+        upf_body& o = out[k];
+        std::memset(&o, 0, sizeof(o));
+        for (int a = 0; a < 3; ++a) {
+            o.pos[a] = B[k].pos[a]; o.vel[a] = B[k].vel[a]; o.angular_velocity[a] = B[k].ang_vel[a];
+            o.linear_momentum[a] = B[k].lin_mom[a]; o.angular_momentum[a] = B[k].ang_mom[a];
+            o.force[a] = B[k].force[a]; o.torque[a] = B[k].torque[a];
+        }
+        for (int a = 0; a < 4; ++a) o.quaternion[a] = B[k].q[a];
+        for (int a = 0; a < 9; ++a) { o.rotation[a] = B[k].R[a]; o.inv_inertia_local[a] = B[k].Iinv_local[a]; o.inv_inertia_world[a] = B[k].Iinv_world[a]; }
+        o.mass = B[k].mass;
+    }
+    return (int)B.size();
+}
+
+extern "C" int orc_sim_stats(orc_sim* s, int max_steps, int* K, float* maxrho) {
+    std::vector<int>& kh = s->prec == 64 ? s->d.K_hist : s->f.K_hist;
+    std::vector<float>& mh = s->prec == 64 ? s->d.maxrho_hist : s->f.maxrho_hist;
+    int m = (int)kh.size();
+    for (int i = 0; i < m && i < max_steps; ++i) { K[i] = kh[i]; if (maxrho) maxrho[i] = mh[i]; }
+    return m;
+}
+
+// Neighbour lists of the current (sorted) state at current positions, in
+// sorted-slot indices: counts[n], flat lists (ascending) into out (cap entries).
+extern "C" int orc_sim_neighbors(orc_sim* s, int* counts, int32_t* out, int cap) {
+    orc::Sim<float>& S = s->f;
+    S.neighbor_search();
+    int w = 0;
+    for (size_t i = 0; i < S.ps.size(); ++i) {
+        counts[i] = (int)S.nbrs[i].size();
+        for (size_t k = 0; k < S.nbrs[i].size(); ++k) { if (w < cap) out[w] = S.nbrs[i][k]; ++w; }
+    }
+    return w;
+}
+
+extern "C" int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+

@@ -1,0 +1,147 @@
+"""ctypes bindings of the CPU oracle (oracle/liborc*.so) and of the reference
+translation units compiled into oracle/_ref/libupfref.so.
+
+Test infrastructure only: the oracle is the checker, never the thing measured.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from unifiedparticleframeworkcuda_amd._abi import Params, Body, ptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+_libs = {}
+
+
+def _bind(lib):
+    f = C.c_float
+    pf = C.POINTER(C.c_float)
+    pi = C.POINTER(C.c_int)
+    pu = C.POINTER(C.c_uint32)
+    pp = C.POINTER(Params)
+    lib.orc_params_init.argtypes = [f, pp]
+    lib.orc_params_set_domain.argtypes = [pp, pf, pf, pf]
+    lib.orc_density_error_factor.argtypes = [pp, pi]
+    lib.orc_luts.argtypes = [f, pf, pf]
+    lib.orc_lut_lookup.argtypes = [f, C.c_int, pf, pi, pf, pf, pf]
+    lib.orc_ztable.argtypes = [pu]
+    lib.orc_keys.argtypes = [pp, C.c_int, pf, pu]
+    lib.orc_box_query.argtypes = [C.c_uint, pu, pu, pu, C.c_int]
+    lib.orc_stable_sort.argtypes = [C.c_int, pu, pu, pi]
+    lib.orc_sim_create.restype = C.c_void_p
+    lib.orc_sim_create.argtypes = [pp, C.c_int, C.c_int, pf, pf, pi]
+    lib.orc_sim_destroy.argtypes = [C.c_void_p]
+    lib.orc_sim_upload_bodies.argtypes = [C.c_void_p, C.c_int, C.POINTER(Body), pi, pf, pf]
+    lib.orc_sim_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.orc_sim_download.argtypes = [C.c_void_p, pf, pf, pf, pf, pf, pf, pi, pu, pi, C.c_int]
+    lib.orc_sim_download_bodies.argtypes = [C.c_void_p, C.POINTER(Body)]
+    lib.orc_sim_stats.argtypes = [C.c_void_p, C.c_int, pi, pf]
+    lib.orc_sim_neighbors.argtypes = [C.c_void_p, pi, pi, C.c_int]
+    return lib
+
+
+def oracle(omp=False):
+    """The oracle library; omp=True gives the OpenMP build (same results)."""
+    name = "liborc_omp.so" if omp else "liborc.so"
+    if name not in _libs:
+        path = os.path.join(ORACLE_DIR, name)
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run __graft_entry__.build() (%s missing)" % path)
+        _libs[name] = _bind(C.CDLL(path))
+    return _libs[name]
+
+
+def ref_lib():
+    """oracle/_ref/libupfref.so or None when it was not built (no reference tree)."""
+    path = os.path.join(ORACLE_DIR, "_ref", "libupfref.so")
+    if not os.path.exists(path):
+        return None
+    if "ref" not in _libs:
+        lib = C.CDLL(path)
+        pf = C.POINTER(C.c_float)
+        pu = C.POINTER(C.c_uint32)
+        lib.ref_constants.argtypes = [C.c_float, pf]
+        lib.ref_luts.argtypes = [C.c_float, pf, pf, C.c_int, pf, pf, pf, pf]
+        lib.ref_kernels.argtypes = [C.c_float, C.c_int, pf, pf, pf, pf]
+        lib.ref_ztable.argtypes = [pu]
+        lib.ref_box_query.argtypes = [C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), pu, C.c_int, pu]
+        lib.ref_rigid_step.argtypes = [pf, C.c_int, pf, pf, pf, C.c_float, C.c_float, C.c_float]
+        _libs["ref"] = lib
+    return _libs["ref"]
+
+
+def params(spacing=0.01, vmax=None, cmin=None, cmax=None):
+    p = Params()
+    oracle().orc_params_init(spacing, C.byref(p))
+    if vmax is not None:
+        v = np.asarray(vmax, np.float32)
+        a = np.asarray(cmin, np.float32)
+        b = np.asarray(cmax, np.float32)
+        rc = oracle().orc_params_set_domain(C.byref(p), ptr(v), ptr(a), ptr(b))
+        if rc != 0:
+            raise ValueError("orc_params_set_domain: %d" % rc)
+    return p
+
+
+class OracleSim:
+    """Stateful oracle simulation (AoS, sorted like the reference's particles_)."""
+
+    def __init__(self, p, pos4, vel4, types, precision=32, omp=True):
+        self.lib = oracle(omp)
+        self.n = len(types)
+        self.pos4 = np.ascontiguousarray(pos4, np.float32)
+        v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+        t = np.ascontiguousarray(types, np.int32)
+        self.h = self.lib.orc_sim_create(C.byref(p), precision, self.n, ptr(self.pos4), ptr(v), ptr(t, C.c_int))
+
+    def upload_bodies(self, bodies, body_id, rel4, pmass=None):
+        arr = (Body * len(bodies))(*bodies)
+        bid = np.ascontiguousarray(body_id, np.int32)
+        rel = np.ascontiguousarray(rel4, np.float32)
+        pm = None if pmass is None else np.ascontiguousarray(pmass, np.float32)
+        self.lib.orc_sim_upload_bodies(self.h, len(bodies), arr, ptr(bid, C.c_int), ptr(rel), ptr(pm))
+
+    def step(self, nsteps, method):
+        rc = self.lib.orc_sim_step(self.h, nsteps, method)
+        if rc != 0:
+            raise RuntimeError("oracle step failed: %d" % rc)
+
+    def state(self, pcisph=True):
+        n = self.n
+        out = dict(pos=np.zeros((n, 4), np.float32), vel=np.zeros((n, 4), np.float32),
+                   rho=np.zeros(n, np.float32), p=np.zeros(n, np.float32),
+                   force=np.zeros((n, 4), np.float32), psi=np.zeros(n, np.float32),
+                   type=np.zeros(n, np.int32), keys=np.zeros(n, np.uint32), ids=np.zeros(n, np.int32))
+        self.lib.orc_sim_download(self.h, ptr(out["pos"]), ptr(out["vel"]), ptr(out["rho"]), ptr(out["p"]),
+                                  ptr(out["force"]), ptr(out["psi"]), ptr(out["type"], C.c_int),
+                                  ptr(out["keys"], C.c_uint32), ptr(out["ids"], C.c_int), 1 if pcisph else 0)
+        return out
+
+    def bodies(self, nb):
+        arr = (Body * nb)()
+        self.lib.orc_sim_download_bodies(self.h, arr)
+        return list(arr)
+
+    def stats(self, max_steps=100000):
+        K = np.zeros(max_steps, np.int32)
+        r = np.zeros(max_steps, np.float32)
+        m = self.lib.orc_sim_stats(self.h, max_steps, ptr(K, C.c_int), ptr(r))
+        return K[:m], r[:m]
+
+    def neighbors(self, cap=None):
+        n = self.n
+        cap = cap or n * 200
+        counts = np.zeros(n, np.int32)
+        flat = np.zeros(cap, np.int32)
+        w = self.lib.orc_sim_neighbors(self.h, ptr(counts, C.c_int), ptr(flat, C.c_int), cap)
+        assert w <= cap
+        return counts, flat[:w]
+
+    def __del__(self):
+        try:
+            self.lib.orc_sim_destroy(self.h)
+        except Exception:
+            pass

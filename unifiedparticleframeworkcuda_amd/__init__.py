@@ -1,0 +1,218 @@
+"""MI355X-native per-step particle pipeline (PCISPH / WCSPH / rigid-fluid
+coupling) of the Unified Particle Framework, behind a C ABI (include/upf.h).
+
+This module is the host-side mirror of the reference's simulation-system API
+(UnifiedPhysics + UnifiedConstants + Scene, src/cpu/UnifiedPhysics.h,
+src/sceneConfigure/UnifiedConstants.h, src/sceneConfigure/Scene.h) for
+Python callers: it loads the in-tree HIP library ``libupf.so`` through
+ctypes and raises when the library is missing -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from ._abi import (Params, DamBreakDesc, RigidSceneDesc, Body, ptr, ERROR_NAMES,  # noqa: F401
+                   UPF_OK, UPF_LIQUID, UPF_FROZEN, UPF_RIGID, UPF_PCISPH_STATIC,
+                   UPF_WCSPH_STATIC, UPF_PCISPH_RIGID, UPF_BOUNDARY_CPU_REF, UPF_BOUNDARY_AKINCI)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libupf.so")
+
+_lib = None
+
+
+class UPFError(RuntimeError):
+    def __init__(self, fn, code):
+        super().__init__("%s failed: %s (%d)" % (fn, ERROR_NAMES.get(code, "?"), code))
+        self.code = code
+
+
+def lib():
+    """The HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libupf.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        pf = C.POINTER(C.c_float)
+        pi = C.POINTER(C.c_int)
+        pu = C.POINTER(C.c_uint32)
+        pp = C.POINTER(Params)
+        vp = C.c_void_p
+        L.upf_params_init.argtypes = [C.c_float, pp]
+        L.upf_params_set_domain.argtypes = [pp, pf, pf, pf]
+        L.upf_build_luts.argtypes = [pp, pf]
+        L.upf_compute_density_error_factor.argtypes = [pp, pi]
+        L.upf_scene_dam_break.argtypes = [C.POINTER(DamBreakDesc), pp, pi, pi, pf, pi]
+        L.upf_create.argtypes = [pp, C.c_int, C.POINTER(vp)]
+        L.upf_destroy.argtypes = [vp]
+        L.upf_set_params.argtypes = [vp, pp]
+        L.upf_upload_particles.argtypes = [vp, C.c_int, pf, pf, pi]
+        L.upf_step.argtypes = [vp, C.c_int, C.c_int]
+        L.upf_download.argtypes = [vp, pf, pf, pf, pf, pf, pf, pi]
+        L.upf_get_step_stats.argtypes = [vp, C.c_int, pi, pf]
+        L.upf_sort_and_bin.argtypes = [vp]
+        L.upf_download_sorted.argtypes = [vp, pu, pi, pu, pi]
+        L.upf_keys.argtypes = [pp, C.c_int, pf, pu]
+        L.upf_radix_sort.argtypes = [C.c_int, pu, pu, pi]
+        L.upf_debug_box_cells.argtypes = [pp, C.c_float, C.c_float, C.c_float, pu, C.c_int]
+        L.upf_version.restype = C.c_char_p
+        for name, args in (("upf_upload_bodies", [vp, C.c_int, C.POINTER(Body), pi, pf]),
+                           ("upf_download_bodies", [vp, C.POINTER(Body)])):
+            if hasattr(L, name):
+                getattr(L, name).argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(fn, rc):
+    if rc != UPF_OK:
+        raise UPFError(fn, rc)
+    return rc
+
+
+# ---------------------------------------------------------------- set-up
+def make_params(spacing=0.01, dt=None):
+    """UnifiedConstants for a particle spacing (src/sceneConfigure/UnifiedConstants.cpp:7-191)."""
+    p = Params()
+    check("upf_params_init", lib().upf_params_init(spacing, C.byref(p)))
+    if dt is not None:
+        p.dt = dt
+    return p
+
+
+def luts(p):
+    out = np.zeros(5 * 301, np.float32)
+    check("upf_build_luts", lib().upf_build_luts(C.byref(p), ptr(out)))
+    return out.reshape(5, 301)
+
+
+def density_error_factor(p):
+    """Fills p.density_error_factor (ComputeDensityErrorFactor, src/cpu/UnifiedPhysics.cpp:5525)."""
+    mx = C.c_int()
+    check("upf_compute_density_error_factor", lib().upf_compute_density_error_factor(C.byref(p), C.byref(mx)))
+    return p.density_error_factor, mx.value
+
+
+def dam_break(p, nx, ny, nz, box, jitter_rel=0.001, wall_rel=1.0, lattice_rel=1.0025, seed=0):
+    """Seeded dam-break scene; sets the domain of p.  Returns (pos4, type, n_fluid)."""
+    d = DamBreakDesc(nx=nx, ny=ny, nz=nz, jitter_rel=jitter_rel, wall_rel=wall_rel, lattice_rel=lattice_rel,
+                     seed=seed)
+    d.box[:] = list(box)
+    nf, nw = C.c_int(), C.c_int()
+    check("upf_scene_dam_break", lib().upf_scene_dam_break(C.byref(d), C.byref(p), C.byref(nf), C.byref(nw),
+                                                             None, None))
+    n = nf.value + nw.value
+    pos = np.zeros((n, 4), np.float32)
+    typ = np.zeros(n, np.int32)
+    check("upf_scene_dam_break", lib().upf_scene_dam_break(C.byref(d), C.byref(p), C.byref(nf), C.byref(nw),
+                                                             ptr(pos), ptr(typ, C.c_int)))
+    return pos, typ, nf.value
+
+
+def standard_dam_break(nx, ny, nz, spacing=0.01, dt=0.001, seed=0, box_per_cell=(0.02, 0.01, 0.01)):
+    """The dam break of the benchmark configurations: an nx*ny*nz lattice in
+    the corner of a box of (2 nx, ny, nz) spacings, dt = 1 ms."""
+    p = make_params(spacing, dt)
+    box = (box_per_cell[0] * nx * spacing / 0.01, box_per_cell[1] * ny * spacing / 0.01,
+           box_per_cell[2] * nz * spacing / 0.01)
+    pos, typ, nf = dam_break(p, nx, ny, nz, box, seed=seed)
+    density_error_factor(p)
+    return p, pos, typ, nf
+
+
+# ------------------------------------------------------------ simulation
+class Simulation:
+    """One simulation on the current HIP device (the reference's UnifiedPhysics
+    GPU path: InitCudaVariables + CudaParticlePhysics, src/cpu/UnifiedPhysics.cpp:615-643,4051-4141)."""
+
+    def __init__(self, params, capacity):
+        self.p = params.copy()
+        self.h = C.c_void_p()
+        check("upf_create", lib().upf_create(C.byref(self.p), int(capacity), C.byref(self.h)))
+        self.n = 0
+
+    def set_params(self, params):
+        self.p = params.copy()
+        check("upf_set_params", lib().upf_set_params(self.h, C.byref(self.p)))
+
+    def upload(self, pos4, types, vel4=None):
+        pos4 = np.ascontiguousarray(pos4, np.float32)
+        types = np.ascontiguousarray(types, np.int32)
+        vel = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+        self.n = len(types)
+        check("upf_upload_particles", lib().upf_upload_particles(self.h, self.n, ptr(pos4), ptr(vel),
+                                                                 ptr(types, C.c_int)))
+
+    def upload_bodies(self, bodies, body_id, rel4):
+        arr = (Body * len(bodies))(*bodies)
+        bid = np.ascontiguousarray(body_id, np.int32)
+        rel = np.ascontiguousarray(rel4, np.float32)
+        check("upf_upload_bodies", lib().upf_upload_bodies(self.h, len(bodies), arr, ptr(bid, C.c_int), ptr(rel)))
+
+    def step(self, nsteps=1, method=UPF_PCISPH_STATIC):
+        check("upf_step", lib().upf_step(self.h, int(nsteps), int(method)))
+
+    def state(self):
+        n = self.n
+        out = dict(pos=np.zeros((n, 4), np.float32), vel=np.zeros((n, 4), np.float32),
+                   rho=np.zeros(n, np.float32), p=np.zeros(n, np.float32),
+                   force=np.zeros((n, 4), np.float32), psi=np.zeros(n, np.float32), type=np.zeros(n, np.int32))
+        check("upf_download", lib().upf_download(self.h, ptr(out["pos"]), ptr(out["vel"]), ptr(out["rho"]),
+                                                 ptr(out["p"]), ptr(out["force"]), ptr(out["psi"]),
+                                                 ptr(out["type"], C.c_int)))
+        return out
+
+    def bodies(self, nb):
+        arr = (Body * nb)()
+        check("upf_download_bodies", lib().upf_download_bodies(self.h, arr))
+        return list(arr)
+
+    def stats(self, max_steps=1 << 20):
+        K = np.zeros(max_steps, np.int32)
+        r = np.zeros(max_steps, np.float32)
+        m = lib().upf_get_step_stats(self.h, max_steps, ptr(K, C.c_int), ptr(r))
+        return K[:m], r[:m]
+
+    def sort_and_bin(self):
+        check("upf_sort_and_bin", lib().upf_sort_and_bin(self.h))
+
+    def sorted_state(self):
+        n = self.n
+        keys = np.zeros(n, np.uint32)
+        ids = np.zeros(n, np.int32)
+        nc = C.c_int()
+        check("upf_download_sorted", lib().upf_download_sorted(self.h, ptr(keys, C.c_uint32), ptr(ids, C.c_int),
+                                                               None, C.byref(nc)))
+        cells = np.zeros((nc.value, 3), np.uint32)
+        check("upf_download_sorted", lib().upf_download_sorted(self.h, None, None, ptr(cells, C.c_uint32),
+                                                               C.byref(nc)))
+        return keys, ids, cells
+
+    def close(self):
+        if self.h:
+            lib().upf_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_keys(p, pos4):
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    keys = np.zeros(len(pos4), np.uint32)
+    check("upf_keys", lib().upf_keys(C.byref(p), len(pos4), ptr(pos4), ptr(keys, C.c_uint32)))
+    return keys
+
+
+def device_radix_sort(keys):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    out = np.zeros_like(keys)
+    perm = np.zeros(len(keys), np.int32)
+    check("upf_radix_sort", lib().upf_radix_sort(len(keys), ptr(keys, C.c_uint32), ptr(out, C.c_uint32),
+                                                 ptr(perm, C.c_int)))
+    return out, perm

@@ -1,0 +1,23 @@
+// upf_internal.h -- declarations shared by the library's translation units.
+#ifndef UPF_INTERNAL_H_
+#define UPF_INTERNAL_H_
+
+#include <cstdint>
+
+#include "upf.h"
+
+namespace upf {
+
+uint32_t spread10(uint32_t v);
+uint32_t morton3(uint32_t x, uint32_t y, uint32_t z);
+bool host_key(const upf_params& p, float x, float y, float z, uint32_t& key);
+float pressure_grad(float dist, float sr);
+void build_luts(float sr, float* out);  // 5 * 301 floats
+float sqrt_threshold(float h);
+uint64_t splitmix64(uint64_t& s);
+double uniform01(uint64_t& s);
+int choose_domain(upf_params* p, const float box[3], float pad);
+
+}  // namespace upf
+
+#endif

@@ -1,0 +1,59 @@
+// upf_kernels.h -- device state layout and kernel declarations of the step.
+#ifndef UPF_KERNELS_H_
+#define UPF_KERNELS_H_
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "upf.h"
+
+namespace upf {
+
+constexpr int LUT_N = 301;
+constexpr int BLOCK = 256;
+
+// Loop control of the PCISPH correction loop, kept on the device.
+struct Ctl {
+    uint32_t max_bits;   // bit pattern of the max predicted density (non-negative floats order like uints)
+    uint32_t done;       // workgroups finished in the current density pass
+    int want;            // number of iterations the loop must run (grows as the loop decides)
+    int too_large;       // density_error_too_large_ latch (src/cpu/UnifiedPhysics.cpp:5373,5397-5398)
+    float last_max;      // max predicted density of the last completed iteration
+    uint32_t err;        // bit (1 << UPF_ERR_*) set on failures
+    uint32_t nbr_max;    // largest neighbour count of the step
+    uint32_t pad;
+};
+
+// Per-particle state of one sorted slot range (scene).  Arrays indexed by slot.
+struct Slots {
+    float4* pos;        // xyz, w = particle type (int bits)
+    float4* vel;
+    int* id;            // upload index
+    float* p_last;      // correction pressure at the end of the previous step's loop
+    float* p_prev;      // the one before (previous_correction_pressure_ of the CPU path)
+    float* rho_last;    // predicted density of the previous step's last iteration
+    float* psi;         // boundary volume (weighted_volume_)
+    float4* x0;         // initial position (predicted position of rigid particles in the CPU path)
+    int* body;          // rigid body index or -1
+    float4* rel;        // initial offset from the body's centre of mass
+};
+
+struct Work {           // per-step arrays, slot order
+    uint32_t* keys;     // keys of the slot order before sorting (written by integration)
+    uint32_t* keys_sorted;
+    uint32_t* perm;     // sorted slot -> previous slot
+    float4* force;      // external force, after integration force + pressure force
+    float4* fp;         // correction pressure force
+    float4* xs;         // predicted position
+    float* p;           // correction pressure of the running loop
+    float* rho;         // predicted density (PCISPH) / density (WCSPH)
+    float* pres;        // WCSPH pressure
+    uint32_t* nbr;      // neighbour lists, column-major: nbr[k * stride + slot]
+    int* ncount;
+    uint2* cells;       // dense cell table indexed by cell Morton code: [first, end)
+};
+
+}  // namespace upf
+
+#endif

@@ -1,0 +1,122 @@
+// upf_scene.cpp -- seeded scene generators (host).  They replace the
+// reference's Scene::build / createFluidObject / createBoxBoundaryFrozenParticles
+// (src/sceneConfigure/Scene.cpp:23-138,597-645,890-1042) with analytic,
+// reproducible inputs: a jittered fluid lattice in a box corner, wall
+// particles on the floor and four sides, and surface-sampled rigid boxes and
+// spheres.  The domain (virtual z-index box, collision box) is chosen so that
+// every particle keys inside the 1024^3 grid and one cell holds the support
+// radius.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "upf.h"
+#include "upf_internal.h"
+
+namespace upf {
+
+uint64_t splitmix64(uint64_t& s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
+
+// Domain for a container of extent box[3] whose wall layer starts at `pad`:
+// cubic virtual box (isotropic cells) grown until the block holds h.
+int choose_domain(upf_params* p, const float box[3], float pad) {
+    float e = 0.0f;
+    for (int a = 0; a < 3; ++a) e = std::max(e, box[a]);
+    e += 2.0f * pad + 2.0f * p->spacing;
+    const float r = p->particle_radius;
+    float cmin[3], cmax[3];
+    for (int a = 0; a < 3; ++a) {
+        cmin[a] = pad + r;
+        cmax[a] = pad + box[a] - r;  // container interior [pad, pad + box]
+    }
+    for (int tries = 0; tries < 400; ++tries) {
+        float vmax[3] = {e, e, e};
+        upf_params q = *p;
+        if (upf_params_set_domain(&q, vmax, cmin, cmax) == UPF_OK) { *p = q; return UPF_OK; }
+        e *= 1.002f;
+    }
+    return UPF_ERR_ARG;
+}
+
+}  // namespace upf
+
+using namespace upf;
+
+extern "C" {
+
+int upf_scene_dam_break(const upf_dam_break_desc* d, upf_params* p, int* n_fluid, int* n_wall,
+                        float* pos4, int* type) {
+    if (!d || !p || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || !(d->wall_rel > 0.0f)) return UPF_ERR_ARG;
+    const double lat = d->lattice_rel > 0.0f ? d->lattice_rel : 1.0;
+    const double dx = p->spacing;
+    const double ws = d->wall_rel * dx;
+    const float pad = p->support_radius;
+    const float box[3] = {(float)(d->box[0] * lat), (float)(d->box[1] * lat), (float)(d->box[2] * lat)};
+    int rc = choose_domain(p, box, pad);
+    if (rc != UPF_OK) return rc;
+    // Container interior [pad, pad + box]; the wall layer lies one particle
+    // radius outside it and the collision box one radius inside it, so the
+    // first fluid layer is one spacing from the wall layer, as in the
+    // reference (walls at h - r, fluid and collision box from h + r:
+    // Scene.cpp:41,902,928, UnifiedConstants.cpp:374-375).
+    const double r = 0.5 * dx;
+    const double o = pad - r;
+    const double lx = box[0] + 2.0 * r, ly = box[1] + 2.0 * r, lz = box[2] + 2.0 * r;
+    // wall lattice: floor (y = o), sides x = o, x = o+lx, z = o, z = o+lz; open top
+    const int wx = (int)std::floor(lx / ws + 0.5), wy = (int)std::floor(ly / ws + 0.5),
+              wz = (int)std::floor(lz / ws + 0.5);
+    std::vector<float> walls;
+    auto add_wall = [&](int i, int j, int k) {
+        walls.push_back((float)(o + i * ws));
+        walls.push_back((float)(o + j * ws));
+        walls.push_back((float)(o + k * ws));
+    };
+    for (int i = 0; i <= wx; ++i)
+        for (int k = 0; k <= wz; ++k) add_wall(i, 0, k);           // floor
+    for (int j = 1; j <= wy; ++j)
+        for (int k = 0; k <= wz; ++k) { add_wall(0, j, k); add_wall(wx, j, k); }   // x sides
+    for (int j = 1; j <= wy; ++j)
+        for (int i = 1; i < wx; ++i) { add_wall(i, j, 0); add_wall(i, j, wz); }    // z sides
+    const int nf = d->nx * d->ny * d->nz;
+    const int nw = (int)(walls.size() / 3);
+    if (n_fluid) *n_fluid = nf;
+    if (n_wall) *n_wall = nw;
+    if (!pos4) return UPF_OK;
+    uint64_t s = d->seed * 0x2545F4914F6CDD1Dull + 1;
+    const double jit = d->jitter_rel * dx;
+    int w = 0;
+    // fluid lattice (y slowest, z fastest, as createFluidObject loops)
+    for (int iy = 0; iy < d->ny; ++iy)
+        for (int ix = 0; ix < d->nx; ++ix)
+            for (int iz = 0; iz < d->nz; ++iz) {
+                const double jx = (2.0 * uniform01(s) - 1.0) * jit;
+                const double jy = (2.0 * uniform01(s) - 1.0) * jit;
+                const double jz = (2.0 * uniform01(s) - 1.0) * jit;
+                pos4[4 * w + 0] = (float)(pad + (ix + 0.5) * lat * dx + jx);
+                pos4[4 * w + 1] = (float)(pad + (iy + 0.5) * lat * dx + jy);
+                pos4[4 * w + 2] = (float)(pad + (iz + 0.5) * lat * dx + jz);
+                pos4[4 * w + 3] = 0.0f;
+                if (type) type[w] = UPF_LIQUID;
+                ++w;
+            }
+    for (int k = 0; k < nw; ++k) {
+        pos4[4 * w + 0] = walls[3 * k];
+        pos4[4 * w + 1] = walls[3 * k + 1];
+        pos4[4 * w + 2] = walls[3 * k + 2];
+        pos4[4 * w + 3] = 0.0f;
+        if (type) type[w] = UPF_FROZEN;
+        ++w;
+    }
+    return UPF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1113 @@
+// upf_sim.hip -- the per-step particle pipeline on gfx950 and its C ABI.
+//
+// One step (PCISPH, static boundaries), all on one HIP stream:
+//   radix sort of z-index keys -> gather of the particle state into key order
+//   -> dense cell table (first/end slot per occupied cell, by boundary
+//   detection, no atomics) -> neighbour lists + external forces + first
+//   prediction (one pass over the query-box candidates) -> K x [predicted
+//   density & pressure with the convergence test on the device; correction
+//   pressure force fused with the next prediction] -> integration + new keys.
+//
+// Float arithmetic follows the reference CPU path (src/cpu/UnifiedPhysics.cpp)
+// operation by operation, neighbours are visited in ascending sorted order
+// like the reference's neighbour lists, and the file is compiled with
+// -ffp-contract=off and correctly rounded division / sqrt, so results match
+// the CPU restatement bit for bit where the reference's arithmetic is IEEE.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "upf.h"
+#include "upf_device.h"
+#include "upf_internal.h"
+#include "upf_kernels.h"
+#include "upf_sort.h"
+
+namespace upf {
+
+__device__ inline float lut_at(const float* lut, float dist, float factor) {
+    // kernel*Lut(dist): index = dist * (1/h) * kLutSize (src/cpu/WeightingKernel.cpp:122-127)
+    const int idx = (int)(dist * factor * 300.0f);
+    return idx >= 300 ? 0.0f : lut[idx];
+}
+
+__device__ inline int ptype(const float4& x) { return __float_as_int(x.w); }
+
+__device__ inline void set_err(Ctl* ctl, int code) { atomicOr(&ctl->err, 1u << code); }
+
+// ------------------------------------------------------------------ keys
+__global__ void k_keys(int n, const float4* __restrict__ pos, uint32_t* __restrict__ keys, upf_params P, Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = pos[i];
+    bool ok;
+    keys[i] = dkey(x.x, x.y, x.z, P.scales, ok);
+    if (!ok) set_err(ctl, UPF_ERR_RANGE);
+}
+
+// --------------------------------------------------------------- gather
+// CopySortedParticleValuesKernel* (src/gpu/particles_kernel.cu:544-654):
+// the persistent particle state follows the sort permutation.  Pressures of
+// the finished loop move one generation back (p_last -> p_prev).
+__global__ void k_gather(int n, const uint32_t* __restrict__ perm, Slots src, Slots dst, const float* __restrict__ p_work,
+                         const float* __restrict__ rho_work, int rigid) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int o = (int)perm[s];
+    dst.pos[s] = src.pos[o];
+    dst.vel[s] = src.vel[o];
+    dst.id[s] = src.id[o];
+    dst.p_prev[s] = src.p_last[o];
+    dst.p_last[s] = p_work[o];
+    dst.rho_last[s] = rho_work[o];
+    dst.psi[s] = src.psi[o];
+    if (rigid) {
+        dst.x0[s] = src.x0[o];
+        dst.body[s] = src.body[o];
+        dst.rel[s] = src.rel[o];
+    }
+}
+
+// ------------------------------------------------------------ cell table
+__global__ void k_fill_cells(uint2* __restrict__ cells, uint32_t ncells) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncells) cells[c] = make_uint2(0u, 0u);
+}
+
+// first / end slot of every occupied cell by boundary detection over the
+// sorted keys (BlockGeneration, src/gpu/particles.cu:904-964, without atomics)
+__global__ void k_cell_bounds(int n, const uint32_t* __restrict__ keys, int shift3, uint2* __restrict__ cells) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t c = keys[i] >> shift3;
+    if (i == 0 || (keys[i - 1] >> shift3) != c) cells[c].x = (uint32_t)i;
+    if (i == n - 1 || (keys[i + 1] >> shift3) != c) cells[c].y = (uint32_t)(i + 1);
+}
+
+// Visit the candidates of the reference's query box around x in ascending
+// slot order; f(j) is called for every j in the covered cells.
+template <class F>
+__device__ inline void for_each_candidate(const float4& x, const upf_params& P, const uint2* __restrict__ cells, F f) {
+    BoxCells bc;
+    bc.init(x.x, x.y, x.z, P);
+    const int cnt = bc.count();
+    for (int e = 0; e < cnt; ++e) {
+        uint32_t c[3];
+        if (!bc.cell(e, c)) continue;
+        if (c[0] >= P.grid_dim || c[1] >= P.grid_dim || c[2] >= P.grid_dim) continue;
+        const uint2 r = cells[dmorton3(c[0], c[1], c[2])];
+        for (uint32_t j = r.x; j < r.y; ++j) f((int)j);
+    }
+}
+
+// --------------------------------------------------- boundary volume (Psi)
+// CalculateWeightedVolume (src/cpu/UnifiedPhysics.cpp:2703-2746): Psi_b = 1 / sum_b W
+__global__ void k_psi(int n, const float4* __restrict__ pos, const uint2* __restrict__ cells, upf_params P,
+                      const float* __restrict__ luts, float lut_factor, float* __restrict__ psi, Ctl* ctl) {
+    __shared__ float lw[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 xi = pos[i];
+    const int ti = ptype(xi);
+    if (ti != UPF_FROZEN && ti != UPF_RIGID) return;
+    const float T = P.support_radius_sq_limit;
+    float inv = 0.0f;
+    for_each_candidate(xi, P, cells, [&](int j) {
+        if (j == i) return;
+        const float4 xj = pos[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < T)) return;
+        const int tj = ptype(xj);
+        if (tj == UPF_FROZEN || tj == UPF_RIGID) inv += lut_at(lw, sqrtf(d2), lut_factor);
+    });
+    if (inv != 0.0f) psi[i] = (float)(1.0 / (double)inv);
+    else set_err(ctl, UPF_ERR_ISOLATED);
+}
+
+struct StepArgs {
+    upf_params P;
+    float lut_factor;   // (float)(1.0 / h)
+    float inv_mass_f;   // 1.0f / m (vmmlib operator/(float))
+    float inv_mass_d;   // (float)(1.0 / m) (static const float invMass, :3714)
+    float rest_vol;     // m / rho0
+    float dens_sq;      // rho0 * rho0
+    int n;
+    int stride;         // column stride of the neighbour lists
+};
+
+// Predicted position + CollisionHandling (src/cpu/UnifiedPhysics.cpp:5426-5454, :5739-5757)
+__device__ inline float4 predict(const float4& x, const float4& v, const float4& F, const float4& Fp,
+                                 const StepArgs& A, Ctl* ctl) {
+    const upf_params& P = A.P;
+    float ax = F.x + Fp.x, ay = F.y + Fp.y, az = F.z + Fp.z;
+    ax = ax * P.dt; ay = ay * P.dt; az = az * P.dt;
+    ax = ax * A.inv_mass_f; ay = ay * A.inv_mass_f; az = az * A.inv_mass_f;
+    const float vx = v.x + ax, vy = v.y + ay, vz = v.z + az;
+    float px = x.x + vx * P.dt, py = x.y + vy * P.dt, pz = x.z + vz * P.dt;
+    px = px < P.collision_min[0] ? P.collision_min[0] : px;
+    px = px > P.collision_max[0] ? P.collision_max[0] : px;
+    py = py < P.collision_min[1] ? P.collision_min[1] : py;
+    py = py > P.collision_max[1] ? P.collision_max[1] : py;
+    pz = pz < P.collision_min[2] ? P.collision_min[2] : pz;
+    pz = pz > P.collision_max[2] ? P.collision_max[2] : pz;
+    if (!(px == px) || !(py == py) || !(pz == pz)) set_err(ctl, UPF_ERR_NAN);
+    return make_float4(px, py, pz, x.w);
+}
+
+// ---------------------------------------------- neighbours + external force
+// GetNeighbors (:2242-2295) + CalculateExternalForcesStaticBoundariesPCISPH
+// (:5198-5250) + the first PredictPositionAndVelocity of the loop.
+// Liquid slots build their neighbour list (ascending slots), accumulate the
+// external force over it and predict; every slot resets its loop pressure.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_nbr_ext_static(StepArgs A, Slots S, Work W, const uint2* __restrict__ cells,
+                                                          const float* __restrict__ luts, Ctl* ctl) {
+    __shared__ float lvl[LUT_N], lpg[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
+        lvl[t] = luts[2 * LUT_N + t];
+        lpg[t] = luts[LUT_N + t];
+    }
+    __syncthreads();
+    const upf_params& P = A.P;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->max_bits = 0u;
+        ctl->done = 0u;
+        ctl->too_large = 1;
+        ctl->want = (0 < P.min_loops || 0 < P.max_loops) ? 1 : 0;
+        ctl->last_max = 0.0f;
+    }
+    if (i >= A.n) return;
+    const float4 xi = S.pos[i];
+    const int ti = ptype(xi);
+    W.p[i] = 0.0f;
+    W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    W.rho[i] = S.rho_last[i];
+    if (ti != UPF_LIQUID) {
+        W.xs[i] = (ti == UPF_RIGID && !P.rigid_predicted_current) ? S.x0[i] : xi;
+        W.ncount[i] = 0;
+        W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float4 vi = S.vel[i];
+    const float h = P.support_radius;
+    const float T = P.support_radius_sq_limit;
+    const float rv = A.rest_vol, visc = P.visc_const;
+    const float own_p = MODE == UPF_BOUNDARY_CPU_REF ? S.p_prev[i] : S.p_last[i];
+    const float own_rho = S.rho_last[i];
+    const float own_psi = S.psi[i];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    int cnt = 0;
+    const int cap = P.max_neighbors;
+    for_each_candidate(xi, P, cells, [&](int j) {
+        if (j == i) return;
+        const float4 xj = S.pos[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < T)) return;
+        if (cnt < cap) W.nbr[(size_t)cnt * A.stride + i] = (uint32_t)j;
+        ++cnt;
+        const float dist = sqrtf(d2);
+        const int tj = ptype(xj);
+        if (tj == UPF_LIQUID) {
+            const float kv = lut_at(lvl, dist, A.lut_factor);
+            const float4 vj = S.vel[j];
+            const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+            fx = fx - vx * rv * rv * visc * kv;
+            fy = fy - vy * rv * rv * visc * kv;
+            fz = fz - vz * rv * rv * visc * kv;
+        } else if (tj == UPF_FROZEN) {
+            // CalculateBoundaryFluidPressureForceHost (:4240-4250)
+            uint32_t dl = (uint32_t)(dist * 300.0f / h);
+            dl = dl > 300u ? 300u : dl;
+            const float g = MODE == UPF_BOUNDARY_CPU_REF ? lut_at(lpg, (float)dl, A.lut_factor) : lpg[dl];
+            const float wv = MODE == UPF_BOUNDARY_CPU_REF ? own_psi : S.psi[j];
+            const float inv = 1.0f / own_rho;
+            const float pg = own_p * inv * inv;
+            const float s1 = (((dx * g) * pg) * P.initial_mass) * P.rest_density;
+            const float s2 = (((dy * g) * pg) * P.initial_mass) * P.rest_density;
+            const float s3 = (((dz * g) * pg) * P.initial_mass) * P.rest_density;
+            fx = fx + (s1 * wv) * -1.0f;
+            fy = fy + (s2 * wv) * -1.0f;
+            fz = fz + (s3 * wv) * -1.0f;
+        }
+    });
+    fy = fy - P.initial_mass * P.gravity;
+    W.ncount[i] = cnt < cap ? cnt : cap;  // lists hold at most cap entries
+    if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
+    atomicMax(&ctl->nbr_max, (uint32_t)cnt);
+    const float4 F = make_float4(fx, fy, fz, 0.0f);
+    W.force[i] = F;
+    W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, ctl);
+}
+
+// ------------------------------------------------------ PCISPH iteration
+__device__ inline float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// ComputePredictedDensityAndPressure (:5456-5491) + max reduction + the loop
+// test of PredictionCorrectionStep (:5376-5398), evaluated by the last
+// workgroup to finish.  Iteration t runs only while t < ctl->want.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                   Ctl* ctl) {
+    if (t >= __atomic_load_n(&ctl->want, __ATOMIC_RELAXED)) return;
+    __shared__ float lw[LUT_N];
+    __shared__ float wmax[BLOCK / 64];
+    __shared__ int last;
+    for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lw[k] = luts[k];
+    __syncthreads();
+    const upf_params& P = A.P;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float mine = 0.0f;
+    if (i < A.n) {
+        const float4 xi = W.xs[i];
+        if (ptype(xi) == UPF_LIQUID) {
+            const float m = P.initial_mass;
+            const float T = P.support_radius_sq_limit;
+            float rho = P.kernel_self * m;
+            const int cnt = W.ncount[i];
+            for (int k = 0; k < cnt; ++k) {
+                const int j = (int)W.nbr[(size_t)k * A.stride + i];
+                const float4 xj = W.xs[j];
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 < T) {
+                    const float w = lut_at(lw, sqrtf(d2), A.lut_factor);
+                    float mj = m;
+                    if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
+                    rho = rho + w * mj;
+                }
+            }
+            const float err = fmax_ref(rho - P.rest_density, 0.0f);
+            W.p[i] = W.p[i] + fmax_ref(err * P.density_error_factor, 0.0f);
+            W.rho[i] = rho;
+            mine = rho;
+        }
+    }
+    mine = wave_max(mine);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = wmax[0];
+        for (int k = 1; k < BLOCK / 64; ++k) b = fmaxf(b, wmax[k]);
+        atomicMax(&ctl->max_bits, __float_as_uint(b));
+        __threadfence();
+        last = atomicAdd(&ctl->done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        const float mx = __uint_as_float(atomicOr(&ctl->max_bits, 0u));
+        const float pct = fmax_ref(0.1f * mx - 100.0f, 0.0f);
+        int too_large = ctl->too_large;
+        if (pct < P.max_density_error_allowed) too_large = 0;
+        const int it = t + 1;
+        const bool next = it < P.min_loops || (too_large && it < P.max_loops);
+        ctl->too_large = too_large;
+        ctl->last_max = mx;
+        ctl->max_bits = 0u;
+        ctl->done = 0u;
+        __atomic_store_n(&ctl->want, next ? it + 1 : it, __ATOMIC_RELAXED);
+    }
+}
+
+// ComputeCorrectivePressureForce (:5493-5522), then the prediction of the
+// next iteration for the same particle.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                 Ctl* ctl) {
+    if (t >= __atomic_load_n(&ctl->want, __ATOMIC_RELAXED)) return;
+    __shared__ float lpg[LUT_N];
+    for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lpg[k] = luts[LUT_N + k];
+    __syncthreads();
+    const upf_params& P = A.P;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const float4 xi = S.pos[i];
+    if (ptype(xi) != UPF_LIQUID) return;
+    const float m = P.initial_mass;
+    const float T = P.support_radius_sq_limit;
+    const float rho0 = P.rest_density;
+    const float pi = W.p[i];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    const int cnt = W.ncount[i];
+    for (int k = 0; k < cnt; ++k) {
+        const int j = (int)W.nbr[(size_t)k * A.stride + i];
+        const float4 xj = S.pos[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < T) {
+            const float g = lut_at(lpg, sqrtf(d2), A.lut_factor);
+            const float pj = W.p[j];
+            const float grad = pi / A.dens_sq + pj / (rho0 * rho0);
+            float mj = m;
+            if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = rho0 * S.psi[j];
+            fx = fx - (((dx * g) * grad) * m) * mj;
+            fy = fy - (((dy * g) * grad) * m) * mj;
+            fz = fz - (((dz * g) * grad) * m) * mj;
+        }
+    }
+    const float4 Fp = make_float4(fx, fy, fz, 0.0f);
+    W.fp[i] = Fp;
+    W.xs[i] = predict(xi, S.vel[i], W.force[i], Fp, A, ctl);
+}
+
+// ------------------------------------------------------------ integration
+// UpdateLiquidParticlePosVelSphCPU (:3705-3742) + BoundaryHandlingBoxPerParticleOMP
+// (:3940-3970) / BoundaryHandlingPureFluidOMP (:3972-3998): symplectic Euler,
+// box clamp with damping -0.5, new z-index key of every particle.
+__global__ void k_integrate(StepArgs A, Slots S, Work W, int pcisph, int pure_fluid_clamp, int has_bodies, Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const upf_params& P = A.P;
+    float4 x = S.pos[i];
+    const int ti = ptype(x);
+    if (ti == UPF_FROZEN) {
+        bool ok;
+        W.keys[i] = dkey(x.x, x.y, x.z, P.scales, ok);
+        if (!ok) set_err(ctl, UPF_ERR_RANGE);
+        return;
+    }
+    float4 v = S.vel[i];
+    if (ti == UPF_LIQUID) {
+        float4 F = W.force[i];
+        if (pcisph) {
+            const float4 Fp = W.fp[i];
+            F.x = F.x + Fp.x; F.y = F.y + Fp.y; F.z = F.z + Fp.z;
+            W.force[i] = F;
+        }
+        v.x = v.x + (F.x * A.inv_mass_d) * P.dt;
+        v.y = v.y + (F.y * A.inv_mass_d) * P.dt;
+        v.z = v.z + (F.z * A.inv_mass_d) * P.dt;
+        x.x = x.x + v.x * P.dt;
+        x.y = x.y + v.y * P.dt;
+        x.z = x.z + v.z * P.dt;
+    }
+    // static driver: every non-frozen particle without a parent body;
+    // coupling driver: liquid particles only
+    const bool clamp = pure_fluid_clamp ? (ti == UPF_LIQUID) : (!has_bodies || S.body[i] < 0);
+    if (clamp) {
+        const float damping = -0.5f;
+        if (x.y < P.collision_min[1]) { x.y = P.collision_min[1]; v.y *= damping; }
+        if (x.y > P.collision_max[1]) { x.y = P.collision_max[1]; v.y *= damping; }
+        if (x.x < P.collision_min[0]) { x.x = P.collision_min[0]; v.x *= damping; }
+        if (x.x > P.collision_max[0]) { x.x = P.collision_max[0]; v.x *= damping; }
+        if (x.z < P.collision_min[2]) { x.z = P.collision_min[2]; v.z *= damping; }
+        if (x.z > P.collision_max[2]) { x.z = P.collision_max[2]; v.z *= damping; }
+    }
+    if (ti == UPF_LIQUID || clamp) {
+        S.pos[i] = x;
+        S.vel[i] = v;
+    }
+    bool ok;
+    W.keys[i] = dkey(x.x, x.y, x.z, P.scales, ok);
+    if (!ok) set_err(ctl, UPF_ERR_RANGE);
+}
+
+// ------------------------------------------------------------------- WCSPH
+// GetNeighbors + CalculateDensityPressure with physicsType 'w' and no wall
+// weight (:2339-2406) + CalculatePressureWCSPH (:2491-2501)
+__global__ __launch_bounds__(BLOCK) void k_nbr_density_wcsph(StepArgs A, Slots S, Work W, const uint2* __restrict__ cells,
+                                                             const float* __restrict__ luts, Ctl* ctl) {
+    __shared__ float lw[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
+    __syncthreads();
+    const upf_params& P = A.P;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const float4 xi = S.pos[i];
+    if (ptype(xi) != UPF_LIQUID) {
+        W.ncount[i] = 0;
+        W.rho[i] = 0.0f;
+        W.pres[i] = 0.0f;
+        W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float T = P.support_radius_sq_limit;
+    const float m = P.initial_mass;
+    float rho = P.kernel_self * m;
+    int cnt = 0;
+    const int cap = P.max_neighbors;
+    for_each_candidate(xi, P, cells, [&](int j) {
+        if (j == i) return;
+        const float4 xj = S.pos[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < T)) return;
+        if (cnt < cap) W.nbr[(size_t)cnt * A.stride + i] = (uint32_t)j;
+        ++cnt;
+        rho = rho + lut_at(lw, sqrtf(d2), A.lut_factor) * m;
+    });
+    W.ncount[i] = cnt < cap ? cnt : cap;
+    if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
+    atomicMax(&ctl->nbr_max, (uint32_t)cnt);
+    const float gamma = 7.0f;
+    const float b = P.rest_density * P.gas_const_wcsph / gamma;
+    const float frac = rho / P.rest_density;
+    const float pg = powf(frac, gamma);
+    W.rho[i] = rho;
+    W.pres[i] = fmax_ref(0.0f, b * (pg - 1.0f));
+}
+
+// CalculateLiquidParticleForces (:3009-3083) + gravity (:2822-2824)
+__global__ __launch_bounds__(BLOCK) void k_force_wcsph(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                       Ctl* ctl) {
+    __shared__ float lvl[LUT_N], lpg[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
+        lvl[t] = luts[2 * LUT_N + t];
+        lpg[t] = luts[LUT_N + t];
+    }
+    __syncthreads();
+    const upf_params& P = A.P;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= A.n) return;
+    const float4 xi = S.pos[i];
+    if (ptype(xi) != UPF_LIQUID) return;
+    const float m = P.initial_mass;
+    const float T = P.support_radius_sq_limit;
+    const float rho_i = W.rho[i], p_i = W.pres[i];
+    const float invD = (float)(1.0 / (double)rho_i);
+    const float pVol = m * invD;
+    const float4 vi = S.vel[i];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    const int cnt = W.ncount[i];
+    for (int k = 0; k < cnt; ++k) {
+        const int j = (int)W.nbr[(size_t)k * A.stride + i];
+        const float4 xj = S.pos[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < T)) continue;
+        if (ptype(xj) != UPF_LIQUID) continue;
+        const float dist = sqrtf(d2);
+        const float kv = lut_at(lvl, dist, A.lut_factor);
+        const float rho_j = W.rho[j];
+        const float nVol = m / rho_j;
+        const float g = lut_at(lpg, dist, A.lut_factor);
+        const float grad = p_i * invD * invD + W.pres[j] / (rho_j * rho_j);
+        fx = fx - (((dx * g) * grad) * m) * m;
+        fy = fy - (((dy * g) * grad) * m) * m;
+        fz = fz - (((dz * g) * grad) * m) * m;
+        const float4 vj = S.vel[j];
+        const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+        fx = fx - vx * pVol * nVol * P.visc_const * kv;
+        fy = fy - vy * pVol * nVol * P.visc_const * kv;
+        fz = fz - vz * pVol * nVol * P.visc_const * kv;
+    }
+    fy = fy - P.initial_mass * P.gravity;
+    W.force[i] = make_float4(fx, fy, fz, 0.0f);
+}
+
+// -------------------------------------------------------- id-order output
+__global__ void k_scatter_by_id(int n, const int* __restrict__ id, const float4* __restrict__ a4, float4* __restrict__ o4,
+                                const float* __restrict__ a1, float* __restrict__ o1) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int d = id[s];
+    if (a4) o4[d] = a4[s];
+    if (a1) o1[d] = a1[s];
+}
+
+}  // namespace upf
+
+// ======================================================================
+// host side
+// ======================================================================
+using namespace upf;
+
+#define HIPCHK(x)                                  \
+    do {                                           \
+        hipError_t e_ = (x);                       \
+        if (e_ != hipSuccess) return UPF_ERR_HIP;  \
+    } while (0)
+
+enum ProfClass { PC_SORT = 0, PC_GATHER, PC_CELLS, PC_PSI, PC_NBR_EXT, PC_DENSITY, PC_FORCE, PC_INTEGRATE,
+                 PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_COUNT };
+static const char* kProfNames[PC_COUNT] = {"radix_sort", "gather", "cell_table", "boundary_volume",
+                                          "neighbours_external_force", "predicted_density", "pressure_force_predict",
+                                          "integrate", "wcsph_neighbours_density", "wcsph_force", "rigid_bodies"};
+
+struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, int>> pending;  // (class, event index of start; stop = start + 1)
+    int used = 0;
+    double ms[PC_COUNT] = {};
+    long long cnt[PC_COUNT] = {};
+    void begin(int cls, hipStream_t st) {
+        if (!on) return;
+        if (used + 2 > (int)pool.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return;
+                pool.push_back(e);
+            }
+        }
+        pending.push_back(std::make_pair(cls, used));
+        (void)hipEventRecord(pool[used], st);
+        used += 2;
+    }
+    void end(hipStream_t st) {
+        if (!on || pending.empty()) return;
+        (void)hipEventRecord(pool[pending.back().second + 1], st);
+    }
+    // after the stream has been synchronised
+    void collect() {
+        for (auto& pr : pending) {
+            float t = 0.0f;
+            if (hipEventElapsedTime(&t, pool[pr.second], pool[pr.second + 1]) == hipSuccess) {
+                ms[pr.first] += t;
+                cnt[pr.first] += 1;
+            }
+        }
+        pending.clear();
+        used = 0;
+    }
+    ~Prof() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+struct upf_sim {
+    Prof prof;
+    upf_params P;
+    int cap = 0;
+    int n = 0;
+    int cur = 0;            // which Slots buffer holds the current state
+    bool uploaded = false;
+    bool psi_ready = false;
+    bool rigid = false;
+    int n_bodies = 0;
+    int last_method = UPF_PCISPH_STATIC;
+    hipStream_t st = nullptr;
+    Slots S[2];
+    Work W;
+    Ctl* ctl = nullptr;
+    Ctl* h_ctl = nullptr;   // pinned mirror
+    float* luts = nullptr;  // 5 * 301
+    void* sort_tmp = nullptr;
+    uint32_t ncells = 0;
+    std::vector<int> k_hist;
+    std::vector<float> rho_hist;
+    std::vector<void*> allocs;
+    // rigid bodies
+    upf_body* d_bodies = nullptr;
+    float* d_body_red = nullptr;
+    int* d_body_start = nullptr;
+
+    StepArgs args() const {
+        StepArgs A;
+        A.P = P;
+        A.lut_factor = (float)(1.0 / (double)P.support_radius);
+        A.inv_mass_f = 1.0f / P.initial_mass;
+        A.inv_mass_d = (float)(1.0 / (double)P.initial_mass);
+        A.rest_vol = P.initial_mass / P.rest_density;
+        A.dens_sq = P.rest_density * P.rest_density;
+        A.n = n;
+        A.stride = cap;
+        return A;
+    }
+    template <class T> hipError_t alloc(T** p, size_t count) {
+        hipError_t e = hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) allocs.push_back((void*)*p);
+        return e;
+    }
+};
+
+static inline dim3 grid_for(int n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+
+static int check_params(const upf_params* p) {
+    if (!p) return UPF_ERR_ARG;
+    if (p->block_size < 2 || p->grid_dim == 0 || p->grid_dim > 512) return UPF_ERR_ARG;
+    if (p->vbox_min[0] != 0.0f || p->vbox_min[1] != 0.0f || p->vbox_min[2] != 0.0f) return UPF_ERR_ARG;
+    if (p->max_neighbors <= 0 || p->max_neighbors > 4096) return UPF_ERR_ARG;
+    if (p->add_boundary_force) return UPF_ERR_ARG;  // not used by the static-boundary drivers here
+    return UPF_OK;
+}
+
+static int alloc_slots(upf_sim* s, Slots& S, int cap) {
+    HIPCHK(s->alloc(&S.pos, cap));
+    HIPCHK(s->alloc(&S.vel, cap));
+    HIPCHK(s->alloc(&S.id, cap));
+    HIPCHK(s->alloc(&S.p_last, cap));
+    HIPCHK(s->alloc(&S.p_prev, cap));
+    HIPCHK(s->alloc(&S.rho_last, cap));
+    HIPCHK(s->alloc(&S.psi, cap));
+    HIPCHK(s->alloc(&S.x0, cap));
+    HIPCHK(s->alloc(&S.body, cap));
+    HIPCHK(s->alloc(&S.rel, cap));
+    return UPF_OK;
+}
+
+// sort + gather + cell table (the first part of every step)
+static int sort_and_bin(upf_sim* s) {
+    const int n = s->n;
+    Work& W = s->W;
+    s->prof.begin(PC_SORT, s->st);
+    HIPCHK(radix_sort(W.keys, nullptr, W.keys_sorted, W.perm, n, 30, s->sort_tmp, s->st));
+    s->prof.end(s->st);
+    const int nxt = 1 - s->cur;
+    s->prof.begin(PC_GATHER, s->st);
+    hipLaunchKernelGGL(k_gather, grid_for(n), dim3(BLOCK), 0, s->st, n, W.perm, s->S[s->cur], s->S[nxt], W.p, W.rho,
+                       s->rigid ? 1 : 0);
+    s->prof.end(s->st);
+    s->cur = nxt;
+    s->prof.begin(PC_CELLS, s->st);
+    hipLaunchKernelGGL(k_fill_cells, dim3((s->ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s->st, W.cells, s->ncells);
+    s->prof.end(s->st);
+    s->prof.begin(PC_CELLS, s->st);
+    hipLaunchKernelGGL(k_cell_bounds, grid_for(n), dim3(BLOCK), 0, s->st, n, W.keys_sorted, (int)(3 * s->P.block_shift),
+                       W.cells);
+    s->prof.end(s->st);
+    HIPCHK(hipGetLastError());
+    return UPF_OK;
+}
+
+static int read_ctl(upf_sim* s) {
+    HIPCHK(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->prof.collect();
+    return UPF_OK;
+}
+
+static int err_code(uint32_t bits) {
+    const int order[] = {UPF_ERR_NAN, UPF_ERR_RANGE, UPF_ERR_NBR_OVERFLOW, UPF_ERR_ISOLATED};
+    for (int c : order)
+        if (bits & (1u << c)) return c;
+    return UPF_OK;
+}
+
+template <int MODE>
+static int pcisph_loop(upf_sim* s, const StepArgs& A) {
+    const dim3 g = grid_for(s->n);
+    Slots& S = s->S[s->cur];
+    int t = 0;
+    const int first = std::min(std::max(s->P.min_loops, 1), std::max(s->P.max_loops, 1));
+    for (; t < first; ++t) {
+        s->prof.begin(PC_DENSITY, s->st);
+        hipLaunchKernelGGL(k_density<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+        s->prof.end(s->st);
+        s->prof.begin(PC_FORCE, s->st);
+        hipLaunchKernelGGL(k_force<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+        s->prof.end(s->st);
+    }
+    for (;;) {
+        int rc = read_ctl(s);
+        if (rc) return rc;
+        const int want = s->h_ctl->want;
+        if (want <= t || (s->h_ctl->err & (1u << UPF_ERR_NAN))) break;
+        for (; t < want; ++t) {
+            s->prof.begin(PC_DENSITY, s->st);
+            hipLaunchKernelGGL(k_density<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+            s->prof.end(s->st);
+            s->prof.begin(PC_FORCE, s->st);
+            hipLaunchKernelGGL(k_force<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+            s->prof.end(s->st);
+        }
+    }
+    s->k_hist.push_back(s->h_ctl->want);
+    s->rho_hist.push_back(s->h_ctl->last_max);
+    return UPF_OK;
+}
+
+static int step_pcisph_static(upf_sim* s) {
+    int rc = sort_and_bin(s);
+    if (rc) return rc;
+    const StepArgs A = s->args();
+    Slots& S = s->S[s->cur];
+    const dim3 g = grid_for(s->n);
+    if (!s->psi_ready) {
+        s->prof.begin(PC_PSI, s->st);
+        hipLaunchKernelGGL(k_psi, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.cells, s->P, s->luts, A.lut_factor, S.psi,
+                           s->ctl);
+        s->prof.end(s->st);
+        s->psi_ready = true;
+    }
+    if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI) {
+        s->prof.begin(PC_NBR_EXT, s->st);
+        hipLaunchKernelGGL(k_nbr_ext_static<UPF_BOUNDARY_AKINCI>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->W.cells,
+                           s->luts, s->ctl);
+        s->prof.end(s->st);
+        rc = pcisph_loop<UPF_BOUNDARY_AKINCI>(s, A);
+    } else {
+        s->prof.begin(PC_NBR_EXT, s->st);
+        hipLaunchKernelGGL(k_nbr_ext_static<UPF_BOUNDARY_CPU_REF>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->W.cells,
+                           s->luts, s->ctl);
+        s->prof.end(s->st);
+        rc = pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
+    }
+    if (rc) return rc;
+    s->prof.begin(PC_INTEGRATE, s->st);
+    hipLaunchKernelGGL(k_integrate, g, dim3(BLOCK), 0, s->st, A, S, s->W, 1, 0, s->rigid ? 1 : 0, s->ctl);
+    s->prof.end(s->st);
+    HIPCHK(hipGetLastError());
+    return UPF_OK;
+}
+
+static int step_wcsph_static(upf_sim* s) {
+    int rc = sort_and_bin(s);
+    if (rc) return rc;
+    const StepArgs A = s->args();
+    Slots& S = s->S[s->cur];
+    const dim3 g = grid_for(s->n);
+    s->prof.begin(PC_WCSPH_DENSITY, s->st);
+    hipLaunchKernelGGL(k_nbr_density_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->W.cells, s->luts, s->ctl);
+    s->prof.end(s->st);
+    s->prof.begin(PC_WCSPH_FORCE, s->st);
+    hipLaunchKernelGGL(k_force_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+    s->prof.end(s->st);
+    s->prof.begin(PC_INTEGRATE, s->st);
+    hipLaunchKernelGGL(k_integrate, g, dim3(BLOCK), 0, s->st, A, S, s->W, 0, 0, s->rigid ? 1 : 0, s->ctl);
+    s->prof.end(s->st);
+    HIPCHK(hipGetLastError());
+    rc = read_ctl(s);
+    if (rc) return rc;
+    s->k_hist.push_back(0);
+    s->rho_hist.push_back(0.0f);
+    return UPF_OK;
+}
+
+
+extern "C" {
+
+const char* upf_version(void) { return "upf-mi355x 0.1 (gfx950)"; }
+
+int upf_create(const upf_params* p, int capacity, upf_sim** out) {
+    if (!out || capacity <= 0) return UPF_ERR_ARG;
+    int rc = check_params(p);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPF_ERR_NO_DEVICE;
+    upf_sim* s = new upf_sim();
+    s->P = *p;
+    s->cap = capacity;
+    rc = UPF_ERR_HIP;
+    do {
+        if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) break;
+        if (alloc_slots(s, s->S[0], capacity) || alloc_slots(s, s->S[1], capacity)) break;
+        Work& W = s->W;
+        if (s->alloc(&W.keys, capacity) || s->alloc(&W.keys_sorted, capacity) || s->alloc(&W.perm, capacity) ||
+            s->alloc(&W.force, capacity) || s->alloc(&W.fp, capacity) || s->alloc(&W.xs, capacity) ||
+            s->alloc(&W.p, capacity) || s->alloc(&W.rho, capacity) || s->alloc(&W.pres, capacity) ||
+            s->alloc(&W.ncount, capacity))
+            break;
+        if (s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors)) break;
+        s->ncells = p->grid_dim * p->grid_dim * p->grid_dim;
+        if (s->alloc(&W.cells, s->ncells)) break;
+        if (s->alloc(&s->luts, 5 * LUT_N)) break;
+        if (s->alloc(&s->ctl, 1)) break;
+        void* tmp = nullptr;
+        if (hipMalloc(&tmp, radix_sort_temp_bytes(capacity)) != hipSuccess) break;
+        s->sort_tmp = tmp;
+        s->allocs.push_back(tmp);
+        if (hipHostMalloc((void**)&s->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) break;
+        std::vector<float> L(5 * LUT_N);
+        build_luts(p->support_radius, L.data());
+        if (hipMemcpy(s->luts, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) break;
+        if (hipMemset(s->ctl, 0, sizeof(Ctl)) != hipSuccess) break;
+        rc = UPF_OK;
+    } while (0);
+    if (rc != UPF_OK) {
+        upf_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return UPF_OK;
+}
+
+int upf_destroy(upf_sim* s) {
+    if (!s) return UPF_ERR_ARG;
+    if (s->st) (void)hipStreamSynchronize(s->st);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    if (s->st) (void)hipStreamDestroy(s->st);
+    delete s;
+    return UPF_OK;
+}
+
+int upf_set_params(upf_sim* s, const upf_params* p) {
+    if (!s || !p) return UPF_ERR_ARG;
+    int rc = check_params(p);
+    if (rc) return rc;
+    if (p->grid_dim != s->P.grid_dim || p->block_size != s->P.block_size || p->max_neighbors != s->P.max_neighbors ||
+        p->support_radius != s->P.support_radius)
+        return UPF_ERR_ARG;
+    for (int a = 0; a < 3; ++a)
+        if (p->scales[a] != s->P.scales[a]) return UPF_ERR_ARG;
+    s->P = *p;
+    return UPF_OK;
+}
+
+int upf_upload_particles(upf_sim* s, int n, const float* pos4, const float* vel4, const int* type) {
+    if (!s || n <= 0 || n > s->cap || !pos4 || !type) return UPF_ERR_ARG;
+    std::vector<float> pos((size_t)4 * n), vel((size_t)4 * n, 0.0f), x0((size_t)4 * n);
+    std::vector<int> id(n), body(n, -1);
+    std::vector<float> rho(n, s->P.rest_density), zero(n, 0.0f), psi(n, s->P.initial_mass / s->P.rest_density);
+    for (int i = 0; i < n; ++i) {
+        const int t = type[i];
+        if (t != UPF_LIQUID && t != UPF_FROZEN && t != UPF_RIGID) return UPF_ERR_ARG;
+        for (int a = 0; a < 3; ++a) {
+            pos[4 * i + a] = pos4[4 * i + a];
+            x0[4 * i + a] = pos4[4 * i + a];
+            if (vel4) vel[4 * i + a] = vel4[4 * i + a];
+        }
+        int tb = t;
+        float w;
+        std::memcpy(&w, &tb, 4);
+        pos[4 * i + 3] = w;
+        x0[4 * i + 3] = w;
+        vel[4 * i + 3] = 0.0f;
+        id[i] = i;
+        uint32_t k;
+        if (!host_key(s->P, pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2], k)) return UPF_ERR_RANGE;
+        // the query box of every particle must stay inside the grid (beyond it the
+        // reference's cell ranges overflow, src/cpu/zIndex.cpp:66)
+        for (int a = 0; a < 3; ++a)
+            if (!(s->P.scales[a] * (pos4[4 * i + a] + s->P.support_radius) < 1024.0f)) return UPF_ERR_RANGE;
+    }
+    Slots& S = s->S[0];
+    s->cur = 0;
+    s->n = n;
+    const size_t b4 = (size_t)n * sizeof(float4), b1 = (size_t)n * sizeof(float);
+    HIPCHK(hipMemcpyAsync(S.pos, pos.data(), b4, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(S.vel, vel.data(), b4, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(S.x0, x0.data(), b4, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(S.id, id.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(S.body, body.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemsetAsync(S.rel, 0, b4, s->st));
+    HIPCHK(hipMemcpyAsync(S.p_last, zero.data(), b1, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(S.p_prev, zero.data(), b1, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(S.rho_last, rho.data(), b1, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(S.psi, psi.data(), b1, hipMemcpyHostToDevice, s->st));
+    // loop arrays of the "previous step" (gathered into p_last / rho_last by the first sort)
+    HIPCHK(hipMemcpyAsync(s->W.p, zero.data(), b1, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->W.rho, rho.data(), b1, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemsetAsync(s->ctl, 0, sizeof(Ctl), s->st));
+    hipLaunchKernelGGL(k_keys, grid_for(n), dim3(BLOCK), 0, s->st, n, S.pos, s->W.keys, s->P, s->ctl);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->uploaded = true;
+    s->psi_ready = false;
+    s->k_hist.clear();
+    s->rho_hist.clear();
+    return UPF_OK;
+}
+
+int upf_step_rigid(upf_sim* s);  // upf_rigid.hip
+
+int upf_step(upf_sim* s, int nsteps, int method) {
+    if (!s || nsteps < 0) return UPF_ERR_ARG;
+    if (!s->uploaded) return UPF_ERR_STATE;
+    if (method == UPF_PCISPH_RIGID && !s->rigid) return UPF_ERR_STATE;
+    for (int k = 0; k < nsteps; ++k) {
+        int rc;
+        if (method == UPF_PCISPH_STATIC) rc = step_pcisph_static(s);
+        else if (method == UPF_WCSPH_STATIC) rc = step_wcsph_static(s);
+        else if (method == UPF_PCISPH_RIGID) rc = upf_step_rigid(s);
+        else return UPF_ERR_ARG;
+        s->last_method = method;
+        if (rc) return rc;
+        const int e = err_code(s->h_ctl->err);
+        if (e) return e;
+    }
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->prof.collect();
+    return UPF_OK;
+}
+
+int upf_profile_enable(upf_sim* s, int on) {
+    if (!s) return UPF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->prof.collect();
+    s->prof.on = on != 0;
+    return UPF_OK;
+}
+int upf_profile_reset(upf_sim* s) {
+    if (!s) return UPF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->prof.collect();
+    for (int c = 0; c < PC_COUNT; ++c) { s->prof.ms[c] = 0.0; s->prof.cnt[c] = 0; }
+    return UPF_OK;
+}
+int upf_profile_classes(void) { return PC_COUNT; }
+const char* upf_profile_name(int cls) { return cls >= 0 && cls < PC_COUNT ? kProfNames[cls] : ""; }
+int upf_profile_read(upf_sim* s, int cls, double* total_ms, long long* launches) {
+    if (!s || cls < 0 || cls >= PC_COUNT) return UPF_ERR_ARG;
+    if (total_ms) *total_ms = s->prof.ms[cls];
+    if (launches) *launches = s->prof.cnt[cls];
+    return UPF_OK;
+}
+
+int upf_sort_and_bin(upf_sim* s) {
+    if (!s) return UPF_ERR_ARG;
+    if (!s->uploaded) return UPF_ERR_STATE;
+    int rc = sort_and_bin(s);
+    if (rc) return rc;
+    // the state is now in key order: its keys are the sorted ones
+    HIPCHK(hipMemcpyAsync(s->W.keys, s->W.keys_sorted, (size_t)s->n * 4, hipMemcpyDeviceToDevice, s->st));
+    // loop arrays follow the new order too (identity gather for the next sort)
+    Slots& S = s->S[s->cur];
+    HIPCHK(hipMemcpyAsync(s->W.p, S.p_last, (size_t)s->n * 4, hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(S.p_last, S.p_prev, (size_t)s->n * 4, hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->W.rho, S.rho_last, (size_t)s->n * 4, hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return UPF_OK;
+}
+
+int upf_download_sorted(upf_sim* s, uint32_t* keys, int32_t* ids, uint32_t* cells, int* n_cells) {
+    if (!s) return UPF_ERR_ARG;
+    const int n = s->n;
+    if (keys) HIPCHK(hipMemcpy(keys, s->W.keys_sorted, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (ids) HIPCHK(hipMemcpy(ids, s->S[s->cur].id, (size_t)n * 4, hipMemcpyDeviceToHost));
+    if (n_cells) {
+        std::vector<uint2> t(s->ncells);
+        HIPCHK(hipMemcpy(t.data(), s->W.cells, (size_t)s->ncells * sizeof(uint2), hipMemcpyDeviceToHost));
+        std::vector<uint32_t> codes;
+        for (uint32_t c = 0; c < s->ncells; ++c)
+            if (t[c].y > t[c].x) codes.push_back(c);
+        std::sort(codes.begin(), codes.end(), [&](uint32_t a, uint32_t b) { return t[a].x < t[b].x; });
+        *n_cells = (int)codes.size();
+        if (cells)
+            for (size_t k = 0; k < codes.size(); ++k) {
+                cells[3 * k] = codes[k];
+                cells[3 * k + 1] = t[codes[k]].x;
+                cells[3 * k + 2] = t[codes[k]].y;
+            }
+    }
+    return UPF_OK;
+}
+
+int upf_download(upf_sim* s, float* pos4, float* vel4, float* rho, float* p, float* force4, float* psi, int* type) {
+    if (!s) return UPF_ERR_ARG;
+    if (!s->uploaded) return UPF_ERR_STATE;
+    const int n = s->n;
+    Slots& S = s->S[s->cur];
+    HIPCHK(hipStreamSynchronize(s->st));
+    std::vector<int> id(n);
+    HIPCHK(hipMemcpy(id.data(), S.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+    std::vector<float> b4((size_t)4 * n), b1(n);
+    auto get4 = [&](const float4* src, float* dst, bool is_pos) -> int {
+        if (!dst) return UPF_OK;
+        HIPCHK(hipMemcpy(b4.data(), src, (size_t)n * 16, hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; ++k) {
+            const int d = id[k];
+            for (int a = 0; a < 3; ++a) dst[4 * d + a] = b4[4 * k + a];
+            dst[4 * d + 3] = 0.0f;
+            if (is_pos && type) {
+                int t;
+                std::memcpy(&t, &b4[4 * k + 3], 4);
+                type[d] = t;
+            }
+        }
+        return UPF_OK;
+    };
+    auto get1 = [&](const float* src, float* dst) -> int {
+        if (!dst) return UPF_OK;
+        HIPCHK(hipMemcpy(b1.data(), src, (size_t)n * 4, hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; ++k) dst[id[k]] = b1[k];
+        return UPF_OK;
+    };
+    // after a step the slot arrays of the loop (W.*) are in the current slot order
+    std::vector<float> tmp4;
+    if (pos4 || type) {
+        if (!pos4) { tmp4.resize((size_t)4 * n); pos4 = tmp4.data(); }
+        if (get4(S.pos, pos4, true)) return UPF_ERR_HIP;
+    }
+    if (get4(S.vel, vel4, false)) return UPF_ERR_HIP;
+    if (get4(s->W.force, force4, false)) return UPF_ERR_HIP;
+    if (get1(s->W.rho, rho)) return UPF_ERR_HIP;
+    if (get1(s->last_method == UPF_WCSPH_STATIC ? s->W.pres : s->W.p, p)) return UPF_ERR_HIP;
+    if (get1(S.psi, psi)) return UPF_ERR_HIP;
+    return UPF_OK;
+}
+
+int upf_get_step_stats(upf_sim* s, int max_steps, int* iterations, float* max_rho) {
+    if (!s) return UPF_ERR_ARG;
+    const int m = (int)s->k_hist.size();
+    for (int i = 0; i < m && i < max_steps; ++i) {
+        if (iterations) iterations[i] = s->k_hist[i];
+        if (max_rho) max_rho[i] = s->rho_hist[i];
+    }
+    return m;
+}
+
+int upf_keys(const upf_params* p, int n, const float* pos4, uint32_t* keys) {
+    if (!p || n <= 0 || !pos4 || !keys) return UPF_ERR_ARG;
+    float4* d_pos = nullptr;
+    uint32_t* d_keys = nullptr;
+    Ctl* ctl = nullptr;
+    int rc = UPF_ERR_HIP;
+    do {
+        if (hipMalloc((void**)&d_pos, (size_t)n * 16) != hipSuccess) break;
+        if (hipMalloc((void**)&d_keys, (size_t)n * 4) != hipSuccess) break;
+        if (hipMalloc((void**)&ctl, sizeof(Ctl)) != hipSuccess) break;
+        if (hipMemset(ctl, 0, sizeof(Ctl)) != hipSuccess) break;
+        if (hipMemcpy(d_pos, pos4, (size_t)n * 16, hipMemcpyHostToDevice) != hipSuccess) break;
+        hipLaunchKernelGGL(k_keys, grid_for(n), dim3(BLOCK), 0, 0, n, d_pos, d_keys, *p, ctl);
+        if (hipGetLastError() != hipSuccess) break;
+        if (hipMemcpy(keys, d_keys, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) break;
+        Ctl h;
+        if (hipMemcpy(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost) != hipSuccess) break;
+        rc = err_code(h.err);
+    } while (0);
+    (void)hipFree(d_pos);
+    (void)hipFree(d_keys);
+    (void)hipFree(ctl);
+    return rc;
+}
+
+int upf_radix_sort(int n, const uint32_t* keys_in, uint32_t* keys_out, int32_t* perm_out) {
+    if (n <= 0 || !keys_in || !keys_out || !perm_out) return UPF_ERR_ARG;
+    uint32_t *dk = nullptr, *dko = nullptr, *dv = nullptr;
+    void* tmp = nullptr;
+    int rc = UPF_ERR_HIP;
+    do {
+        if (hipMalloc((void**)&dk, (size_t)n * 4) != hipSuccess) break;
+        if (hipMalloc((void**)&dko, (size_t)n * 4) != hipSuccess) break;
+        if (hipMalloc((void**)&dv, (size_t)n * 4) != hipSuccess) break;
+        if (hipMalloc(&tmp, radix_sort_temp_bytes(n)) != hipSuccess) break;
+        if (hipMemcpy(dk, keys_in, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess) break;
+        if (radix_sort(dk, nullptr, dko, dv, n, 32, tmp, 0) != hipSuccess) break;
+        if (hipMemcpy(keys_out, dko, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) break;
+        if (hipMemcpy(perm_out, dv, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) break;
+        rc = UPF_OK;
+    } while (0);
+    (void)hipFree(dk);
+    (void)hipFree(dko);
+    (void)hipFree(dv);
+    (void)hipFree(tmp);
+    return rc;
+}
+
+// Host-side mirror of the query-box enumeration, for tests: the cells (Morton
+// codes of cells inside the grid) visited for a particle at (x, y, z), in
+// visiting order.  Returns their number.
+int upf_debug_box_cells(const upf_params* p, float x, float y, float z, uint32_t* out, int cap) {
+    if (!p) return -1;
+    BoxCells bc;
+    bc.init(x, y, z, *p);
+    int m = 0;
+    for (int e = 0; e < bc.count(); ++e) {
+        uint32_t c[3];
+        if (!bc.cell(e, c)) continue;
+        if (c[0] >= p->grid_dim || c[1] >= p->grid_dim || c[2] >= p->grid_dim) continue;
+        if (m < cap) out[m] = dmorton3(c[0], c[1], c[2]);
+        ++m;
+    }
+    return m;
+}
+
+}  // extern "C"
+
+// rigid-body step lives in upf_rigid.hip; until it is built this stub keeps
+// the method unavailable.
+__attribute__((weak)) int upf_step_rigid(upf_sim*) { return UPF_ERR_ARG; }
