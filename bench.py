@@ -65,11 +65,14 @@ def run_gpu(args):
     L.upf_profile_reset.argtypes = [C.c_void_p]
     L.upf_profile_enable(sim.h, 1)
     L.upf_profile_reset(sim.h)
-    import torch
-    torch.cuda.synchronize()
+    # The library links the system HIP runtime; torch bundles its own, and the
+    # two do not share one process, so the device is synchronised through the
+    # library's runtime (upf_step also returns only after its stream drained).
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
     sim.step(args.steps, upf.UPF_PCISPH_STATIC)
-    torch.cuda.synchronize()
+    hip.hipDeviceSynchronize()
     dt = time.perf_counter() - t0
     K, rmax = sim.stats()
     Kt = K[-args.steps:]

@@ -1143,3 +1143,40 @@ extern "C" int orc_num_threads(void) {
 }
 
 
+
+// One rigid-body update of CalculateRigidBodyForceTorqueMomenta for a body
+// made of n particles (positions, forces, masses), in the state layout of
+// oracle/ref_driver.cpp: pos[3] vel[3] angvel[3] P[3] L[3] q[4] R[9]
+// Iinv_local[9] Iinv_world[9] mass.
+extern "C" int orc_rigid_step(float* st, int n, const float* ppos, const float* pforce, const float* pmass,
+                              float dt, float g, float terminal_speed, int renorm) {
+    orc::Sim<float> s;
+    std::memset(&s.P, 0, sizeof(s.P));
+    s.P.dt = dt;
+    s.P.gravity = g;
+    s.P.terminal_speed = terminal_speed;
+    s.P.renormalize_quaternion = renorm;
+    s.ps.resize(n);
+    orc::Body b;
+    for (int i = 0; i < n; ++i) {
+        s.ps[i] = orc::Particle<float>();
+        s.ps[i].pos = orc::V3<float>(ppos[3 * i], ppos[3 * i + 1], ppos[3 * i + 2]);
+        s.ps[i].force = orc::V3<float>(pforce[3 * i], pforce[3 * i + 1], pforce[3 * i + 2]);
+        s.ps[i].mass = pmass[i];
+        b.idx.push_back(i);
+    }
+    for (int a = 0; a < 3; ++a) {
+        b.pos[a] = st[a]; b.vel[a] = st[3 + a]; b.ang_vel[a] = st[6 + a]; b.lin_mom[a] = st[9 + a];
+        b.ang_mom[a] = st[12 + a]; b.force[a] = 0.0f; b.torque[a] = 0.0f;
+    }
+    for (int a = 0; a < 4; ++a) b.q[a] = st[15 + a];
+    for (int a = 0; a < 9; ++a) { b.R[a] = st[19 + a]; b.Iinv_local[a] = st[28 + a]; b.Iinv_world[a] = st[37 + a]; }
+    b.mass = st[46];
+    s.rigid_integrate(b);
+    for (int a = 0; a < 3; ++a) {
+        st[a] = b.pos[a]; st[3 + a] = b.vel[a]; st[6 + a] = b.ang_vel[a]; st[9 + a] = b.lin_mom[a]; st[12 + a] = b.ang_mom[a];
+    }
+    for (int a = 0; a < 4; ++a) st[15 + a] = b.q[a];
+    for (int a = 0; a < 9; ++a) { st[19 + a] = b.R[a]; st[37 + a] = b.Iinv_world[a]; }
+    return 0;
+}

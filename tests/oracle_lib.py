@@ -40,6 +40,7 @@ def _bind(lib):
     lib.orc_sim_download_bodies.argtypes = [C.c_void_p, C.POINTER(Body)]
     lib.orc_sim_stats.argtypes = [C.c_void_p, C.c_int, pi, pf]
     lib.orc_sim_neighbors.argtypes = [C.c_void_p, pi, pi, C.c_int]
+    lib.orc_rigid_step.argtypes = [pf, C.c_int, pf, pf, pf, f, f, f, C.c_int]
     return lib
 
 

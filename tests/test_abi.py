@@ -1,0 +1,78 @@
+"""The C ABI library loads on a machine without a GPU and exports every
+entry point include/upf.h declares; host-only helpers work without a device."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+import unifiedparticleframeworkcuda_amd as upf
+from unifiedparticleframeworkcuda_amd._abi import Params, DamBreakDesc, RigidSceneDesc, Body
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:int|const char\*|void)\s+(upf_\w+)\s*\(", text, re.M))
+    return sorted(names)
+
+
+def test_every_declared_symbol_is_exported():
+    lib = upf.lib()
+    names = declared_functions()
+    assert len(names) > 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_sizes_match_header_layout():
+    # ctypes mirror vs the C layout: field count x 4 bytes plus alignment of the u64 seed
+    n_arr = sum(1 for _, t in Params._fields_ if hasattr(t, "_length_"))
+    assert C.sizeof(Params) == 4 * (len(Params._fields_) - n_arr) + 12 * n_arr
+    assert C.sizeof(Body) % 4 == 0
+    assert DamBreakDesc.seed.offset % 8 == 0 and RigidSceneDesc.seed.offset % 8 == 0
+
+
+def test_version_string():
+    assert b"gfx950" in upf.lib().upf_version()
+
+
+def test_host_helpers_without_device():
+    p = upf.make_params(0.01, 0.001)
+    assert abs(p.support_radius - 0.0200002) < 1e-7
+    pos, typ, nf = upf.dam_break(p, 8, 8, 8, (0.16, 0.08, 0.08))
+    assert nf == 512 and np.sum(typ == upf.UPF_LIQUID) == 512
+    assert p.block_size >= 2 and p.block_size >= max(p.scales) * p.support_radius
+    f = typ == upf.UPF_LIQUID
+    # fluid inside the collision box (up to the jitter), walls outside it
+    jit = 0.001 * 0.01 + 1e-7
+    assert np.all(pos[f, :3] >= np.array(p.collision_min) - jit)
+    assert np.all(pos[f, :3] <= np.array(p.collision_max) + jit)
+    w = ~f
+    inside = np.all((pos[w, :3] > np.array(p.collision_min)) & (pos[w, :3] < np.array(p.collision_max)), axis=1)
+    assert not inside.any()
+    d, mx = upf.density_error_factor(p)
+    assert d > 0 and mx == 32
+    L = upf.luts(p)
+    assert L.shape == (5, 301) and L[0, 300] == 0.0
+
+
+def test_set_domain_rejects_cells_smaller_than_h():
+    p = upf.make_params(0.01)
+    # scales * h = 1024 * h / E just above a power of two -> block smaller than h
+    E = 1024 * p.support_radius / 16.5
+    vmax = (C.c_float * 3)(E, E, E)
+    cmin = (C.c_float * 3)(0.0, 0.0, 0.0)
+    cmax = (C.c_float * 3)(E * 0.5, E * 0.5, E * 0.5)
+    rc = upf.lib().upf_params_set_domain(C.byref(p), vmax, cmin, cmax)
+    assert rc == upf._abi.UPF_ERR_ARG
+
+
+def test_density_error_factor_rejects_reference_ub_spacing():
+    p = upf.make_params(0.0125)
+    rc = upf.lib().upf_compute_density_error_factor(C.byref(p), None)
+    assert rc == upf._abi.UPF_ERR_ARG
