@@ -42,9 +42,10 @@ def class_bytes(name, n, nf, ncells):
         "radix_sort": 12 * n + 20 * n,          # key read + (key, perm) write, keys from integration
         "gather": 76 * n,
         "cell_table": 4 * n + 8 * ncells,
-        "neighbours_external_force": 40 * n + 36 * nf + 80 * nf,  # + the fused first prediction
+        "neighbour_lists": 20 * n + 4 * n,      # positions + keys read, counts written
+        "external_force_predict": 40 * n + 36 * nf + 80 * nf,  # + the fused first prediction
         "predicted_density": 16 * n + 12 * nf,
-        "pressure_force_predict": 20 * n + 16 * nf + 80 * nf,     # + the fused next prediction
+        "pressure_force_predict": 20 * n + 16 * nf + 80 * nf,  # + the fused next prediction
         "integrate": 96 * nf,
     }
     return table.get(name)

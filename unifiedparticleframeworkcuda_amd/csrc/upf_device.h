@@ -57,21 +57,24 @@ __host__ __device__ inline int dmsb(uint32_t v) {  // index of the highest set b
 // Per axis the box covers n = 1..3 consecutive cell coordinates.  Two cells
 // compare in Morton order by the highest interleaved bit in which they differ;
 // among n consecutive coordinates that bit can only be one of <= 2 "split"
-// levels per axis.  Sorting the <= 6 splits by (3 * level + axis) and counting
+// levels per axis.  Ranking the <= 6 splits by (3 * level + axis) and counting
 // through them as a binary number enumerates the cells in Morton order.
+// Every array index below is a compile-time constant after unrolling, so the
+// state lives in registers.
 struct BoxCells {
     uint32_t lo[3];
     int n[3];
-    int nd;                 // number of splits
-    int dax[6], dkind[6];   // axis, kind (0: n==2 split, 1: high split, 2: low split)
-    int hcase[3];           // for n==3: 0 -> high split separates {v0}|{v1,v2}; 1 -> {v0,v1}|{v2}
-    int pos_of[3][3];       // bit position (0 = most significant) of an axis' split kinds
+    int hcase[3];   // n == 3: 0 -> the high split separates {v0}|{v1,v2}; 1 -> {v0,v1}|{v2}
+    int shA[3];     // shift of the axis' first split bit in the enumeration index
+    int shB[3];     // shift of its second split bit (n == 3 only)
+    int nd;         // number of splits
 
     __host__ __device__ void init(float x, float y, float z, const upf_params& P) {
         const float h = P.support_radius;
         const float c[3] = {x, y, z};
-        int prio[6];
-        nd = 0;
+        uint32_t l3[3];
+        int n3[3];
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
             float lo_f = P.scales[a] * (c[a] - h);
             float hi_f = P.scales[a] * (c[a] + h);
@@ -79,30 +82,44 @@ struct BoxCells {
             hi_f = hi_f <= 1024.0f ? hi_f : 1024.0f; // MIN(max_query_box, GRID_RESOLUTION)
             const uint32_t l = ((uint32_t)(unsigned short)lo_f) >> P.block_shift;
             const uint32_t u = ((uint32_t)(unsigned short)hi_f) >> P.block_shift;
+            l3[a] = l;
+            n3[a] = (int)(u - l) + 1;
+        }
+        init_cells(l3, n3);
+    }
+
+    // the cells lo[a] .. lo[a] + n[a] - 1 per axis (n[a] in 1..3)
+    __host__ __device__ void init_cells(const uint32_t l3[3], const int n3[3]) {
+        int prA[3], prB[3];
+        bool vA[3], vB[3];
+        nd = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const uint32_t l = l3[a];
+            int cnt = n3[a];
+            cnt = cnt > 3 ? 3 : (cnt < 1 ? 1 : cnt);
             lo[a] = l;
-            n[a] = (int)(u - l) + 1;
-            hcase[a] = 0;
-            if (n[a] == 2) {
-                dax[nd] = a; dkind[nd] = 0; prio[nd] = 3 * dmsb(l ^ (l + 1)) + a; ++nd;
-            } else if (n[a] >= 3) {
-                n[a] = 3;
-                const int m01 = dmsb(l ^ (l + 1)), m12 = dmsb((l + 1) ^ (l + 2));
-                hcase[a] = m01 > m12 ? 0 : 1;
-                const int hi = m01 > m12 ? m01 : m12, low = m01 > m12 ? m12 : m01;
-                dax[nd] = a; dkind[nd] = 1; prio[nd] = 3 * hi + a; ++nd;
-                dax[nd] = a; dkind[nd] = 2; prio[nd] = 3 * low + a; ++nd;
-            }
+            n[a] = cnt;
+            const int m01 = dmsb(l ^ (l + 1)), m12 = dmsb((l + 1) ^ (l + 2));
+            hcase[a] = m01 > m12 ? 0 : 1;
+            vA[a] = cnt >= 2;
+            vB[a] = cnt == 3;
+            prA[a] = 3 * (cnt == 2 ? m01 : (m01 > m12 ? m01 : m12)) + a;
+            prB[a] = 3 * (m01 > m12 ? m12 : m01) + a;
+            nd += (int)vA[a] + (int)vB[a];
         }
-        // sort splits by priority, descending (insertion sort, <= 6 entries)
-        for (int i = 1; i < nd; ++i) {
-            for (int j = i; j > 0 && prio[j] > prio[j - 1]; --j) {
-                int t = prio[j]; prio[j] = prio[j - 1]; prio[j - 1] = t;
-                t = dax[j]; dax[j] = dax[j - 1]; dax[j - 1] = t;
-                t = dkind[j]; dkind[j] = dkind[j - 1]; dkind[j - 1] = t;
+        // rank of a split = number of valid splits with a higher priority
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            int rA = 0, rB = 0;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                rA += (vA[b] && prA[b] > prA[a]) + (vB[b] && prB[b] > prA[a]);
+                rB += (vA[b] && prA[b] > prB[a]) + (vB[b] && prB[b] > prB[a]);
             }
+            shA[a] = nd - 1 - rA;
+            shB[a] = nd - 1 - rB;
         }
-        for (int a = 0; a < 3; ++a) pos_of[a][0] = pos_of[a][1] = pos_of[a][2] = -1;
-        for (int k = 0; k < nd; ++k) pos_of[dax[k]][dkind[k]] = k;
     }
 
     __host__ __device__ int count() const { return 1 << nd; }
@@ -110,24 +127,26 @@ struct BoxCells {
     // cell of enumeration index e (0 .. count()-1); false for indices that
     // do not name a cell (duplicates of the n == 3 singleton group)
     __host__ __device__ bool cell(int e, uint32_t out[3]) const {
+        bool ok = true;
+#pragma unroll
         for (int a = 0; a < 3; ++a) {
             int off = 0;
             if (n[a] == 2) {
-                off = (e >> (nd - 1 - pos_of[a][0])) & 1;
+                off = (e >> shA[a]) & 1;
             } else if (n[a] == 3) {
-                const int hb = (e >> (nd - 1 - pos_of[a][1])) & 1;
-                const int lb = (e >> (nd - 1 - pos_of[a][2])) & 1;
+                const int hb = (e >> shA[a]) & 1;
+                const int lb = (e >> shB[a]) & 1;
                 if (hcase[a] == 0) {
-                    if (hb == 0) { if (lb) return false; off = 0; }
-                    else off = 1 + lb;
+                    ok = ok && !(hb == 0 && lb);
+                    off = hb ? 1 + lb : 0;
                 } else {
-                    if (hb == 1) { if (lb) return false; off = 2; }
-                    else off = lb;
+                    ok = ok && !(hb == 1 && lb);
+                    off = hb ? 2 : lb;
                 }
             }
             out[a] = lo[a] + (uint32_t)off;
         }
-        return true;
+        return ok;
     }
 };
 

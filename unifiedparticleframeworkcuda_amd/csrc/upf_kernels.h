@@ -14,9 +14,11 @@ constexpr int LUT_N = 301;
 constexpr int BLOCK = 256;
 
 // Loop control of the PCISPH correction loop, kept on the device.
+constexpr int MAX_SLOTS = 64;  // the max reduction spreads its atomics over 64 cache lines
+
 struct Ctl {
     uint32_t max_bits;   // bit pattern of the max predicted density (non-negative floats order like uints)
-    uint32_t done;       // workgroups finished in the current density pass
+    uint32_t done;       // unused (kept for layout)
     int want;            // number of iterations the loop must run (grows as the loop decides)
     int too_large;       // density_error_too_large_ latch (src/cpu/UnifiedPhysics.cpp:5373,5397-5398)
     float last_max;      // max predicted density of the last completed iteration
@@ -47,6 +49,8 @@ struct Work {           // per-step arrays, slot order
     float4* fp;         // correction pressure force
     float4* xs;         // predicted position
     float* p;           // correction pressure of the running loop
+    float* pdiv;        // p / (rho0 * rho0), the neighbour term of the pressure force
+    uint32_t* max_slots; // MAX_SLOTS x 32 uints: per-slot max of the predicted density (bits)
     float* rho;         // predicted density (PCISPH) / density (WCSPH)
     float* pres;        // WCSPH pressure
     uint32_t* nbr;      // neighbour lists, column-major: nbr[k * stride + slot]

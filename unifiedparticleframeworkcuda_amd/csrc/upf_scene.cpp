@@ -72,13 +72,17 @@ int upf_scene_dam_break(const upf_dam_break_desc* d, upf_params* p, int* n_fluid
     const double o = pad - r;
     const double lx = box[0] + 2.0 * r, ly = box[1] + 2.0 * r, lz = box[2] + 2.0 * r;
     // wall lattice: floor (y = o), sides x = o, x = o+lx, z = o, z = o+lz; open top
-    const int wx = (int)std::floor(lx / ws + 0.5), wy = (int)std::floor(ly / ws + 0.5),
-              wz = (int)std::floor(lz / ws + 0.5);
+    // per axis the wall grid spans the wall extent exactly (the sampling step
+    // is stretched to a whole number of steps), so the far walls keep the
+    // same gap to the fluid as the near ones
+    const int wx = std::max(1, (int)std::floor(lx / ws + 0.5)), wy = std::max(1, (int)std::floor(ly / ws + 0.5)),
+              wz = std::max(1, (int)std::floor(lz / ws + 0.5));
+    const double sx = lx / wx, sy = ly / wy, sz = lz / wz;
     std::vector<float> walls;
     auto add_wall = [&](int i, int j, int k) {
-        walls.push_back((float)(o + i * ws));
-        walls.push_back((float)(o + j * ws));
-        walls.push_back((float)(o + k * ws));
+        walls.push_back((float)(o + i * sx));
+        walls.push_back((float)(o + j * sy));
+        walls.push_back((float)(o + k * sz));
     };
     for (int i = 0; i <= wx; ++i)
         for (int k = 0; k <= wz; ++k) add_wall(i, 0, k);           // floor

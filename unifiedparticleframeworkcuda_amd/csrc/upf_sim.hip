@@ -91,7 +91,7 @@ __global__ void k_cell_bounds(int n, const uint32_t* __restrict__ keys, int shif
 // Visit the candidates of the reference's query box around x in ascending
 // slot order; f(j) is called for every j in the covered cells.
 template <class F>
-__device__ inline void for_each_candidate(const float4& x, const upf_params& P, const uint2* __restrict__ cells, F f) {
+__device__ __forceinline__ void for_each_candidate(const float4& x, const upf_params& P, const uint2* __restrict__ cells, F f) {
     BoxCells bc;
     bc.init(x.x, x.y, x.z, P);
     const int cnt = bc.count();
@@ -118,7 +118,7 @@ __global__ void k_psi(int n, const float4* __restrict__ pos, const uint2* __rest
     if (ti != UPF_FROZEN && ti != UPF_RIGID) return;
     const float T = P.support_radius_sq_limit;
     float inv = 0.0f;
-    for_each_candidate(xi, P, cells, [&](int j) {
+    for_each_candidate(xi, P, cells, [&](int j) __attribute__((always_inline)) {
         if (j == i) return;
         const float4 xj = pos[j];
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -161,14 +161,173 @@ __device__ inline float4 predict(const float4& x, const float4& v, const float4&
     return make_float4(px, py, pz, x.w);
 }
 
-// ---------------------------------------------- neighbours + external force
-// GetNeighbors (:2242-2295) + CalculateExternalForcesStaticBoundariesPCISPH
-// (:5198-5250) + the first PredictPositionAndVelocity of the loop.
-// Liquid slots build their neighbour list (ascending slots), accumulate the
-// external force over it and predict; every slot resets its loop pressure.
+// ------------------------------------------------------- neighbour lists
+// Entries pack the neighbour slot and the kernel-table bin of the pair's
+// distance at the positions of the step: (slot << 9) | bin.  The bin is the
+// index of kernel*Lut(dist) (src/cpu/WeightingKernel.cpp:122-127), 300 when the
+// table returns 0; the passes that work on the step's positions read their
+// kernel values from it without recomputing the distance.
+constexpr int NB_SLOT_SHIFT = 9;
+constexpr uint32_t NB_BIN_MASK = 511u;
+constexpr int NB_ROUNDS = 4;              // candidates per lane per chunk
+constexpr int NB_CHUNK = 64 * NB_ROUNDS;
+
+__device__ inline uint32_t compact10(uint32_t v) {  // inverse of dspread10
+    v &= 0x09249249u;
+    v = (v ^ (v >> 2)) & 0x030C30C3u;
+    v = (v ^ (v >> 4)) & 0x0300F00Fu;
+    v = (v ^ (v >> 8)) & 0x030000FFu;
+    v = (v ^ (v >> 16)) & 0x000003FFu;
+    return v;
+}
+
+__device__ inline uint32_t lut_bin(float dist, float factor) {
+    const int idx = (int)(dist * factor * 300.0f);
+    return idx >= 300 ? 300u : (uint32_t)idx;
+}
+
+// GetNeighbors (src/cpu/UnifiedPhysics.cpp:2242-2295) for every liquid slot,
+// one wave per 64 consecutive slots.  For each cell met in its slots the wave
+// lists the occupied cells of the 3x3x3 stencil in Morton order (so that the
+// candidates come in ascending slot order), then tests the candidates 64 at
+// a time against each particle of the cell: the candidate's cell must lie in
+// the particle's query box (the reference's BoxQuery) and its distance must be
+// below h.  Ballots keep each list in ascending order.
+__global__ __launch_bounds__(64) void k_nbr_build(int n, const float4* __restrict__ pos,
+                                                  const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
+                                                  upf_params P, float lut_factor, uint32_t* __restrict__ nbr, int stride,
+                                                  int* __restrict__ ncount, Ctl* ctl) {
+    __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
+    __shared__ int pcnt[64];
+    const int lane = threadIdx.x;
+    const int base = blockIdx.x * 64;
+    if (base >= n) return;
+    const int wend = min(base + 64, n);
+    const uint32_t shift3 = 3u * P.block_shift;
+    const uint32_t G = P.grid_dim;
+    const float T = P.support_radius_sq_limit;
+    const float h = P.support_radius;
+    const int cap = P.max_neighbors;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    pcnt[lane] = 0;
+    int s = base;
+    while (s < wend) {
+        const uint32_t c = keys[s] >> shift3;
+        const uint2 cr = cells[c];
+        const int pe = min((int)cr.y, wend);
+        const uint32_t cc[3] = {compact10(c), compact10(c >> 1), compact10(c >> 2)};
+        // stencil cells in Morton order: lane e names enumeration index e
+        BoxCells bc;
+        {
+            // a degenerate "box" whose cells are the clipped 3x3x3 stencil
+            int nn[3];
+            uint32_t lo[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = cc[a] > 0 ? cc[a] - 1 : 0;
+                const uint32_t hi = cc[a] + 1 < G ? cc[a] + 1 : G - 1;
+                nn[a] = (int)(hi - lo[a]) + 1;
+            }
+            bc.init_cells(lo, nn);
+        }
+        uint32_t rs = 0, rl = 0, rcell = 0;
+        bool valid = false;
+        if (lane < bc.count()) {
+            uint32_t q[3];
+            if (bc.cell(lane, q)) {
+                const uint2 r = cells[dmorton3(q[0], q[1], q[2])];
+                valid = r.y > r.x;
+                rs = r.x;
+                rl = r.y - r.x;
+                rcell = q[0] | (q[1] << 10) | (q[2] << 20);
+            }
+        }
+        const uint64_t vm = __ballot(valid);
+        const int nr = __popcll(vm);
+        const int rank = __popcll(vm & lt);
+        if (valid) { r_start[rank] = rs; r_cell[rank] = rcell; r_pref[rank + 1] = rl; }
+        if (lane == 0) r_pref[0] = 0;
+        __syncthreads();
+        if (lane == 0)
+            for (int k = 1; k <= nr; ++k) r_pref[k] += r_pref[k - 1];
+        __syncthreads();
+        const int M = (int)r_pref[nr];
+        for (int c0 = 0; c0 < M; c0 += NB_CHUNK) {
+            // this lane's candidates of the chunk
+            float4 cx[NB_ROUNDS];
+            int cj[NB_ROUNDS];
+            uint32_t ccell[NB_ROUNDS];
+            int k = 0;  // range of the lane's candidate; rises with the round
+#pragma unroll
+            for (int r = 0; r < NB_ROUNDS; ++r) {
+                const int ci = c0 + r * 64 + lane;
+                cj[r] = -1;
+                if (ci < M) {
+                    while (k + 1 < nr && (int)r_pref[k + 1] <= ci) ++k;
+                    cj[r] = (int)r_start[k] + (ci - (int)r_pref[k]);
+                    ccell[r] = r_cell[k];
+                    cx[r] = pos[cj[r]];
+                }
+            }
+            for (int p = s; p < pe; ++p) {
+                const float4 xp = pos[p];
+                if (ptype(xp) != UPF_LIQUID) continue;
+                // the particle's query box in cell coordinates
+                uint32_t blo[3], bhi[3];
+                const float xv[3] = {xp.x, xp.y, xp.z};
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    float lo_f = P.scales[a] * (xv[a] - h);
+                    float hi_f = P.scales[a] * (xv[a] + h);
+                    lo_f = lo_f >= 0.0f ? lo_f : 0.0f;
+                    hi_f = hi_f <= 1024.0f ? hi_f : 1024.0f;
+                    blo[a] = ((uint32_t)(unsigned short)lo_f) >> P.block_shift;
+                    bhi[a] = ((uint32_t)(unsigned short)hi_f) >> P.block_shift;
+                }
+                int cnt = pcnt[p - base];
+#pragma unroll
+                for (int r = 0; r < NB_ROUNDS; ++r) {
+                    bool pass = false;
+                    float d2 = 0.0f;
+                    if (cj[r] >= 0 && cj[r] != p) {
+                        const uint32_t qx = ccell[r] & 1023u, qy = (ccell[r] >> 10) & 1023u, qz = ccell[r] >> 20;
+                        const bool inbox = qx >= blo[0] && qx <= bhi[0] && qy >= blo[1] && qy <= bhi[1] &&
+                                           qz >= blo[2] && qz <= bhi[2];
+                        const float dx = xp.x - cx[r].x, dy = xp.y - cx[r].y, dz = xp.z - cx[r].z;
+                        d2 = dx * dx + dy * dy + dz * dz;
+                        pass = inbox && d2 < T;
+                    }
+                    const uint64_t m = __ballot(pass);
+                    if (pass) {
+                        const int slot = cnt + __popcll(m & lt);
+                        if (slot < cap)
+                            nbr[(size_t)slot * stride + p] =
+                                ((uint32_t)cj[r] << NB_SLOT_SHIFT) | lut_bin(sqrtf(d2), lut_factor);
+                    }
+                    cnt += __popcll(m);
+                }
+                if (lane == 0) pcnt[p - base] = cnt;
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+        s = pe;
+    }
+    __syncthreads();
+    const int i = base + lane;
+    if (i < wend) {
+        const int cnt = pcnt[lane];
+        ncount[i] = cnt < cap ? cnt : cap;
+        if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
+    }
+}
+
+// CalculateExternalForcesStaticBoundariesPCISPH (:5198-5250) over the lists +
+// the first PredictPositionAndVelocity of the loop; every slot resets its
+// loop pressure.
 template <int MODE>
-__global__ __launch_bounds__(BLOCK) void k_nbr_ext_static(StepArgs A, Slots S, Work W, const uint2* __restrict__ cells,
-                                                          const float* __restrict__ luts, Ctl* ctl) {
+__global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                      Ctl* ctl) {
     __shared__ float lvl[LUT_N], lpg[LUT_N];
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
         lvl[t] = luts[2 * LUT_N + t];
@@ -178,8 +337,6 @@ __global__ __launch_bounds__(BLOCK) void k_nbr_ext_static(StepArgs A, Slots S, W
     const upf_params& P = A.P;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->max_bits = 0u;
-        ctl->done = 0u;
         ctl->too_large = 1;
         ctl->want = (0 < P.min_loops || 0 < P.max_loops) ? 1 : 0;
         ctl->last_max = 0.0f;
@@ -188,49 +345,49 @@ __global__ __launch_bounds__(BLOCK) void k_nbr_ext_static(StepArgs A, Slots S, W
     const float4 xi = S.pos[i];
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
+    W.pdiv[i] = 0.0f;
     W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     W.rho[i] = S.rho_last[i];
     if (ti != UPF_LIQUID) {
         W.xs[i] = (ti == UPF_RIGID && !P.rigid_predicted_current) ? S.x0[i] : xi;
-        W.ncount[i] = 0;
         W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
     const float4 vi = S.vel[i];
     const float h = P.support_radius;
-    const float T = P.support_radius_sq_limit;
     const float rv = A.rest_vol, visc = P.visc_const;
     const float own_p = MODE == UPF_BOUNDARY_CPU_REF ? S.p_prev[i] : S.p_last[i];
-    const float own_rho = S.rho_last[i];
+    const float inv = 1.0f / S.rho_last[i];
+    const float pg = own_p * inv * inv;
     const float own_psi = S.psi[i];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    int cnt = 0;
-    const int cap = P.max_neighbors;
-    for_each_candidate(xi, P, cells, [&](int j) {
-        if (j == i) return;
-        const float4 xj = S.pos[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < T)) return;
-        if (cnt < cap) W.nbr[(size_t)cnt * A.stride + i] = (uint32_t)j;
-        ++cnt;
-        const float dist = sqrtf(d2);
+    const int cnt = W.ncount[i];
+    const uint32_t* L = W.nbr + i;
+    const size_t st = (size_t)A.stride;
+    auto pair = [&](uint32_t e, const float4& xj, const float4& vj) __attribute__((always_inline)) {
         const int tj = ptype(xj);
         if (tj == UPF_LIQUID) {
-            const float kv = lut_at(lvl, dist, A.lut_factor);
-            const float4 vj = S.vel[j];
+            const float kv = lvl[e & NB_BIN_MASK];
             const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
             fx = fx - vx * rv * rv * visc * kv;
             fy = fy - vy * rv * rv * visc * kv;
             fz = fz - vz * rv * rv * visc * kv;
         } else if (tj == UPF_FROZEN) {
-            // CalculateBoundaryFluidPressureForceHost (:4240-4250)
-            uint32_t dl = (uint32_t)(dist * 300.0f / h);
-            dl = dl > 300u ? 300u : dl;
-            const float g = MODE == UPF_BOUNDARY_CPU_REF ? lut_at(lpg, (float)dl, A.lut_factor) : lpg[dl];
-            const float wv = MODE == UPF_BOUNDARY_CPU_REF ? own_psi : S.psi[j];
-            const float inv = 1.0f / own_rho;
-            const float pg = own_p * inv * inv;
+            // CalculateBoundaryFluidPressureForceHost (:4240-4250).  The CPU path
+            // passes the integer bin as a distance to the table look-up, which
+            // always yields the table's +0 (entry 0 or past the end).
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            float g, wv;
+            if (MODE == UPF_BOUNDARY_CPU_REF) {
+                g = 0.0f;
+                wv = own_psi;
+            } else {
+                const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+                uint32_t dl = (uint32_t)(dist * 300.0f / h);
+                dl = dl > 300u ? 300u : dl;
+                g = lpg[dl];
+                wv = S.psi[(int)(e >> NB_SLOT_SHIFT)];
+            }
             const float s1 = (((dx * g) * pg) * P.initial_mass) * P.rest_density;
             const float s2 = (((dy * g) * pg) * P.initial_mass) * P.rest_density;
             const float s3 = (((dz * g) * pg) * P.initial_mass) * P.rest_density;
@@ -238,11 +395,28 @@ __global__ __launch_bounds__(BLOCK) void k_nbr_ext_static(StepArgs A, Slots S, W
             fy = fy + (s2 * wv) * -1.0f;
             fz = fz + (s3 * wv) * -1.0f;
         }
-    });
+    };
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        uint32_t e[4];
+        float4 xj[4], vj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = L[(k + u) * st];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = (int)(e[u] >> NB_SLOT_SHIFT);
+            xj[u] = S.pos[j];
+            vj[u] = S.vel[j];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pair(e[u], xj[u], vj[u]);
+    }
+    for (; k < cnt; ++k) {
+        const uint32_t e = L[k * st];
+        const int j = (int)(e >> NB_SLOT_SHIFT);
+        pair(e, S.pos[j], S.vel[j]);
+    }
     fy = fy - P.initial_mass * P.gravity;
-    W.ncount[i] = cnt < cap ? cnt : cap;  // lists hold at most cap entries
-    if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
-    atomicMax(&ctl->nbr_max, (uint32_t)cnt);
     const float4 F = make_float4(fx, fy, fz, 0.0f);
     W.force[i] = F;
     W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, ctl);
@@ -254,16 +428,16 @@ __device__ inline float wave_max(float v) {
     return v;
 }
 
-// ComputePredictedDensityAndPressure (:5456-5491) + max reduction + the loop
-// test of PredictionCorrectionStep (:5376-5398), evaluated by the last
-// workgroup to finish.  Iteration t runs only while t < ctl->want.
+// ComputePredictedDensityAndPressure (:5456-5491) + the max reduction of the
+// predicted density: each workgroup folds its max into one of MAX_SLOTS
+// padded slots (atomics spread over 64 cache lines).  Iteration t runs only
+// while t < ctl->want.
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                    Ctl* ctl) {
     if (t >= __atomic_load_n(&ctl->want, __ATOMIC_RELAXED)) return;
     __shared__ float lw[LUT_N];
     __shared__ float wmax[BLOCK / 64];
-    __shared__ int last;
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lw[k] = luts[k];
     __syncthreads();
     const upf_params& P = A.P;
@@ -276,8 +450,32 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
             const float T = P.support_radius_sq_limit;
             float rho = P.kernel_self * m;
             const int cnt = W.ncount[i];
-            for (int k = 0; k < cnt; ++k) {
-                const int j = (int)W.nbr[(size_t)k * A.stride + i];
+            const uint32_t* L = W.nbr + i;
+            const size_t st = (size_t)A.stride;
+            int k = 0;
+            // four neighbours per round: their loads are issued together,
+            // the sum still runs in list order
+            for (; k + 4 <= cnt; k += 4) {
+                int j[4];
+                float4 xj[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) j[u] = (int)(L[(k + u) * st] >> NB_SLOT_SHIFT);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xj[u] = W.xs[j[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const float dx = xi.x - xj[u].x, dy = xi.y - xj[u].y, dz = xi.z - xj[u].z;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    if (d2 < T) {
+                        const float w = lut_at(lw, sqrtf(d2), A.lut_factor);
+                        float mj = m;
+                        if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj[u]) != UPF_LIQUID) mj = P.rest_density * S.psi[j[u]];
+                        rho = rho + w * mj;
+                    }
+                }
+            }
+            for (; k < cnt; ++k) {
+                const int j = (int)(L[k * st] >> NB_SLOT_SHIFT);
                 const float4 xj = W.xs[j];
                 const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                 const float d2 = dx * dx + dy * dy + dz * dz;
@@ -289,7 +487,9 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 }
             }
             const float err = fmax_ref(rho - P.rest_density, 0.0f);
-            W.p[i] = W.p[i] + fmax_ref(err * P.density_error_factor, 0.0f);
+            const float p = W.p[i] + fmax_ref(err * P.density_error_factor, 0.0f);
+            W.p[i] = p;
+            W.pdiv[i] = p / (P.rest_density * P.rest_density);
             W.rho[i] = rho;
             mine = rho;
         }
@@ -300,14 +500,21 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     if (threadIdx.x == 0) {
         float b = wmax[0];
         for (int k = 1; k < BLOCK / 64; ++k) b = fmaxf(b, wmax[k]);
-        atomicMax(&ctl->max_bits, __float_as_uint(b));
-        __threadfence();
-        last = atomicAdd(&ctl->done, 1u) == gridDim.x - 1;
+        if (b > 0.0f) atomicMax(&W.max_slots[(blockIdx.x & (MAX_SLOTS - 1)) * 32], __float_as_uint(b));
     }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        const float mx = __uint_as_float(atomicOr(&ctl->max_bits, 0u));
+}
+
+// The loop test of PredictionCorrectionStep (:5376-5398) on the device: max
+// over the slots, densityErrorInPercent, the too-large latch and the number
+// of iterations the loop runs.  One wave.
+__global__ void k_loop_control(int t, const upf_params P, uint32_t* __restrict__ max_slots, Ctl* ctl) {
+    if (t >= __atomic_load_n(&ctl->want, __ATOMIC_RELAXED)) return;
+    const int l = threadIdx.x;
+    uint32_t b = max_slots[l * 32];
+    max_slots[l * 32] = 0u;
+    for (int o = 32; o > 0; o >>= 1) b = max(b, (uint32_t)__shfl_xor((int)b, o, 64));
+    if (l == 0) {
+        const float mx = __uint_as_float(b);
         const float pct = fmax_ref(0.1f * mx - 100.0f, 0.0f);
         int too_large = ctl->too_large;
         if (pct < P.max_density_error_allowed) too_large = 0;
@@ -315,8 +522,6 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
         const bool next = it < P.min_loops || (too_large && it < P.max_loops);
         ctl->too_large = too_large;
         ctl->last_max = mx;
-        ctl->max_bits = 0u;
-        ctl->done = 0u;
         __atomic_store_n(&ctl->want, next ? it + 1 : it, __ATOMIC_RELAXED);
     }
 }
@@ -338,24 +543,45 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     const float m = P.initial_mass;
     const float T = P.support_radius_sq_limit;
     const float rho0 = P.rest_density;
-    const float pi = W.p[i];
+    const float pi_d = W.p[i] / A.dens_sq;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
-    for (int k = 0; k < cnt; ++k) {
-        const int j = (int)W.nbr[(size_t)k * A.stride + i];
-        const float4 xj = S.pos[j];
+    const uint32_t* L = W.nbr + i;
+    const size_t st = (size_t)A.stride;
+    // every listed pair is closer than h at these (current) positions, so the
+    // reference's `dist < h` test holds; the table bin comes with the entry
+    auto pair = [&](uint32_t e, const float4& xj, float pdj) __attribute__((always_inline)) {
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (d2 < T) {
-            const float g = lut_at(lpg, sqrtf(d2), A.lut_factor);
-            const float pj = W.p[j];
-            const float grad = pi / A.dens_sq + pj / (rho0 * rho0);
-            float mj = m;
-            if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = rho0 * S.psi[j];
-            fx = fx - (((dx * g) * grad) * m) * mj;
-            fy = fy - (((dy * g) * grad) * m) * mj;
-            fz = fz - (((dz * g) * grad) * m) * mj;
+        const float g = lpg[e & NB_BIN_MASK];
+        // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
+        // precomputed per particle by the density pass (same division)
+        const float grad = pi_d + pdj;
+        float mj = m;
+        if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = rho0 * S.psi[(int)(e >> NB_SLOT_SHIFT)];
+        fx = fx - (((dx * g) * grad) * m) * mj;
+        fy = fy - (((dy * g) * grad) * m) * mj;
+        fz = fz - (((dz * g) * grad) * m) * mj;
+    };
+    int k = 0;
+    for (; k + 4 <= cnt; k += 4) {
+        uint32_t e[4];
+        float4 xj[4];
+        float pd[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = L[(k + u) * st];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = (int)(e[u] >> NB_SLOT_SHIFT);
+            xj[u] = S.pos[j];
+            pd[u] = W.pdiv[j];
         }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pair(e[u], xj[u], pd[u]);
+    }
+    for (; k < cnt; ++k) {
+        const uint32_t e = L[k * st];
+        const int j = (int)(e >> NB_SLOT_SHIFT);
+        pair(e, S.pos[j], W.pdiv[j]);
     }
     const float4 Fp = make_float4(fx, fy, fz, 0.0f);
     W.fp[i] = Fp;
@@ -415,10 +641,10 @@ __global__ void k_integrate(StepArgs A, Slots S, Work W, int pcisph, int pure_fl
 }
 
 // ------------------------------------------------------------------- WCSPH
-// GetNeighbors + CalculateDensityPressure with physicsType 'w' and no wall
-// weight (:2339-2406) + CalculatePressureWCSPH (:2491-2501)
-__global__ __launch_bounds__(BLOCK) void k_nbr_density_wcsph(StepArgs A, Slots S, Work W, const uint2* __restrict__ cells,
-                                                             const float* __restrict__ luts, Ctl* ctl) {
+// CalculateDensityPressure with physicsType 'w' and no wall weight
+// (:2339-2406) + CalculatePressureWCSPH (:2491-2501), over the lists
+__global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                         Ctl* ctl) {
     __shared__ float lw[LUT_N];
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
     __syncthreads();
@@ -427,30 +653,15 @@ __global__ __launch_bounds__(BLOCK) void k_nbr_density_wcsph(StepArgs A, Slots S
     if (i >= A.n) return;
     const float4 xi = S.pos[i];
     if (ptype(xi) != UPF_LIQUID) {
-        W.ncount[i] = 0;
         W.rho[i] = 0.0f;
         W.pres[i] = 0.0f;
         W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
-    const float T = P.support_radius_sq_limit;
     const float m = P.initial_mass;
     float rho = P.kernel_self * m;
-    int cnt = 0;
-    const int cap = P.max_neighbors;
-    for_each_candidate(xi, P, cells, [&](int j) {
-        if (j == i) return;
-        const float4 xj = S.pos[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < T)) return;
-        if (cnt < cap) W.nbr[(size_t)cnt * A.stride + i] = (uint32_t)j;
-        ++cnt;
-        rho = rho + lut_at(lw, sqrtf(d2), A.lut_factor) * m;
-    });
-    W.ncount[i] = cnt < cap ? cnt : cap;
-    if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
-    atomicMax(&ctl->nbr_max, (uint32_t)cnt);
+    const int cnt = W.ncount[i];
+    for (int k = 0; k < cnt; ++k) rho = rho + lw[W.nbr[(size_t)k * A.stride + i] & NB_BIN_MASK] * m;
     const float gamma = 7.0f;
     const float b = P.rest_density * P.gas_const_wcsph / gamma;
     const float frac = rho / P.rest_density;
@@ -474,7 +685,6 @@ __global__ __launch_bounds__(BLOCK) void k_force_wcsph(StepArgs A, Slots S, Work
     const float4 xi = S.pos[i];
     if (ptype(xi) != UPF_LIQUID) return;
     const float m = P.initial_mass;
-    const float T = P.support_radius_sq_limit;
     const float rho_i = W.rho[i], p_i = W.pres[i];
     const float invD = (float)(1.0 / (double)rho_i);
     const float pVol = m * invD;
@@ -482,17 +692,15 @@ __global__ __launch_bounds__(BLOCK) void k_force_wcsph(StepArgs A, Slots S, Work
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
     for (int k = 0; k < cnt; ++k) {
-        const int j = (int)W.nbr[(size_t)k * A.stride + i];
+        const uint32_t e = W.nbr[(size_t)k * A.stride + i];
+        const int j = (int)(e >> NB_SLOT_SHIFT);
         const float4 xj = S.pos[j];
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < T)) continue;
         if (ptype(xj) != UPF_LIQUID) continue;
-        const float dist = sqrtf(d2);
-        const float kv = lut_at(lvl, dist, A.lut_factor);
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float kv = lvl[e & NB_BIN_MASK];
         const float rho_j = W.rho[j];
         const float nVol = m / rho_j;
-        const float g = lut_at(lpg, dist, A.lut_factor);
+        const float g = lpg[e & NB_BIN_MASK];
         const float grad = p_i * invD * invD + W.pres[j] / (rho_j * rho_j);
         fx = fx - (((dx * g) * grad) * m) * m;
         fy = fy - (((dy * g) * grad) * m) * m;
@@ -530,11 +738,12 @@ using namespace upf;
         if (e_ != hipSuccess) return UPF_ERR_HIP;  \
     } while (0)
 
-enum ProfClass { PC_SORT = 0, PC_GATHER, PC_CELLS, PC_PSI, PC_NBR_EXT, PC_DENSITY, PC_FORCE, PC_INTEGRATE,
-                 PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_COUNT };
+enum ProfClass { PC_SORT = 0, PC_GATHER, PC_CELLS, PC_PSI, PC_NBR_BUILD, PC_NBR_EXT, PC_DENSITY, PC_FORCE,
+                 PC_INTEGRATE, PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_COUNT };
 static const char* kProfNames[PC_COUNT] = {"radix_sort", "gather", "cell_table", "boundary_volume",
-                                          "neighbours_external_force", "predicted_density", "pressure_force_predict",
-                                          "integrate", "wcsph_neighbours_density", "wcsph_force", "rigid_bodies"};
+                                          "neighbour_lists", "external_force_predict", "predicted_density",
+                                          "pressure_force_predict", "integrate", "wcsph_density", "wcsph_force",
+                                          "rigid_bodies"};
 
 struct Prof {
     bool on = false;
@@ -627,6 +836,7 @@ static inline dim3 grid_for(int n) { return dim3((unsigned)((n + BLOCK - 1) / BL
 
 static int check_params(const upf_params* p) {
     if (!p) return UPF_ERR_ARG;
+    if (p->max_neighbors > 4096) return UPF_ERR_ARG;
     if (p->block_size < 2 || p->grid_dim == 0 || p->grid_dim > 512) return UPF_ERR_ARG;
     if (p->vbox_min[0] != 0.0f || p->vbox_min[1] != 0.0f || p->vbox_min[2] != 0.0f) return UPF_ERR_ARG;
     if (p->max_neighbors <= 0 || p->max_neighbors > 4096) return UPF_ERR_ARG;
@@ -687,36 +897,51 @@ static int err_code(uint32_t bits) {
 }
 
 template <int MODE>
-static int pcisph_loop(upf_sim* s, const StepArgs& A) {
+static void enqueue_iteration(upf_sim* s, const StepArgs& A, int t) {
     const dim3 g = grid_for(s->n);
     Slots& S = s->S[s->cur];
+    s->prof.begin(PC_DENSITY, s->st);
+    hipLaunchKernelGGL(k_density<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    hipLaunchKernelGGL(k_loop_control, dim3(1), dim3(64), 0, s->st, t, s->P, s->W.max_slots, s->ctl);
+    s->prof.end(s->st);
+    s->prof.begin(PC_FORCE, s->st);
+    hipLaunchKernelGGL(k_force<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    s->prof.end(s->st);
+}
+
+// PredictionCorrectionStep (:5367-5424).  The loop test runs on the device;
+// the host enqueues iterations ahead (as many as the previous step needed)
+// and reads the decision back once the enqueued ones are done.  Iterations
+// beyond the device's decision return at their first instruction.
+template <int MODE>
+static int pcisph_loop(upf_sim* s, const StepArgs& A) {
+    const int maxl = std::max(s->P.max_loops, 1);
+    const int lo = std::min(std::max(s->P.min_loops, 1), maxl);
+    // a skipped iteration costs three near-empty launches, a read-back round
+    // trip about as much as three of them: speculate a few iterations ahead
+    const int prev = s->k_hist.empty() ? lo : s->k_hist.back();
+    const int ahead = std::min(std::min(std::max(lo, prev), lo + 4), maxl);
     int t = 0;
-    const int first = std::min(std::max(s->P.min_loops, 1), std::max(s->P.max_loops, 1));
-    for (; t < first; ++t) {
-        s->prof.begin(PC_DENSITY, s->st);
-        hipLaunchKernelGGL(k_density<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-        s->prof.end(s->st);
-        s->prof.begin(PC_FORCE, s->st);
-        hipLaunchKernelGGL(k_force<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-        s->prof.end(s->st);
-    }
+    for (; t < ahead; ++t) enqueue_iteration<MODE>(s, A, t);
     for (;;) {
         int rc = read_ctl(s);
         if (rc) return rc;
-        const int want = s->h_ctl->want;
+        const int want = s->h_ctl->want;  // final once want <= enqueued iterations
         if (want <= t || (s->h_ctl->err & (1u << UPF_ERR_NAN))) break;
-        for (; t < want; ++t) {
-            s->prof.begin(PC_DENSITY, s->st);
-            hipLaunchKernelGGL(k_density<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-            s->prof.end(s->st);
-            s->prof.begin(PC_FORCE, s->st);
-            hipLaunchKernelGGL(k_force<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-            s->prof.end(s->st);
-        }
+        const int stop = std::min(t + 4, maxl);
+        for (; t < stop; ++t) enqueue_iteration<MODE>(s, A, t);
     }
     s->k_hist.push_back(s->h_ctl->want);
     s->rho_hist.push_back(s->h_ctl->last_max);
     return UPF_OK;
+}
+
+static void build_lists(upf_sim* s, const StepArgs& A) {
+    Slots& S = s->S[s->cur];
+    s->prof.begin(PC_NBR_BUILD, s->st);
+    hipLaunchKernelGGL(k_nbr_build, dim3((unsigned)((s->n + 63) / 64)), dim3(64), 0, s->st, s->n, S.pos,
+                       s->W.keys_sorted, s->W.cells, s->P, A.lut_factor, s->W.nbr, A.stride, s->W.ncount, s->ctl);
+    s->prof.end(s->st);
 }
 
 static int step_pcisph_static(upf_sim* s) {
@@ -732,19 +957,15 @@ static int step_pcisph_static(upf_sim* s) {
         s->prof.end(s->st);
         s->psi_ready = true;
     }
-    if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI) {
-        s->prof.begin(PC_NBR_EXT, s->st);
-        hipLaunchKernelGGL(k_nbr_ext_static<UPF_BOUNDARY_AKINCI>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->W.cells,
-                           s->luts, s->ctl);
-        s->prof.end(s->st);
-        rc = pcisph_loop<UPF_BOUNDARY_AKINCI>(s, A);
-    } else {
-        s->prof.begin(PC_NBR_EXT, s->st);
-        hipLaunchKernelGGL(k_nbr_ext_static<UPF_BOUNDARY_CPU_REF>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->W.cells,
-                           s->luts, s->ctl);
-        s->prof.end(s->st);
-        rc = pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
-    }
+    build_lists(s, A);
+    s->prof.begin(PC_NBR_EXT, s->st);
+    if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI)
+        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+    else
+        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+    s->prof.end(s->st);
+    rc = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? pcisph_loop<UPF_BOUNDARY_AKINCI>(s, A)
+                                                            : pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
     if (rc) return rc;
     s->prof.begin(PC_INTEGRATE, s->st);
     hipLaunchKernelGGL(k_integrate, g, dim3(BLOCK), 0, s->st, A, S, s->W, 1, 0, s->rigid ? 1 : 0, s->ctl);
@@ -759,8 +980,9 @@ static int step_wcsph_static(upf_sim* s) {
     const StepArgs A = s->args();
     Slots& S = s->S[s->cur];
     const dim3 g = grid_for(s->n);
+    build_lists(s, A);
     s->prof.begin(PC_WCSPH_DENSITY, s->st);
-    hipLaunchKernelGGL(k_nbr_density_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->W.cells, s->luts, s->ctl);
+    hipLaunchKernelGGL(k_density_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
     s->prof.begin(PC_WCSPH_FORCE, s->st);
     hipLaunchKernelGGL(k_force_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
@@ -776,13 +998,13 @@ static int step_wcsph_static(upf_sim* s) {
     return UPF_OK;
 }
 
-
 extern "C" {
 
 const char* upf_version(void) { return "upf-mi355x 0.1 (gfx950)"; }
 
 int upf_create(const upf_params* p, int capacity, upf_sim** out) {
-    if (!out || capacity <= 0) return UPF_ERR_ARG;
+    // list entries hold the neighbour slot in 23 bits
+    if (!out || capacity <= 0 || capacity >= (1 << (32 - NB_SLOT_SHIFT))) return UPF_ERR_ARG;
     int rc = check_params(p);
     if (rc) return rc;
     int ndev = 0;
@@ -798,6 +1020,7 @@ int upf_create(const upf_params* p, int capacity, upf_sim** out) {
         if (s->alloc(&W.keys, capacity) || s->alloc(&W.keys_sorted, capacity) || s->alloc(&W.perm, capacity) ||
             s->alloc(&W.force, capacity) || s->alloc(&W.fp, capacity) || s->alloc(&W.xs, capacity) ||
             s->alloc(&W.p, capacity) || s->alloc(&W.rho, capacity) || s->alloc(&W.pres, capacity) ||
+            s->alloc(&W.pdiv, capacity) || s->alloc(&W.max_slots, MAX_SLOTS * 32) ||
             s->alloc(&W.ncount, capacity))
             break;
         if (s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors)) break;
@@ -814,6 +1037,7 @@ int upf_create(const upf_params* p, int capacity, upf_sim** out) {
         build_luts(p->support_radius, L.data());
         if (hipMemcpy(s->luts, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) break;
         if (hipMemset(s->ctl, 0, sizeof(Ctl)) != hipSuccess) break;
+        if (hipMemset(W.max_slots, 0, MAX_SLOTS * 32 * sizeof(uint32_t)) != hipSuccess) break;
         rc = UPF_OK;
     } while (0);
     if (rc != UPF_OK) {
