@@ -171,6 +171,8 @@ typedef struct upf_rigid_scene_desc {
     float density_min, density_max; /* body density range (kg/m^3)           */
     float jitter_rel;
     float surface_rel;      /* surface sampling / spacing                    */
+    float wall_rel;         /* wall sampling / spacing                       */
+    float lattice_rel;      /* fluid lattice spacing / spacing               */
     uint64_t seed;
 } upf_rigid_scene_desc;
 
@@ -195,9 +197,16 @@ typedef struct upf_body {
     float torque[3];            /* last reduced torque (diagnostic)           */
 } upf_body;
 
-/* Rigid scene.  Call with pos4 == NULL to get the counts.  body_id / rel
- * (initial offset from the centre of mass, float4) are per particle, -1 / 0
- * for non-rigid particles.  Also sets the domain of *p.                    */
+/* Rigid scene (config of the coupling benchmark; replaces the OBJ-based
+ * rigid objects of Scene::build, src/sceneConfigure/Scene.cpp:59-131): the
+ * fluid lattice and walls of upf_scene_dam_break plus n_bodies bodies --
+ * even ones cubes, odd ones spheres -- with seeded size and density, sampled
+ * on their surface every surface_rel * spacing and stacked above the fluid.
+ * Call with pos4 == NULL to get the counts.  body_id / rel4 are per
+ * particle: -1 / 0 for non-rigid particles; rel4 = initial offset from the
+ * body's centre of mass, w = the particle's mass.  Bodies carry the inverse
+ * of their solid inertia tensor (the reference computes it into a temporary
+ * and keeps a zero one, src/cpu/UnifiedPhysics.cpp:150).  Sets the domain.  */
 int upf_scene_rigid(const upf_rigid_scene_desc* d, upf_params* p,
                     int* n_fluid, int* n_rigid, int* n_wall,
                     float* pos4, int* type, int* body_id, float* rel4,

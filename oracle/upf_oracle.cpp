@@ -340,7 +340,9 @@ template <class T> class Sim {
     void weighted_volume(int i) {
         Particle<T>& p = ps[i];
         if (!(p.type == UPF_FROZEN || p.type == UPF_RIGID)) return;
-        T inv = 0;
+        // Akinci et al. 2012 eq. 4 sums over the boundary neighbourhood
+        // including the particle itself; the CPU path's neighbour list excludes it
+        T inv = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? kernel_self : T(0);
         const std::vector<int>& nb = nbrs[i];
         for (size_t k = 0; k < nb.size(); ++k) {
             const Particle<T>& q = ps[nb[k]];
@@ -350,7 +352,10 @@ template <class T> class Sim {
                 if (q.type == UPF_FROZEN || q.type == UPF_RIGID) inv += w;
             }
         }
-        if (inv != 0) p.psi = (T)(1.0 / (double)inv);
+        // CPU path: 1 / sum W (:2735); Akinci mode: the GPU path's factor 0.7
+        // (src/gpu/particles_kernel.cu:5927-5946) on the self-inclusive sum, which
+        // makes one boundary layer -- at any sampling -- weigh about one fluid layer
+        if (inv != 0) p.psi = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? (T)(0.7 / (double)inv) : (T)(1.0 / (double)inv);
         else err = UPF_ERR_ISOLATED;
     }
 

@@ -45,6 +45,7 @@ def lib():
         L.upf_build_luts.argtypes = [pp, pf]
         L.upf_compute_density_error_factor.argtypes = [pp, pi]
         L.upf_scene_dam_break.argtypes = [C.POINTER(DamBreakDesc), pp, pi, pi, pf, pi]
+        L.upf_scene_rigid.argtypes = [C.POINTER(RigidSceneDesc), pp, pi, pi, pi, pf, pi, pi, pf, C.POINTER(Body)]
         L.upf_create.argtypes = [pp, C.c_int, C.POINTER(vp)]
         L.upf_destroy.argtypes = [vp]
         L.upf_set_params.argtypes = [vp, pp]
@@ -120,6 +121,48 @@ def standard_dam_break(nx, ny, nz, spacing=0.01, dt=0.001, seed=0, box_per_cell=
     pos, typ, nf = dam_break(p, nx, ny, nz, box, seed=seed)
     density_error_factor(p)
     return p, pos, typ, nf
+
+
+def rigid_scene(p, nx, ny, nz, box, n_bodies, size=(0.05, 0.15), density=(300.0, 2000.0), jitter_rel=0.001,
+                surface_rel=0.5, wall_rel=1.0, lattice_rel=1.0025, seed=2):
+    """Fluid + walls + surface-sampled rigid boxes/spheres; sets the domain of p.
+    Returns (pos4, type, body_id, rel4, bodies, counts)."""
+    d = RigidSceneDesc(nx=nx, ny=ny, nz=nz, n_bodies=n_bodies, size_min=size[0], size_max=size[1],
+                       density_min=density[0], density_max=density[1], jitter_rel=jitter_rel,
+                       surface_rel=surface_rel, wall_rel=wall_rel, lattice_rel=lattice_rel, seed=seed)
+    d.box[:] = list(box)
+    nf, nr, nw = C.c_int(), C.c_int(), C.c_int()
+    L = lib()
+    check("upf_scene_rigid", L.upf_scene_rigid(C.byref(d), C.byref(p), C.byref(nf), C.byref(nr), C.byref(nw),
+                                               None, None, None, None, None))
+    n = nf.value + nr.value + nw.value
+    pos = np.zeros((n, 4), np.float32)
+    typ = np.zeros(n, np.int32)
+    bid = np.zeros(n, np.int32)
+    rel = np.zeros((n, 4), np.float32)
+    bodies = (Body * max(n_bodies, 1))()
+    check("upf_scene_rigid", L.upf_scene_rigid(C.byref(d), C.byref(p), C.byref(nf), C.byref(nr), C.byref(nw),
+                                               ptr(pos), ptr(typ, C.c_int), ptr(bid, C.c_int), ptr(rel), bodies))
+    return pos, typ, bid, rel, list(bodies)[:n_bodies], (nf.value, nr.value, nw.value)
+
+
+def standard_rigid_scene(nx=64, ny=64, nz=128, n_bodies=64, spacing=0.01, dt=0.001, seed=2,
+                         mode=UPF_BOUNDARY_AKINCI):
+    """The rigid-fluid coupling workload: an nx*ny*nz fluid block (x, y up, z) in a
+    container 2 nx spacings long, nz deep, tall enough for the bodies stacked in
+    layers 0.2 m apart above the fluid, from where they drop onto it;
+    Akinci boundary volumes, surface tension 0.05, adhesion 0.01."""
+    p = make_params(spacing, dt)
+    lx, lz = 2 * nx * spacing, nz * spacing
+    per_layer = max(1, int(lx * 1.0025 / 0.2)) * max(1, int(lz * 1.0025 / 0.2))
+    layers = -(-n_bodies // per_layer)
+    box = (lx, ny * spacing + 0.15 + 0.2 * layers + 0.05, lz)
+    out = rigid_scene(p, nx, ny, nz, box, n_bodies, seed=seed)
+    p.surface_tension_gamma = 0.05
+    p.surface_adhesion_beta = 0.01
+    p.boundary_pressure_mode = mode
+    density_error_factor(p)
+    return (p,) + out
 
 
 # ------------------------------------------------------------ simulation

@@ -111,6 +111,8 @@ class RigidSceneDesc(C.Structure):
         ("density_min", C.c_float), ("density_max", C.c_float),
         ("jitter_rel", C.c_float),
         ("surface_rel", C.c_float),
+        ("wall_rel", C.c_float),
+        ("lattice_rel", C.c_float),
         ("seed", C.c_uint64),
     ]
 

@@ -124,3 +124,131 @@ int upf_scene_dam_break(const upf_dam_break_desc* d, upf_params* p, int* n_fluid
 }
 
 }  // extern "C"
+
+namespace {
+
+struct RigidOut {
+    std::vector<float> pos, rel;  // xyz per particle; rel w = particle mass
+    std::vector<int> body;
+};
+
+// points on the surface of a cube of edge a centred at the origin: the cube
+// lattice points with at least one coordinate on a face
+void sample_cube(double a, double s, std::vector<double>& pts) {
+    const int m = std::max(1, (int)std::floor(a / s + 0.5));
+    const double st = a / m;
+    for (int i = 0; i <= m; ++i)
+        for (int j = 0; j <= m; ++j)
+            for (int k = 0; k <= m; ++k) {
+                if (i != 0 && i != m && j != 0 && j != m && k != 0 && k != m) continue;
+                pts.push_back(-0.5 * a + i * st);
+                pts.push_back(-0.5 * a + j * st);
+                pts.push_back(-0.5 * a + k * st);
+            }
+}
+
+// Fibonacci lattice on a sphere of radius r (about one point per s^2)
+void sample_sphere(double r, double s, std::vector<double>& pts) {
+    const int n = std::max(8, (int)std::floor(4.0 * M_PI * r * r / (s * s) + 0.5));
+    const double ga = M_PI * (3.0 - std::sqrt(5.0));
+    for (int i = 0; i < n; ++i) {
+        const double y = 1.0 - 2.0 * (i + 0.5) / n;
+        const double rr = std::sqrt(std::max(0.0, 1.0 - y * y));
+        const double th = ga * i;
+        pts.push_back(r * std::cos(th) * rr);
+        pts.push_back(r * y);
+        pts.push_back(r * std::sin(th) * rr);
+    }
+}
+
+}  // namespace
+
+extern "C" int upf_scene_rigid(const upf_rigid_scene_desc* d, upf_params* p, int* n_fluid, int* n_rigid, int* n_wall,
+                               float* pos4, int* type, int* body_id, float* rel4, upf_body* bodies) {
+    if (!d || !p || d->nx <= 0 || d->ny <= 0 || d->nz <= 0 || d->n_bodies < 0 || !(d->surface_rel > 0.0f)) return UPF_ERR_ARG;
+    // fluid + walls of the dam break
+    upf_dam_break_desc db;
+    std::memset(&db, 0, sizeof(db));
+    db.nx = d->nx; db.ny = d->ny; db.nz = d->nz;
+    for (int a = 0; a < 3; ++a) db.box[a] = d->box[a];
+    db.jitter_rel = d->jitter_rel;
+    db.wall_rel = d->wall_rel > 0.0f ? d->wall_rel : 1.0f;
+    db.lattice_rel = d->lattice_rel > 0.0f ? d->lattice_rel : 1.0f;
+    db.seed = d->seed;
+    int nf = 0, nw = 0;
+    int rc = upf_scene_dam_break(&db, p, &nf, &nw, nullptr, nullptr);
+    if (rc != UPF_OK) return rc;
+    // bodies: columns 0.2 m apart over the floor, layers 0.2 m apart above the fluid
+    const double lat = db.lattice_rel;
+    const double pad = p->support_radius;
+    const double Lx = d->box[0] * lat, Ly = d->box[1] * lat, Lz = d->box[2] * lat;
+    const double fluid_top = pad + d->ny * lat * p->spacing;
+    const double slot = 0.2;
+    const int cx = std::max(1, (int)std::floor(Lx / slot)), cz = std::max(1, (int)std::floor(Lz / slot));
+    uint64_t rs = d->seed * 0x9E3779B97F4A7C15ull + 7;
+    RigidOut R;
+    std::vector<upf_body> B(d->n_bodies);
+    const double s = d->surface_rel * p->spacing;
+    for (int b = 0; b < d->n_bodies; ++b) {
+        const double size = d->size_min + (d->size_max - d->size_min) * uniform01(rs);
+        const double dens = d->density_min + (d->density_max - d->density_min) * uniform01(rs);
+        const int col = b % (cx * cz), layer = b / (cx * cz);
+        const double jx = (uniform01(rs) - 0.5) * 0.04, jz = (uniform01(rs) - 0.5) * 0.04, jy = uniform01(rs) * 0.03;
+        const double ccx = pad + (col % cx + 0.5) * (Lx / cx) + jx;
+        const double ccz = pad + (col / cx + 0.5) * (Lz / cz) + jz;
+        const double ccy = fluid_top + 0.1 + 0.5 * size + layer * slot + jy;
+        if (ccy + 0.5 * size > pad + Ly - p->particle_radius) return UPF_ERR_ARG;  // container too low
+        std::vector<double> pts;
+        double mass, I;
+        if (b % 2 == 0) {
+            sample_cube(size, s, pts);
+            mass = dens * size * size * size;
+            I = mass * size * size / 6.0;
+        } else {
+            sample_sphere(0.5 * size, s, pts);
+            mass = dens * M_PI * size * size * size / 6.0;
+            I = 0.4 * mass * 0.25 * size * size;
+        }
+        const int np = (int)(pts.size() / 3);
+        const float pm = (float)(mass / np);
+        for (int k = 0; k < np; ++k) {
+            R.pos.push_back((float)(ccx + pts[3 * k])); R.pos.push_back((float)(ccy + pts[3 * k + 1]));
+            R.pos.push_back((float)(ccz + pts[3 * k + 2]));
+            R.rel.push_back((float)pts[3 * k]); R.rel.push_back((float)pts[3 * k + 1]); R.rel.push_back((float)pts[3 * k + 2]);
+            R.rel.push_back(pm);
+            R.body.push_back(b);
+        }
+        upf_body& o = B[b];
+        std::memset(&o, 0, sizeof(o));
+        o.pos[0] = (float)ccx; o.pos[1] = (float)ccy; o.pos[2] = (float)ccz;
+        o.mass = (float)mass;
+        o.quaternion[0] = 1.0f;
+        for (int k = 0; k < 3; ++k) {
+            o.rotation[4 * k] = 1.0f;
+            o.inv_inertia_local[4 * k] = (float)(1.0 / I);
+            o.inv_inertia_world[4 * k] = (float)(1.0 / I);
+        }
+    }
+    const int nr = (int)R.body.size();
+    if (n_fluid) *n_fluid = nf;
+    if (n_wall) *n_wall = nw;
+    if (n_rigid) *n_rigid = nr;
+    if (!pos4) return UPF_OK;
+    rc = upf_scene_dam_break(&db, p, &nf, &nw, pos4, type);  // fluid, then walls
+    if (rc != UPF_OK) return rc;
+    const int n0 = nf + nw;
+    for (int i = 0; i < n0; ++i) {
+        if (body_id) body_id[i] = -1;
+        if (rel4) { rel4[4 * i] = rel4[4 * i + 1] = rel4[4 * i + 2] = rel4[4 * i + 3] = 0.0f; }
+    }
+    for (int k = 0; k < nr; ++k) {
+        const int i = n0 + k;
+        pos4[4 * i] = R.pos[3 * k]; pos4[4 * i + 1] = R.pos[3 * k + 1]; pos4[4 * i + 2] = R.pos[3 * k + 2];
+        pos4[4 * i + 3] = 0.0f;
+        if (type) type[i] = UPF_RIGID;
+        if (body_id) body_id[i] = R.body[k];
+        if (rel4) for (int a = 0; a < 4; ++a) rel4[4 * i + a] = R.rel[4 * k + a];
+    }
+    if (bodies) std::memcpy(bodies, B.data(), B.size() * sizeof(upf_body));
+    return UPF_OK;
+}

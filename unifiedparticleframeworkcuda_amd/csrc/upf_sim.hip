@@ -117,7 +117,9 @@ __global__ void k_psi(int n, const float4* __restrict__ pos, const uint2* __rest
     const int ti = ptype(xi);
     if (ti != UPF_FROZEN && ti != UPF_RIGID) return;
     const float T = P.support_radius_sq_limit;
-    float inv = 0.0f;
+    // Akinci et al. 2012 eq. 4 sums over the boundary neighbourhood including the
+    // particle itself (W(0) = the table's entry 0); the CPU path excludes it
+    float inv = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? lw[0] : 0.0f;
     for_each_candidate(xi, P, cells, [&](int j) __attribute__((always_inline)) {
         if (j == i) return;
         const float4 xj = pos[j];
@@ -127,7 +129,9 @@ __global__ void k_psi(int n, const float4* __restrict__ pos, const uint2* __rest
         const int tj = ptype(xj);
         if (tj == UPF_FROZEN || tj == UPF_RIGID) inv += lut_at(lw, sqrtf(d2), lut_factor);
     });
-    if (inv != 0.0f) psi[i] = (float)(1.0 / (double)inv);
+    // CPU path: 1 / sum W (:2735); Akinci mode: the GPU path's factor 0.7
+    // (src/gpu/particles_kernel.cu:5927-5946) on the self-inclusive sum
+    if (inv != 0.0f) psi[i] = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? (float)(0.7 / (double)inv) : (float)(1.0 / (double)inv);
     else set_err(ctl, UPF_ERR_ISOLATED);
 }
 
@@ -715,6 +719,388 @@ __global__ __launch_bounds__(BLOCK) void k_force_wcsph(StepArgs A, Slots S, Work
     W.force[i] = make_float4(fx, fy, fz, 0.0f);
 }
 
+// ------------------------------------------------------ rigid coupling
+// Neighbour lists of rigid particles.  Their keys are those of the position
+// before the last rigid update (the reference recomputes keys before it moves
+// the rigid particles, src/cpu/UnifiedPhysics.cpp:5007-5022), so their query box
+// need not sit on their key's cell stencil: each lane walks its own box.
+__global__ __launch_bounds__(BLOCK) void k_nbr_build_box(int n, const float4* __restrict__ pos,
+                                                         const uint2* __restrict__ cells, upf_params P, float lut_factor,
+                                                         int owner_type, uint32_t* __restrict__ nbr, int stride,
+                                                         int* __restrict__ ncount, Ctl* ctl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 xi = pos[i];
+    if (ptype(xi) != owner_type) return;
+    const float T = P.support_radius_sq_limit;
+    const int cap = P.max_neighbors;
+    int cnt = 0;
+    for_each_candidate(xi, P, cells, [&](int j) __attribute__((always_inline)) {
+        if (j == i) return;
+        const float4 xj = pos[j];
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (!(d2 < T)) return;
+        if (cnt < cap) nbr[(size_t)cnt * stride + i] = ((uint32_t)j << NB_SLOT_SHIFT) | lut_bin(sqrtf(d2), lut_factor);
+        ++cnt;
+    });
+    ncount[i] = cnt < cap ? cnt : cap;
+    if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
+}
+
+// Boundary volumes of the rigid particles only (walls keep theirs).
+__global__ void k_psi_rigid(int n, const float4* __restrict__ pos, const uint32_t* __restrict__ nbr, int stride,
+                            const int* __restrict__ ncount, upf_params P, const float* __restrict__ luts,
+                            float* __restrict__ psi, Ctl* ctl) {
+    __shared__ float lw[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (ptype(pos[i]) != UPF_RIGID) return;
+    float inv = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? lw[0] : 0.0f;
+    const int cnt = ncount[i];
+    for (int k = 0; k < cnt; ++k) {
+        const uint32_t e = nbr[(size_t)k * stride + i];
+        const int tj = ptype(pos[e >> NB_SLOT_SHIFT]);
+        if (tj == UPF_FROZEN || tj == UPF_RIGID) inv += lw[e & NB_BIN_MASK];
+    }
+    if (inv != 0.0f) psi[i] = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? (float)(0.7 / (double)inv) : (float)(1.0 / (double)inv);
+    else set_err(ctl, UPF_ERR_ISOLATED);
+}
+
+// CalculateBoundaryFluidPressureForceHost (:4240-4250) for a liquid particle
+// at xl (density rho, pressure pr, volume wv) and a boundary particle at xb
+__device__ inline void boundary_pressure(int mode, float dist, float h, const float4& xl, const float4& xb, float rho,
+                                         float pr, float wv, const float* lpg, const upf_params& P, float& ox,
+                                         float& oy, float& oz) {
+    uint32_t dl = (uint32_t)(dist * 300.0f / h);
+    dl = dl > 300u ? 300u : dl;
+    const float g = mode == UPF_BOUNDARY_CPU_REF ? 0.0f : lpg[dl];  // the CPU path's look-up always yields +0
+    const float inv = 1.0f / rho;
+    const float pg = pr * inv * inv;
+    const float dx = xl.x - xb.x, dy = xl.y - xb.y, dz = xl.z - xb.z;
+    ox = (((((dx * g) * pg) * P.initial_mass) * P.rest_density) * wv) * -1.0f;
+    oy = (((((dy * g) * pg) * P.initial_mass) * P.rest_density) * wv) * -1.0f;
+    oz = (((((dz * g) * pg) * P.initial_mass) * P.rest_density) * wv) * -1.0f;
+}
+
+// CalculateExternalForcesVersatilCouplingPCISPH (:5252-5365) over the lists
+// for liquid and rigid particles, + the first prediction of the loop; every
+// slot resets its loop pressure.  Pressures and densities of neighbours are
+// the ones the previous step's loop ended with.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                        Ctl* ctl) {
+    __shared__ float lvl[LUT_N], lpg[LUT_N], lst[LUT_N], lad[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
+        lpg[t] = luts[LUT_N + t];
+        lvl[t] = luts[2 * LUT_N + t];
+        lst[t] = luts[3 * LUT_N + t];
+        lad[t] = luts[4 * LUT_N + t];
+    }
+    __syncthreads();
+    const upf_params& P = A.P;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->too_large = 1;
+        ctl->want = (0 < P.min_loops || 0 < P.max_loops) ? 1 : 0;
+        ctl->last_max = 0.0f;
+    }
+    if (i >= A.n) return;
+    const float4 xi = S.pos[i];
+    const int ti = ptype(xi);
+    W.p[i] = 0.0f;
+    W.pdiv[i] = 0.0f;
+    W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    W.rho[i] = S.rho_last[i];
+    if (ti != UPF_LIQUID && ti != UPF_RIGID) {
+        W.xs[i] = xi;
+        W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const float4 vi = S.vel[i];
+    const float h = P.support_radius;
+    const float m = P.initial_mass;
+    const float rv = A.rest_vol, visc = P.visc_const;
+    const float quarter = h * 0.25f;
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    const int cnt = W.ncount[i];
+    // the liquid particle's own pressure / density / volume (CPU path:
+    // previous_correction_pressure_ and its own weighted_volume_)
+    const float own_p = MODE == UPF_BOUNDARY_CPU_REF ? S.p_prev[i] : S.p_last[i];
+    const float own_rho = S.rho_last[i];
+    const float own_psi = S.psi[i];
+    for (int k = 0; k < cnt; ++k) {
+        const uint32_t e = W.nbr[(size_t)k * A.stride + i];
+        const int j = (int)(e >> NB_SLOT_SHIFT);
+        const float4 xj = S.pos[j];
+        const int tj = ptype(xj);
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+        uint32_t dl = (uint32_t)(dist * 300.0f / h);
+        dl = dl > 300u ? 300u : dl;
+        if (ti == UPF_LIQUID) {
+            if (tj == UPF_LIQUID) {
+                const float kv = lvl[e & NB_BIN_MASK];
+                const float4 vj = S.vel[j];
+                const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+                fx = fx - vx * rv * rv * visc * kv;
+                fy = fy - vy * rv * rv * visc * kv;
+                fz = fz - vz * rv * rv * visc * kv;
+                if (dist >= quarter) {  // CalculateSurfaceTensionForcePCISPHHost (:1990-1998)
+                    const float sc = lst[dl];
+                    const float id = 1.0f / dist;
+                    fx = fx - dx * P.surface_tension_gamma * m * m * sc * id;
+                    fy = fy - dy * P.surface_tension_gamma * m * m * sc * id;
+                    fz = fz - dz * P.surface_tension_gamma * m * m * sc * id;
+                }
+            } else if (tj == UPF_FROZEN || tj == UPF_RIGID) {
+                float ox, oy, oz;
+                const float wv = MODE == UPF_BOUNDARY_CPU_REF ? own_psi : S.psi[j];
+                boundary_pressure(MODE, dist, h, xi, xj, own_rho, own_p, wv, lpg, P, ox, oy, oz);
+                fx = fx + ox; fy = fy + oy; fz = fz + oz;
+                if (dist >= quarter) {  // CalculateSurfaceCohesionForcePCISPHHost (:2000-2008)
+                    const float sc = lad[dl];
+                    const float id = 1.0f / dist;
+                    const float pj = S.psi[j];
+                    fx = fx - dx * P.surface_adhesion_beta * m * P.rest_density * pj * sc * id;
+                    fy = fy - dy * P.surface_adhesion_beta * m * P.rest_density * pj * sc * id;
+                    fz = fz - dz * P.surface_adhesion_beta * m * P.rest_density * pj * sc * id;
+                }
+            }
+        } else {  // rigid
+            if (tj == UPF_RIGID) {
+                const float overlap = 2.0f * P.particle_radius - dist;
+                if (overlap > 0.00001f) {
+                    // CalculateSpringForceHost + CalculateDampingForceHost (:2010-2030), r_ij = x_j - x_i
+                    const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+                    const float sx = -1.0f * P.spring_coefficient * overlap * rx / dist;
+                    const float sy = -1.0f * P.spring_coefficient * overlap * ry / dist;
+                    const float sz = -1.0f * P.spring_coefficient * overlap * rz / dist;
+                    const float4 vj = S.vel[j];
+                    const float cx = P.damping_coefficient * (vj.x - vi.x);
+                    const float cy = P.damping_coefficient * (vj.y - vi.y);
+                    const float cz = P.damping_coefficient * (vj.z - vi.z);
+                    fx = fx + (sx + cx);
+                    fy = fy + (sy + cy);
+                    fz = fz + (sz + cz);
+                }
+            } else if (tj == UPF_LIQUID) {
+                // the liquid's pressure force on this boundary particle, reversed (:5316-5321)
+                float ox, oy, oz;
+                const float wv = MODE == UPF_BOUNDARY_CPU_REF ? S.psi[j] : S.psi[i];
+                boundary_pressure(MODE, dist, h, xj, xi, S.rho_last[j], S.p_last[j], wv, lpg, P, ox, oy, oz);
+                fx = fx - ox; fy = fy - oy; fz = fz - oz;
+                if (dist >= quarter) {
+                    const float sc = lad[dl];
+                    const float id = 1.0f / dist;
+                    const float pj = S.psi[j];
+                    const float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
+                    fx = fx + rx * P.surface_adhesion_beta * m * P.rest_density * pj * sc * id;
+                    fy = fy + ry * P.surface_adhesion_beta * m * P.rest_density * pj * sc * id;
+                    fz = fz + rz * P.surface_adhesion_beta * m * P.rest_density * pj * sc * id;
+                }
+            }
+        }
+    }
+    if (ti == UPF_LIQUID) {
+        fy = fy - P.initial_mass * P.gravity;
+        // CalculateBoundaryForcePerLiquidParticleHost (:2032-2069)
+        const float inv = 1.0f / P.force_distance;
+        const float fd = P.force_distance, fmax = P.max_boundary_force;
+        const float c[3] = {xi.x, xi.y, xi.z};
+        float b[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float mn = P.collision_min[a], mx = P.collision_max[a];
+            if (c[a] < mn + fd) {
+                const float v = (float)((double)((mn + fd - c[a]) * inv) * 2.0 * (double)fmax);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) b[q] = b[q] + (q == a ? 1.0f : 0.0f) * v;
+            }
+            if (c[a] > mx - fd) {
+                const float v = (float)((double)((c[a] + fd - mx) * inv) * 2.0 * (double)fmax);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) b[q] = b[q] + (q == a ? -1.0f : 0.0f) * v;
+            }
+        }
+        fx = fx + b[0]; fy = fy + b[1]; fz = fz + b[2];
+        const float4 F = make_float4(fx, fy, fz, 0.0f);
+        W.force[i] = F;
+        W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, ctl);
+    } else {
+        // CalculateBoundaryForcePerRigidParticleHost (:2071-2132)
+        const float r = P.particle_radius, kb = P.spring_coefficient_boundary, cd = P.damping_coefficient;
+        const float c[3] = {xi.x, xi.y, xi.z};
+        float b[3] = {0.0f, 0.0f, 0.0f};
+        bool hit = false;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float mn = P.collision_min[a], mx = P.collision_max[a];
+            if (c[a] - r < mn) {
+                hit = true;
+                const float v = mn + r - c[a];
+#pragma unroll
+                for (int q = 0; q < 3; ++q) b[q] = b[q] + ((q == a ? 1.0f : 0.0f) * v) * kb;
+            }
+            if (c[a] + r > mx) {
+                hit = true;
+                const float v = c[a] + r - mx;
+#pragma unroll
+                for (int q = 0; q < 3; ++q) b[q] = b[q] + ((q == a ? -1.0f : 0.0f) * v) * kb;
+            }
+        }
+        if (hit) { b[0] -= cd * vi.x; b[1] -= cd * vi.y; b[2] -= cd * vi.z; }
+        fx = fx + b[0]; fy = fy + b[1]; fz = fz + b[2];
+        W.force[i] = make_float4(fx, fy, fz, 0.0f);
+        W.xs[i] = P.rigid_predicted_current ? xi : S.x0[i];
+    }
+}
+
+// key of a slot for grouping rigid particles by body (non-rigid last)
+__global__ void k_body_keys(int n, const int* __restrict__ body, const float4* __restrict__ pos,
+                            uint32_t* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = body[i];
+    keys[i] = (b >= 0 && ptype(pos[i]) == UPF_RIGID) ? (uint32_t)b : 0xFFFFu;
+}
+
+__global__ void k_body_bounds(int n, const uint32_t* __restrict__ keys, int nb, int* __restrict__ start) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = keys[i];
+    if (k >= (uint32_t)nb) {
+        if (i == 0 || keys[i - 1] < (uint32_t)nb) start[nb] = i;  // end of the rigid block
+        return;
+    }
+    if (i == 0 || keys[i - 1] != k) start[k] = i;
+    if (i == n - 1) start[nb] = n;
+}
+
+__device__ inline void mat3mul(const float* A, const float* B, float* C) {  // Matrix3x3::operator*
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            C[r * 3 + c] = A[r * 3 + 0] * B[0 * 3 + c] + A[r * 3 + 1] * B[1 * 3 + c] + A[r * 3 + 2] * B[2 * 3 + c];
+}
+
+// CalculateRigidBodyForceTorqueMomenta (:3620-3703) with RigidBody::UpdateMomenta,
+// PerformStep, UpdateRotationMatrix (src/cpu/RigidBody.cpp:55-160): one thread
+// per body sums its particles' forces, torques and weights in slot order.
+__global__ void k_rigid_bodies(int nb, upf_body* __restrict__ bodies, const int* __restrict__ start,
+                               const uint32_t* __restrict__ members, const float4* __restrict__ pos,
+                               const float4* __restrict__ force, const float4* __restrict__ rel, upf_params P) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    upf_body B = bodies[b];
+    float F[3] = {0.0f, 0.0f, 0.0f}, Tq[3] = {0.0f, 0.0f, 0.0f};
+    const int s0 = start[b];
+    int s1 = nb;
+    for (int k = b + 1; k <= nb; ++k) {
+        if (start[k] >= 0) { s1 = start[k]; break; }
+    }
+    for (int k = s0; s0 >= 0 && k < s1; ++k) {
+        const int i = (int)members[k];
+        const float4 f = force[i];
+        const float4 x = pos[i];
+        F[0] = F[0] + f.x; F[1] = F[1] + f.y; F[2] = F[2] + f.z;
+        const float r0 = x.x - B.pos[0], r1 = x.y - B.pos[1], r2 = x.z - B.pos[2];
+        Tq[0] = Tq[0] + (r1 * f.z - r2 * f.y);
+        Tq[1] = Tq[1] + (r2 * f.x - r0 * f.z);
+        Tq[2] = Tq[2] + (r0 * f.y - r1 * f.x);
+        const float gy = -rel[i].w * P.gravity;
+        F[0] = F[0] + 0.0f; F[1] = F[1] + gy; F[2] = F[2] + 0.0f;
+    }
+    const float dt = P.dt;
+    const float terminal = B.mass * P.terminal_speed;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) B.linear_momentum[a] = B.linear_momentum[a] + F[a] * dt;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        if (B.linear_momentum[a] > 0.0f) B.linear_momentum[a] = fmin_ref(B.linear_momentum[a], terminal);
+        else B.linear_momentum[a] = fmax_ref(B.linear_momentum[a], -terminal);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) B.angular_momentum[a] = B.angular_momentum[a] + Tq[a] * dt;
+    float Rt[9], tmp[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Rt[r * 3 + c] = B.rotation[c * 3 + r];
+    mat3mul(B.rotation, B.inv_inertia_local, tmp);
+    mat3mul(tmp, Rt, B.inv_inertia_world);
+    {
+        const float inv = 1.0f / B.mass;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) B.vel[a] = B.linear_momentum[a] * inv;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) B.pos[a] = B.pos[a] + B.vel[a] * dt;
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+        B.angular_velocity[r] = B.inv_inertia_world[r * 3 + 0] * B.angular_momentum[0] +
+                                B.inv_inertia_world[r * 3 + 1] * B.angular_momentum[1] +
+                                B.inv_inertia_world[r * 3 + 2] * B.angular_momentum[2];
+    const float* w = B.angular_velocity;
+    const float sp = sqrtf(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    if (sp > 0.0f) {
+        const float ll = (float)(1.0 / (double)sp);
+        const float ax = w[0] * ll, ay = w[1] * ll, az = w[2] * ll;
+        const float ang = sp * dt;
+        const float ds = cosf(ang / 2.0f);
+        const float sn = sinf(ang / 2.0f);
+        const float dvx = ax * sn, dvy = ay * sn, dvz = az * sn;
+        const float s = B.quaternion[0], vx = B.quaternion[1], vy = B.quaternion[2], vz = B.quaternion[3];
+        B.quaternion[0] = s * ds - vx * dvx - vy * dvy - vz * dvz;
+        B.quaternion[1] = ds * vx + s * dvx + dvy * vz - dvz * vy;
+        B.quaternion[2] = ds * vy + s * dvy + dvz * vx - dvx * vz;
+        B.quaternion[3] = ds * vz + s * dvz + dvx * vy - dvy * vx;
+    }
+    if (P.renormalize_quaternion) {
+        const float* q = B.quaternion;
+        const float l = sqrtf(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        if (l > 0.0f) {
+            const float il = 1.0f / l;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) B.quaternion[a] *= il;
+        }
+    }
+    const float qw = B.quaternion[0], qx = B.quaternion[1], qy = B.quaternion[2], qz = B.quaternion[3];
+    const float xx = qx * qx, yy = qy * qy, zz = qz * qz, xy = qx * qy, xz = qx * qz, yz = qy * qz, wx = qw * qx,
+                wy = qw * qy, wz = qw * qz;
+    B.rotation[0] = 1.0f - 2.0f * (yy + zz); B.rotation[1] = 2.0f * (xy - wz); B.rotation[2] = 2.0f * (xz + wy);
+    B.rotation[3] = 2.0f * (xy + wz); B.rotation[4] = 1.0f - 2.0f * (xx + zz); B.rotation[5] = 2.0f * (yz - wx);
+    B.rotation[6] = 2.0f * (xz - wy); B.rotation[7] = 2.0f * (yz + wx); B.rotation[8] = 1.0f - 2.0f * (xx + yy);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { B.force[a] = F[a]; B.torque[a] = Tq[a]; }
+    bodies[b] = B;
+}
+
+// SynRigidParticlesPerRigidBody (:3744-3792): x = x_cm + R r0, v = v_cm + w x (R r0)
+__global__ void k_rigid_sync(int n, Slots S, const upf_body* __restrict__ bodies) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 x = S.pos[i];
+    if (ptype(x) != UPF_RIGID) return;
+    const int b = S.body[i];
+    if (b < 0) return;
+    const upf_body& B = bodies[b];
+    const float4 r0 = S.rel[i];
+    float r[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) r[q] = B.rotation[q * 3 + 0] * r0.x + B.rotation[q * 3 + 1] * r0.y + B.rotation[q * 3 + 2] * r0.z;
+    const float* w = B.angular_velocity;
+    const float c0 = w[1] * r[2] - w[2] * r[1], c1 = w[2] * r[0] - w[0] * r[2], c2 = w[0] * r[1] - w[1] * r[0];
+    S.vel[i] = make_float4(B.vel[0] + c0, B.vel[1] + c1, B.vel[2] + c2, 0.0f);
+    x.x = B.pos[0] + r[0];
+    x.y = B.pos[1] + r[1];
+    x.z = B.pos[2] + r[2];
+    S.pos[i] = x;
+}
+
 // -------------------------------------------------------- id-order output
 __global__ void k_scatter_by_id(int n, const int* __restrict__ id, const float4* __restrict__ a4, float4* __restrict__ o4,
                                 const float* __restrict__ a1, float* __restrict__ o1) {
@@ -810,8 +1196,10 @@ struct upf_sim {
     std::vector<void*> allocs;
     // rigid bodies
     upf_body* d_bodies = nullptr;
-    float* d_body_red = nullptr;
-    int* d_body_start = nullptr;
+    int* d_body_start = nullptr;       // first member slot of each body (n_bodies + 1, -1 when empty)
+    uint32_t* d_body_keys = nullptr;   // scratch: body key per slot
+    uint32_t* d_body_keys2 = nullptr;
+    uint32_t* d_body_members = nullptr;  // slots grouped by body, ascending within a body
 
     StepArgs args() const {
         StepArgs A;
@@ -974,6 +1362,55 @@ static int step_pcisph_static(upf_sim* s) {
     return UPF_OK;
 }
 
+// CpuParticlePhysicsVersatileCouplingPCISPH (src/cpu/UnifiedPhysics.cpp:4991-5024)
+static int step_pcisph_rigid(upf_sim* s) {
+    int rc = sort_and_bin(s);
+    if (rc) return rc;
+    const StepArgs A = s->args();
+    Slots& S = s->S[s->cur];
+    const dim3 g = grid_for(s->n);
+    build_lists(s, A);  // liquid owners
+    s->prof.begin(PC_NBR_BUILD, s->st);
+    hipLaunchKernelGGL(k_nbr_build_box, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.cells, s->P, A.lut_factor,
+                       (int)UPF_RIGID, s->W.nbr, A.stride, s->W.ncount, s->ctl);
+    s->prof.end(s->st);
+    if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI) {
+        s->prof.begin(PC_PSI, s->st);
+        if (!s->psi_ready) {  // walls: once (they do not move)
+            hipLaunchKernelGGL(k_psi, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.cells, s->P, s->luts, A.lut_factor,
+                               S.psi, s->ctl);
+            s->psi_ready = true;
+        }
+        hipLaunchKernelGGL(k_psi_rigid, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.nbr, A.stride, s->W.ncount, s->P,
+                           s->luts, S.psi, s->ctl);
+        s->prof.end(s->st);
+    }
+    s->prof.begin(PC_NBR_EXT, s->st);
+    if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI)
+        hipLaunchKernelGGL(k_ext_coupling<UPF_BOUNDARY_AKINCI>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+    else
+        hipLaunchKernelGGL(k_ext_coupling<UPF_BOUNDARY_CPU_REF>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+    s->prof.end(s->st);
+    rc = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? pcisph_loop<UPF_BOUNDARY_AKINCI>(s, A)
+                                                            : pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
+    if (rc) return rc;
+    s->prof.begin(PC_INTEGRATE, s->st);
+    hipLaunchKernelGGL(k_integrate, g, dim3(BLOCK), 0, s->st, A, S, s->W, 1, 1, 1, s->ctl);
+    s->prof.end(s->st);
+    // rigid bodies: members grouped by body in slot order, one thread per body
+    s->prof.begin(PC_RIGID, s->st);
+    hipLaunchKernelGGL(k_body_keys, g, dim3(BLOCK), 0, s->st, s->n, S.body, S.pos, s->d_body_keys);
+    HIPCHK(radix_sort(s->d_body_keys, nullptr, s->d_body_keys2, s->d_body_members, s->n, 16, s->sort_tmp, s->st));
+    HIPCHK(hipMemsetAsync(s->d_body_start, 0xFF, (size_t)(s->n_bodies + 1) * sizeof(int), s->st));
+    hipLaunchKernelGGL(k_body_bounds, g, dim3(BLOCK), 0, s->st, s->n, s->d_body_keys2, s->n_bodies, s->d_body_start);
+    hipLaunchKernelGGL(k_rigid_bodies, dim3((s->n_bodies + 63) / 64), dim3(64), 0, s->st, s->n_bodies, s->d_bodies,
+                       s->d_body_start, s->d_body_members, S.pos, s->W.force, S.rel, s->P);
+    hipLaunchKernelGGL(k_rigid_sync, g, dim3(BLOCK), 0, s->st, s->n, S, s->d_bodies);
+    s->prof.end(s->st);
+    HIPCHK(hipGetLastError());
+    return UPF_OK;
+}
+
 static int step_wcsph_static(upf_sim* s) {
     int rc = sort_and_bin(s);
     if (rc) return rc;
@@ -1052,6 +1489,8 @@ int upf_destroy(upf_sim* s) {
     if (!s) return UPF_ERR_ARG;
     if (s->st) (void)hipStreamSynchronize(s->st);
     for (void* p : s->allocs) (void)hipFree(p);
+    if (s->d_bodies) (void)hipFree(s->d_bodies);
+    if (s->d_body_start) (void)hipFree(s->d_body_start);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
@@ -1126,7 +1565,6 @@ int upf_upload_particles(upf_sim* s, int n, const float* pos4, const float* vel4
     return UPF_OK;
 }
 
-int upf_step_rigid(upf_sim* s);  // upf_rigid.hip
 
 int upf_step(upf_sim* s, int nsteps, int method) {
     if (!s || nsteps < 0) return UPF_ERR_ARG;
@@ -1136,7 +1574,7 @@ int upf_step(upf_sim* s, int nsteps, int method) {
         int rc;
         if (method == UPF_PCISPH_STATIC) rc = step_pcisph_static(s);
         else if (method == UPF_WCSPH_STATIC) rc = step_wcsph_static(s);
-        else if (method == UPF_PCISPH_RIGID) rc = upf_step_rigid(s);
+        else if (method == UPF_PCISPH_RIGID) rc = step_pcisph_rigid(s);
         else return UPF_ERR_ARG;
         s->last_method = method;
         if (rc) return rc;
@@ -1254,6 +1692,54 @@ int upf_download(upf_sim* s, float* pos4, float* vel4, float* rho, float* p, flo
     return UPF_OK;
 }
 
+int upf_upload_bodies(upf_sim* s, int n_bodies, const upf_body* bodies, const int* body_id, const float* rel4) {
+    if (!s || n_bodies <= 0 || n_bodies >= 0xFFFF || !bodies || !body_id || !rel4) return UPF_ERR_ARG;
+    if (!s->uploaded) return UPF_ERR_STATE;
+    const int n = s->n;
+    if (s->d_bodies) {
+        (void)hipFree(s->d_bodies);
+        (void)hipFree(s->d_body_start);
+        s->d_bodies = nullptr;
+    }
+    HIPCHK(hipMalloc((void**)&s->d_bodies, (size_t)n_bodies * sizeof(upf_body)));
+    HIPCHK(hipMalloc((void**)&s->d_body_start, (size_t)(n_bodies + 1) * sizeof(int)));
+    if (!s->d_body_keys) {
+        HIPCHK(s->alloc(&s->d_body_keys, s->cap));
+        HIPCHK(s->alloc(&s->d_body_keys2, s->cap));
+        HIPCHK(s->alloc(&s->d_body_members, s->cap));
+    }
+    HIPCHK(hipMemcpy(s->d_bodies, bodies, (size_t)n_bodies * sizeof(upf_body), hipMemcpyHostToDevice));
+    // per-particle body data in the current slot order: slots are still in
+    // upload order unless a step ran, so scatter through the ids
+    std::vector<int> id(n), bid(n);
+    std::vector<float> rel((size_t)4 * n), psi(n);
+    Slots& S = s->S[s->cur];
+    HIPCHK(hipMemcpy(id.data(), S.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(psi.data(), S.psi, (size_t)n * 4, hipMemcpyDeviceToHost));
+    for (int k = 0; k < n; ++k) {
+        const int u = id[k];
+        bid[k] = body_id[u];
+        if (bid[k] >= n_bodies) return UPF_ERR_ARG;
+        for (int a = 0; a < 4; ++a) rel[4 * k + a] = rel4[4 * u + a];
+        // a rigid particle's weighted_volume_ = its mass / rest density (UnifiedParticle.cpp:24)
+        if (bid[k] >= 0) psi[k] = rel4[4 * u + 3] / s->P.rest_density;
+    }
+    HIPCHK(hipMemcpy(S.body, bid.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(S.rel, rel.data(), (size_t)n * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(S.psi, psi.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    s->n_bodies = n_bodies;
+    s->rigid = true;
+    return UPF_OK;
+}
+
+int upf_download_bodies(upf_sim* s, upf_body* out) {
+    if (!s || !out) return UPF_ERR_ARG;
+    if (!s->d_bodies) return UPF_ERR_STATE;
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipMemcpy(out, s->d_bodies, (size_t)s->n_bodies * sizeof(upf_body), hipMemcpyDeviceToHost));
+    return UPF_OK;
+}
+
 int upf_get_step_stats(upf_sim* s, int max_steps, int* iterations, float* max_rho) {
     if (!s) return UPF_ERR_ARG;
     const int m = (int)s->k_hist.size();
@@ -1332,6 +1818,3 @@ int upf_debug_box_cells(const upf_params* p, float x, float y, float z, uint32_t
 
 }  // extern "C"
 
-// rigid-body step lives in upf_rigid.hip; until it is built this stub keeps
-// the method unavailable.
-__attribute__((weak)) int upf_step_rigid(upf_sim*) { return UPF_ERR_ARG; }
