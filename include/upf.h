@@ -273,20 +273,35 @@ int upf_radix_sort(int n, const uint32_t* keys_in, uint32_t* keys_out, int32_t* 
 
 /* ------------------------------------------------------ batched form */
 /* Many independent scenes stepped together (no reference counterpart; the
- * reference holds one simulation per process).  Scenes share the params
- * except for their domain, which comes from each scene's own params.      */
+ * reference holds one simulation per process).  Scenes share the physics
+ * params; each brings its own domain (z-index scales, cell grid, collision
+ * box) and density error factor in its own params, and its particles occupy
+ * a fixed slot range.  One
+ * launch per stage serves every scene: per-scene cell tables and loop
+ * control, every scene running its own PCISPH loop to its own convergence.
+ * Results per scene equal those of a plain simulation of that scene.       */
 typedef struct upf_batch upf_batch;
+/* counts[k] particles in scene k; max_record_steps bounds the per-step
+ * statistics kept on the device.                                          */
 int upf_batch_create(int n_scenes, const upf_params* scene_params,
-                     const int* counts, upf_batch** out);
+                     const int* counts, int max_record_steps, upf_batch** out);
 int upf_batch_destroy(upf_batch* b);
 /* Particles of all scenes concatenated scene by scene. */
 int upf_batch_upload(upf_batch* b, const float* pos4, const float* vel4, const int* type);
-/* nsteps PCISPH static-boundary steps of every scene.  use_graph: capture the
- * PCISPH correction loop of the step in a hipGraph with the convergence test
- * on the device.                                                           */
+/* nsteps PCISPH static-boundary steps of every scene.  use_graph != 0: each
+ * step replays one hipGraph holding the whole step, the correction loop
+ * included (max_loops iterations, the convergence test on the device;
+ * iterations past every scene's decision exit at once), with no host
+ * round trip; errors are checked once at the end.  use_graph == 0: the host
+ * reads the loop decision back as upf_step does.                          */
 int upf_batch_step(upf_batch* b, int nsteps, int use_graph);
+/* Concatenated upload order, like upf_batch_upload. */
 int upf_batch_download(upf_batch* b, float* pos4, float* vel4, float* rho);
-int upf_batch_get_iterations(upf_batch* b, int max_entries, int* iterations);
+/* iterations / max_rho as [step][scene] for up to max_steps steps; returns
+ * the number of steps recorded (negative UPF_ERR_* on failure).           */
+int upf_batch_get_stats(upf_batch* b, int max_steps, int* iterations, float* max_rho);
+/* The batch's simulation handle, for the profiling and download calls.    */
+int upf_batch_sim(upf_batch* b, upf_sim** sim);
 
 /* ------------------------------------------------------ profiling */
 /* Per-kernel-class device time measured with HIP events recorded on the

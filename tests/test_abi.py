@@ -76,3 +76,29 @@ def test_density_error_factor_rejects_reference_ub_spacing():
     p = upf.make_params(0.0125)
     rc = upf.lib().upf_compute_density_error_factor(C.byref(p), None)
     assert rc == upf._abi.UPF_ERR_ARG
+
+
+def test_batch_create_validates_before_touching_a_device():
+    """Scenes must share the physics (domains and density error factors may
+    differ); counts must be positive.  Argument errors come before the device
+    query, so they show without a GPU."""
+    scenes = upf.batch_dam_breaks(3, seed=4, count_range=(200, 2000))
+    ps = [s[0] for s in scenes]
+    counts = [len(s[2]) for s in scenes]
+    L = upf.lib()
+    h = C.c_void_p()
+
+    def create(params, cnt):
+        arr = (Params * len(params))(*params)
+        c = np.asarray(cnt, np.int32)
+        return L.upf_batch_create(len(params), arr, c.ctypes.data_as(C.POINTER(C.c_int)), 16, C.byref(h))
+
+    bad = [p.copy() for p in ps]
+    bad[1].visc_const *= 2.0
+    assert create(bad, counts) == upf._abi.UPF_ERR_ARG
+    assert create(ps, [counts[0], 0, counts[2]]) == upf._abi.UPF_ERR_ARG
+    rc = create(ps, counts)
+    if rc == upf._abi.UPF_OK:  # a device is present
+        L.upf_batch_destroy(h)
+    else:
+        assert rc == upf._abi.UPF_ERR_NO_DEVICE

@@ -59,6 +59,17 @@ def lib():
         L.upf_radix_sort.argtypes = [C.c_int, pu, pu, pi]
         L.upf_debug_box_cells.argtypes = [pp, C.c_float, C.c_float, C.c_float, pu, C.c_int]
         L.upf_version.restype = C.c_char_p
+        L.upf_batch_create.argtypes = [C.c_int, pp, pi, C.c_int, C.POINTER(vp)]
+        L.upf_batch_destroy.argtypes = [vp]
+        L.upf_batch_upload.argtypes = [vp, pf, pf, pi]
+        L.upf_batch_step.argtypes = [vp, C.c_int, C.c_int]
+        L.upf_batch_download.argtypes = [vp, pf, pf, pf]
+        L.upf_batch_get_stats.argtypes = [vp, C.c_int, pi, pf]
+        L.upf_batch_sim.argtypes = [vp, C.POINTER(vp)]
+        L.upf_profile_enable.argtypes = [vp, C.c_int]
+        L.upf_profile_reset.argtypes = [vp]
+        L.upf_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+        L.upf_profile_name.restype = C.c_char_p
         for name, args in (("upf_upload_bodies", [vp, C.c_int, C.POINTER(Body), pi, pf]),
                            ("upf_download_bodies", [vp, C.POINTER(Body)])):
             if hasattr(L, name):
@@ -259,3 +270,97 @@ def device_radix_sort(keys):
     check("upf_radix_sort", lib().upf_radix_sort(len(keys), ptr(keys, C.c_uint32), ptr(out, C.c_uint32),
                                                  ptr(perm, C.c_int)))
     return out, perm
+
+
+def profile(handle):
+    """Per-kernel-class device time {name: (total_ms, launches)} of a simulation handle."""
+    L = lib()
+    out = {}
+    for c in range(L.upf_profile_classes()):
+        ms, cnt = C.c_double(), C.c_longlong()
+        L.upf_profile_read(handle, c, C.byref(ms), C.byref(cnt))
+        if cnt.value:
+            out[L.upf_profile_name(c).decode()] = (ms.value, cnt.value)
+    return out
+
+
+# ------------------------------------------------------------ batched form
+def batch_dam_breaks(n_scenes=256, seed=0, count_range=(2048, 65536), aspect_range=(0.5, 2.0), spacing=0.01,
+                     dt=0.001):
+    """Uneven independent dam breaks: scene k has a fluid block of about c_k
+    particles, c_k log-uniform in count_range, with block aspect ratio
+    a_k = width / height uniform in aspect_range (nx = nz = a ny), in the
+    corner of its own box of (2 nx, ny, nz) spacings with walls, jitter seed
+    seed + k.  Returns a list of (params, pos4, type, n_fluid)."""
+    rng = np.random.RandomState(seed)
+    lo, hi = np.log(count_range[0]), np.log(count_range[1])
+    scenes = []
+    for k in range(n_scenes):
+        c = float(np.exp(rng.uniform(lo, hi)))
+        a = float(rng.uniform(*aspect_range))
+        ny = max(2, int(round((c / (a * a)) ** (1.0 / 3.0))))
+        nx = max(2, int(round(a * ny)))
+        scenes.append(standard_dam_break(nx, ny, nx, spacing=spacing, dt=dt, seed=seed + k))
+    return scenes
+
+
+class Batch:
+    """Many independent scenes stepped together (upf_batch_*): one launch per
+    stage for all scenes, the whole step replayed as a hipGraph."""
+
+    def __init__(self, scene_params, counts, max_record_steps=1024):
+        arr = (Params * len(scene_params))(*scene_params)
+        cnt = np.ascontiguousarray(counts, np.int32)
+        self.n_scenes = len(scene_params)
+        self.counts = cnt
+        self.h = C.c_void_p()
+        check("upf_batch_create", lib().upf_batch_create(self.n_scenes, arr, ptr(cnt, C.c_int),
+                                                         int(max_record_steps), C.byref(self.h)))
+        self.n = int(cnt.sum())
+
+    @classmethod
+    def from_scenes(cls, scenes, max_record_steps=1024):
+        b = cls([sc[0] for sc in scenes], [len(sc[2]) for sc in scenes], max_record_steps)
+        b.upload(np.concatenate([sc[1] for sc in scenes]), np.concatenate([sc[2] for sc in scenes]))
+        return b
+
+    def upload(self, pos4, types, vel4=None):
+        pos4 = np.ascontiguousarray(pos4, np.float32)
+        types = np.ascontiguousarray(types, np.int32)
+        vel = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+        check("upf_batch_upload", lib().upf_batch_upload(self.h, ptr(pos4), ptr(vel), ptr(types, C.c_int)))
+
+    def step(self, nsteps=1, use_graph=True):
+        check("upf_batch_step", lib().upf_batch_step(self.h, int(nsteps), 1 if use_graph else 0))
+
+    def state(self):
+        n = self.n
+        out = dict(pos=np.zeros((n, 4), np.float32), vel=np.zeros((n, 4), np.float32), rho=np.zeros(n, np.float32))
+        check("upf_batch_download", lib().upf_batch_download(self.h, ptr(out["pos"]), ptr(out["vel"]),
+                                                             ptr(out["rho"])))
+        return out
+
+    def stats(self, max_steps=4096):
+        K = np.zeros((max_steps, self.n_scenes), np.int32)
+        r = np.zeros((max_steps, self.n_scenes), np.float32)
+        m = lib().upf_batch_get_stats(self.h, max_steps, ptr(K, C.c_int), ptr(r))
+        if m < 0:
+            raise UPFError("upf_batch_get_stats", -m)
+        m = min(m, max_steps)
+        return K[:m], r[:m]
+
+    def sim_handle(self):
+        s = C.c_void_p()
+        check("upf_batch_sim", lib().upf_batch_sim(self.h, C.byref(s)))
+        return s
+
+    def close(self):
+        if self.h:
+            lib().upf_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

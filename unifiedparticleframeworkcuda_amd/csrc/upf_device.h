@@ -70,18 +70,21 @@ struct BoxCells {
     int nd;         // number of splits
 
     __host__ __device__ void init(float x, float y, float z, const upf_params& P) {
-        const float h = P.support_radius;
+        init(x, y, z, P.support_radius, P.scales, P.block_shift);
+    }
+
+    __host__ __device__ void init(float x, float y, float z, float h, const float* scales, uint32_t block_shift) {
         const float c[3] = {x, y, z};
         uint32_t l3[3];
         int n3[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            float lo_f = P.scales[a] * (c[a] - h);
-            float hi_f = P.scales[a] * (c[a] + h);
+            float lo_f = scales[a] * (c[a] - h);
+            float hi_f = scales[a] * (c[a] + h);
             lo_f = lo_f >= 0.0f ? lo_f : 0.0f;       // MAX(min_query_box, 0)
             hi_f = hi_f <= 1024.0f ? hi_f : 1024.0f; // MIN(max_query_box, GRID_RESOLUTION)
-            const uint32_t l = ((uint32_t)(unsigned short)lo_f) >> P.block_shift;
-            const uint32_t u = ((uint32_t)(unsigned short)hi_f) >> P.block_shift;
+            const uint32_t l = ((uint32_t)(unsigned short)lo_f) >> block_shift;
+            const uint32_t u = ((uint32_t)(unsigned short)hi_f) >> block_shift;
             l3[a] = l;
             n3[a] = (int)(u - l) + 1;
         }

@@ -16,15 +16,29 @@ constexpr int BLOCK = 256;
 // Loop control of the PCISPH correction loop, kept on the device.
 constexpr int MAX_SLOTS = 64;  // the max reduction spreads its atomics over 64 cache lines
 
-struct Ctl {
-    uint32_t max_bits;   // bit pattern of the max predicted density (non-negative floats order like uints)
-    uint32_t done;       // unused (kept for layout)
+struct Ctl {              // whole simulation
+    int want_any;        // largest number of iterations any scene's loop must run
+    uint32_t err;        // bit (1 << UPF_ERR_*) set on failures
+    uint32_t step;       // steps recorded in the per-scene statistics
+    uint32_t pad;
+};
+
+struct SceneCtl {         // one scene's PCISPH loop
     int want;            // number of iterations the loop must run (grows as the loop decides)
     int too_large;       // density_error_too_large_ latch (src/cpu/UnifiedPhysics.cpp:5373,5397-5398)
     float last_max;      // max predicted density of the last completed iteration
-    uint32_t err;        // bit (1 << UPF_ERR_*) set on failures
-    uint32_t nbr_max;    // largest neighbour count of the step
-    uint32_t pad;
+    int pad;
+};
+
+// The domain of one scene: its z-index grid, collision box, cell table, slot
+// range and PCISPH density error factor.  A plain simulation is one scene.
+struct Dom {
+    float scales[3];
+    float cmin[3], cmax[3];
+    uint32_t block_shift, grid_dim;
+    uint32_t cell_off;   // first entry of the scene's cell table
+    int slot_off, n;     // the scene's slot range
+    float delta;         // density error factor (computed on the scene's own prototype block)
 };
 
 // Per-particle state of one sorted slot range (scene).  Arrays indexed by slot.

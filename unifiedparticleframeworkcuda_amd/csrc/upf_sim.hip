@@ -39,13 +39,36 @@ __device__ inline int ptype(const float4& x) { return __float_as_int(x.w); }
 
 __device__ inline void set_err(Ctl* ctl, int code) { atomicOr(&ctl->err, 1u << code); }
 
+struct StepArgs {
+    upf_params P;        // physics shared by all scenes
+    float lut_factor;    // (float)(1.0 / h)
+    float inv_mass_f;    // 1.0f / m (vmmlib operator/(float))
+    float inv_mass_d;    // (float)(1.0 / m) (static const float invMass, :3714)
+    float rest_vol;      // m / rho0
+    float dens_sq;       // rho0 * rho0
+    int n;
+    int stride;          // column stride of the neighbour lists
+    Dom dom0;            // the domain when there is one scene
+    const Dom* doms;     // per-scene domains (batched form), else null
+    const int* scene_of; // scene of every slot (batched form), else null
+    SceneCtl* sctl;      // per-scene loop control
+    uint32_t* max_slots; // per-scene max slots (x 32 uints of padding)
+    int n_scenes;
+    int slots_per_scene;
+};
+
+__device__ inline int scene_at(const StepArgs& A, int i) { return A.scene_of ? A.scene_of[i] : 0; }
+__device__ inline Dom dom_at(const StepArgs& A, int i) { return A.scene_of ? A.doms[A.scene_of[i]] : A.dom0; }
+__device__ inline int gstride() { return gridDim.x * blockDim.x; }
+
 // ------------------------------------------------------------------ keys
-__global__ void k_keys(int n, const float4* __restrict__ pos, uint32_t* __restrict__ keys, upf_params P, Ctl* ctl) {
+__global__ void k_keys(StepArgs A, const float4* __restrict__ pos, uint32_t* __restrict__ keys, Ctl* ctl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= A.n) return;
     const float4 x = pos[i];
+    const Dom d = dom_at(A, i);
     bool ok;
-    keys[i] = dkey(x.x, x.y, x.z, P.scales, ok);
+    keys[i] = dkey(x.x, x.y, x.z, d.scales, ok);
     if (!ok) set_err(ctl, UPF_ERR_RANGE);
 }
 
@@ -80,39 +103,47 @@ __global__ void k_fill_cells(uint2* __restrict__ cells, uint32_t ncells) {
 
 // first / end slot of every occupied cell by boundary detection over the
 // sorted keys (BlockGeneration, src/gpu/particles.cu:904-964, without atomics)
-__global__ void k_cell_bounds(int n, const uint32_t* __restrict__ keys, int shift3, uint2* __restrict__ cells) {
+__global__ void k_cell_bounds(StepArgs A, const uint32_t* __restrict__ keys, uint2* __restrict__ cells) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= A.n) return;
+    const Dom d = dom_at(A, i);
+    const uint32_t shift3 = 3u * d.block_shift;
     const uint32_t c = keys[i] >> shift3;
-    if (i == 0 || (keys[i - 1] >> shift3) != c) cells[c].x = (uint32_t)i;
-    if (i == n - 1 || (keys[i + 1] >> shift3) != c) cells[c].y = (uint32_t)(i + 1);
+    uint2* T = cells + d.cell_off;
+    if (i == d.slot_off || (keys[i - 1] >> shift3) != c) T[c].x = (uint32_t)i;
+    if (i == d.slot_off + d.n - 1 || (keys[i + 1] >> shift3) != c) T[c].y = (uint32_t)(i + 1);
 }
 
 // Visit the candidates of the reference's query box around x in ascending
 // slot order; f(j) is called for every j in the covered cells.
 template <class F>
-__device__ __forceinline__ void for_each_candidate(const float4& x, const upf_params& P, const uint2* __restrict__ cells, F f) {
+__device__ __forceinline__ void for_each_candidate(const float4& x, float h, const Dom& d, const uint2* __restrict__ cells,
+                                                   F f) {
     BoxCells bc;
-    bc.init(x.x, x.y, x.z, P);
+    bc.init(x.x, x.y, x.z, h, d.scales, d.block_shift);
     const int cnt = bc.count();
+    const uint2* T = cells + d.cell_off;
     for (int e = 0; e < cnt; ++e) {
         uint32_t c[3];
         if (!bc.cell(e, c)) continue;
-        if (c[0] >= P.grid_dim || c[1] >= P.grid_dim || c[2] >= P.grid_dim) continue;
-        const uint2 r = cells[dmorton3(c[0], c[1], c[2])];
+        if (c[0] >= d.grid_dim || c[1] >= d.grid_dim || c[2] >= d.grid_dim) continue;
+        const uint2 r = T[dmorton3(c[0], c[1], c[2])];
         for (uint32_t j = r.x; j < r.y; ++j) f((int)j);
     }
 }
 
 // --------------------------------------------------- boundary volume (Psi)
 // CalculateWeightedVolume (src/cpu/UnifiedPhysics.cpp:2703-2746): Psi_b = 1 / sum_b W
-__global__ void k_psi(int n, const float4* __restrict__ pos, const uint2* __restrict__ cells, upf_params P,
-                      const float* __restrict__ luts, float lut_factor, float* __restrict__ psi, Ctl* ctl) {
+__global__ void k_psi(StepArgs A, const float4* __restrict__ pos, const uint2* __restrict__ cells,
+                      const float* __restrict__ luts, float* __restrict__ psi, Ctl* ctl) {
     __shared__ float lw[LUT_N];
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
     __syncthreads();
+    const upf_params& P = A.P;
+    const float lut_factor = A.lut_factor;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= A.n) return;
+    const Dom d = dom_at(A, i);
     const float4 xi = pos[i];
     const int ti = ptype(xi);
     if (ti != UPF_FROZEN && ti != UPF_RIGID) return;
@@ -120,7 +151,7 @@ __global__ void k_psi(int n, const float4* __restrict__ pos, const uint2* __rest
     // Akinci et al. 2012 eq. 4 sums over the boundary neighbourhood including the
     // particle itself (W(0) = the table's entry 0); the CPU path excludes it
     float inv = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? lw[0] : 0.0f;
-    for_each_candidate(xi, P, cells, [&](int j) __attribute__((always_inline)) {
+    for_each_candidate(xi, P.support_radius, d, cells, [&](int j) __attribute__((always_inline)) {
         if (j == i) return;
         const float4 xj = pos[j];
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -135,32 +166,22 @@ __global__ void k_psi(int n, const float4* __restrict__ pos, const uint2* __rest
     else set_err(ctl, UPF_ERR_ISOLATED);
 }
 
-struct StepArgs {
-    upf_params P;
-    float lut_factor;   // (float)(1.0 / h)
-    float inv_mass_f;   // 1.0f / m (vmmlib operator/(float))
-    float inv_mass_d;   // (float)(1.0 / m) (static const float invMass, :3714)
-    float rest_vol;     // m / rho0
-    float dens_sq;      // rho0 * rho0
-    int n;
-    int stride;         // column stride of the neighbour lists
-};
 
 // Predicted position + CollisionHandling (src/cpu/UnifiedPhysics.cpp:5426-5454, :5739-5757)
 __device__ inline float4 predict(const float4& x, const float4& v, const float4& F, const float4& Fp,
-                                 const StepArgs& A, Ctl* ctl) {
+                                 const StepArgs& A, const Dom& d, Ctl* ctl) {
     const upf_params& P = A.P;
     float ax = F.x + Fp.x, ay = F.y + Fp.y, az = F.z + Fp.z;
     ax = ax * P.dt; ay = ay * P.dt; az = az * P.dt;
     ax = ax * A.inv_mass_f; ay = ay * A.inv_mass_f; az = az * A.inv_mass_f;
     const float vx = v.x + ax, vy = v.y + ay, vz = v.z + az;
     float px = x.x + vx * P.dt, py = x.y + vy * P.dt, pz = x.z + vz * P.dt;
-    px = px < P.collision_min[0] ? P.collision_min[0] : px;
-    px = px > P.collision_max[0] ? P.collision_max[0] : px;
-    py = py < P.collision_min[1] ? P.collision_min[1] : py;
-    py = py > P.collision_max[1] ? P.collision_max[1] : py;
-    pz = pz < P.collision_min[2] ? P.collision_min[2] : pz;
-    pz = pz > P.collision_max[2] ? P.collision_max[2] : pz;
+    px = px < d.cmin[0] ? d.cmin[0] : px;
+    px = px > d.cmax[0] ? d.cmax[0] : px;
+    py = py < d.cmin[1] ? d.cmin[1] : py;
+    py = py > d.cmax[1] ? d.cmax[1] : py;
+    pz = pz < d.cmin[2] ? d.cmin[2] : pz;
+    pz = pz > d.cmax[2] ? d.cmax[2] : pz;
     if (!(px == px) || !(py == py) || !(pz == pz)) set_err(ctl, UPF_ERR_NAN);
     return make_float4(px, py, pz, x.w);
 }
@@ -197,18 +218,19 @@ __device__ inline uint32_t lut_bin(float dist, float factor) {
 // a time against each particle of the cell: the candidate's cell must lie in
 // the particle's query box (the reference's BoxQuery) and its distance must be
 // below h.  Ballots keep each list in ascending order.
-__global__ __launch_bounds__(64) void k_nbr_build(int n, const float4* __restrict__ pos,
+__global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
-                                                  upf_params P, float lut_factor, uint32_t* __restrict__ nbr, int stride,
-                                                  int* __restrict__ ncount, Ctl* ctl) {
+                                                  uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
     __shared__ int pcnt[64];
+    const upf_params& P = A.P;
+    const float lut_factor = A.lut_factor;
+    const int n = A.n;
+    const int stride = A.stride;
     const int lane = threadIdx.x;
     const int base = blockIdx.x * 64;
     if (base >= n) return;
     const int wend = min(base + 64, n);
-    const uint32_t shift3 = 3u * P.block_shift;
-    const uint32_t G = P.grid_dim;
     const float T = P.support_radius_sq_limit;
     const float h = P.support_radius;
     const int cap = P.max_neighbors;
@@ -216,8 +238,12 @@ __global__ __launch_bounds__(64) void k_nbr_build(int n, const float4* __restric
     pcnt[lane] = 0;
     int s = base;
     while (s < wend) {
+        const Dom d = dom_at(A, s);  // the cell's scene
+        const uint32_t shift3 = 3u * d.block_shift;
+        const uint32_t G = d.grid_dim;
+        const uint2* T3 = cells + d.cell_off;
         const uint32_t c = keys[s] >> shift3;
-        const uint2 cr = cells[c];
+        const uint2 cr = T3[c];
         const int pe = min((int)cr.y, wend);
         const uint32_t cc[3] = {compact10(c), compact10(c >> 1), compact10(c >> 2)};
         // stencil cells in Morton order: lane e names enumeration index e
@@ -239,7 +265,7 @@ __global__ __launch_bounds__(64) void k_nbr_build(int n, const float4* __restric
         if (lane < bc.count()) {
             uint32_t q[3];
             if (bc.cell(lane, q)) {
-                const uint2 r = cells[dmorton3(q[0], q[1], q[2])];
+                const uint2 r = T3[dmorton3(q[0], q[1], q[2])];
                 valid = r.y > r.x;
                 rs = r.x;
                 rl = r.y - r.x;
@@ -281,12 +307,12 @@ __global__ __launch_bounds__(64) void k_nbr_build(int n, const float4* __restric
                 const float xv[3] = {xp.x, xp.y, xp.z};
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    float lo_f = P.scales[a] * (xv[a] - h);
-                    float hi_f = P.scales[a] * (xv[a] + h);
+                    float lo_f = d.scales[a] * (xv[a] - h);
+                    float hi_f = d.scales[a] * (xv[a] + h);
                     lo_f = lo_f >= 0.0f ? lo_f : 0.0f;
                     hi_f = hi_f <= 1024.0f ? hi_f : 1024.0f;
-                    blo[a] = ((uint32_t)(unsigned short)lo_f) >> P.block_shift;
-                    bhi[a] = ((uint32_t)(unsigned short)hi_f) >> P.block_shift;
+                    blo[a] = ((uint32_t)(unsigned short)lo_f) >> d.block_shift;
+                    bhi[a] = ((uint32_t)(unsigned short)hi_f) >> d.block_shift;
                 }
                 int cnt = pcnt[p - base];
 #pragma unroll
@@ -339,13 +365,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     }
     __syncthreads();
     const upf_params& P = A.P;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->too_large = 1;
-        ctl->want = (0 < P.min_loops || 0 < P.max_loops) ? 1 : 0;
-        ctl->last_max = 0.0f;
-    }
-    if (i >= A.n) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
     const float4 xi = S.pos[i];
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
@@ -355,7 +375,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     if (ti != UPF_LIQUID) {
         W.xs[i] = (ti == UPF_RIGID && !P.rigid_predicted_current) ? S.x0[i] : xi;
         W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        return;
+        continue;
     }
     const float4 vi = S.vel[i];
     const float h = P.support_radius;
@@ -423,7 +443,8 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     fy = fy - P.initial_mass * P.gravity;
     const float4 F = make_float4(fx, fy, fz, 0.0f);
     W.force[i] = F;
-    W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, ctl);
+    W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, dom_at(A, i), ctl);
+    }
 }
 
 // ------------------------------------------------------ PCISPH iteration
@@ -433,101 +454,177 @@ __device__ inline float wave_max(float v) {
 }
 
 // ComputePredictedDensityAndPressure (:5456-5491) + the max reduction of the
-// predicted density: each workgroup folds its max into one of MAX_SLOTS
-// padded slots (atomics spread over 64 cache lines).  Iteration t runs only
-// while t < ctl->want.
+// predicted density per scene: maxima fold into a few padded slots per scene
+// (atomics spread over cache lines).  Persistent grid-stride kernel; iteration
+// t of a scene runs only while t < its want, and the kernel returns at once
+// when no scene runs iteration t.
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                    Ctl* ctl) {
-    if (t >= __atomic_load_n(&ctl->want, __ATOMIC_RELAXED)) return;
+    if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
     __shared__ float lw[LUT_N];
     __shared__ float wmax[BLOCK / 64];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lw[k] = luts[k];
     __syncthreads();
     const upf_params& P = A.P;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    float mine = 0.0f;
-    if (i < A.n) {
-        const float4 xi = W.xs[i];
-        if (ptype(xi) == UPF_LIQUID) {
-            const float m = P.initial_mass;
-            const float T = P.support_radius_sq_limit;
-            float rho = P.kernel_self * m;
-            const int cnt = W.ncount[i];
-            const uint32_t* L = W.nbr + i;
-            const size_t st = (size_t)A.stride;
-            int k = 0;
-            // four neighbours per round: their loads are issued together,
-            // the sum still runs in list order
-            for (; k + 4 <= cnt; k += 4) {
-                int j[4];
-                float4 xj[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) j[u] = (int)(L[(k + u) * st] >> NB_SLOT_SHIFT);
-#pragma unroll
-                for (int u = 0; u < 4; ++u) xj[u] = W.xs[j[u]];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const float dx = xi.x - xj[u].x, dy = xi.y - xj[u].y, dz = xi.z - xj[u].z;
+    const int lane = threadIdx.x & 63;
+    const int sps = A.slots_per_scene;
+    float run = 0.0f;  // single scene: running max over the thread's slots
+    const int n_round = (A.n + gstride() - 1) / gstride();
+    for (int r = 0; r < n_round; ++r) {
+        const int i = r * gstride() + blockIdx.x * blockDim.x + threadIdx.x;
+        float mine = 0.0f;
+        int sc = -1;
+        if (i < A.n) {
+            const float4 xi = W.xs[i];
+            sc = scene_at(A, i);
+            if (ptype(xi) == UPF_LIQUID && t < A.sctl[sc].want) {
+                const float m = P.initial_mass;
+                const float T = P.support_radius_sq_limit;
+                float rho = P.kernel_self * m;
+                const int cnt = W.ncount[i];
+                const uint32_t* L = W.nbr + i;
+                const size_t st = (size_t)A.stride;
+                int k = 0;
+                // four neighbours per round: their loads are issued together,
+                // the sum still runs in list order
+                for (; k + 4 <= cnt; k += 4) {
+                    int j[4];
+                    float4 xj[4];
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u) j[u] = (int)(L[(k + u) * st] >> NB_SLOT_SHIFT);
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u) xj[u] = W.xs[j[u]];
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float dx = xi.x - xj[u].x, dy = xi.y - xj[u].y, dz = xi.z - xj[u].z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        if (d2 < T) {
+                            const float w = lut_at(lw, sqrtf(d2), A.lut_factor);
+                            float mj = m;
+                            if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj[u]) != UPF_LIQUID) mj = P.rest_density * S.psi[j[u]];
+                            rho = rho + w * mj;
+                        }
+                    }
+                }
+                for (; k < cnt; ++k) {
+                    const int j = (int)(L[k * st] >> NB_SLOT_SHIFT);
+                    const float4 xj = W.xs[j];
+                    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                     const float d2 = dx * dx + dy * dy + dz * dz;
                     if (d2 < T) {
                         const float w = lut_at(lw, sqrtf(d2), A.lut_factor);
                         float mj = m;
-                        if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj[u]) != UPF_LIQUID) mj = P.rest_density * S.psi[j[u]];
+                        if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
                         rho = rho + w * mj;
                     }
                 }
+                const float err = fmax_ref(rho - P.rest_density, 0.0f);
+                const float delta = A.scene_of ? A.doms[sc].delta : P.density_error_factor;
+                const float p = W.p[i] + fmax_ref(err * delta, 0.0f);
+                W.p[i] = p;
+                W.pdiv[i] = p / (P.rest_density * P.rest_density);
+                W.rho[i] = rho;
+                mine = rho;
+            } else {
+                sc = -1;
             }
-            for (; k < cnt; ++k) {
-                const int j = (int)(L[k * st] >> NB_SLOT_SHIFT);
-                const float4 xj = W.xs[j];
-                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                if (d2 < T) {
-                    const float w = lut_at(lw, sqrtf(d2), A.lut_factor);
-                    float mj = m;
-                    if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
-                    rho = rho + w * mj;
-                }
-            }
-            const float err = fmax_ref(rho - P.rest_density, 0.0f);
-            const float p = W.p[i] + fmax_ref(err * P.density_error_factor, 0.0f);
-            W.p[i] = p;
-            W.pdiv[i] = p / (P.rest_density * P.rest_density);
-            W.rho[i] = rho;
-            mine = rho;
+        }
+        if (A.scene_of == nullptr) {
+            run = fmaxf(run, mine);
+            continue;
+        }
+        // several scenes: reduce per wave when the wave holds one scene
+        const uint64_t act = __ballot(sc >= 0);
+        if (act == 0ull) continue;
+        const int first = __shfl(sc, __ffsll((long long)act) - 1, 64);
+        const bool uniform = __ballot(sc >= 0 && sc != first) == 0ull;
+        if (uniform) {
+            const float m = wave_max(mine);
+            if (lane == 0)
+                atomicMax(&A.max_slots[(first * sps + ((blockIdx.x * 4 + (threadIdx.x >> 6)) % sps)) * 32],
+                          __float_as_uint(m));
+        } else if (sc >= 0) {
+            atomicMax(&A.max_slots[(sc * sps + lane % sps) * 32], __float_as_uint(mine));
         }
     }
-    mine = wave_max(mine);
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mine;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float b = wmax[0];
-        for (int k = 1; k < BLOCK / 64; ++k) b = fmaxf(b, wmax[k]);
-        if (b > 0.0f) atomicMax(&W.max_slots[(blockIdx.x & (MAX_SLOTS - 1)) * 32], __float_as_uint(b));
+    if (A.scene_of == nullptr) {
+        run = wave_max(run);
+        if (lane == 0) wmax[threadIdx.x >> 6] = run;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float b = wmax[0];
+            for (int k = 1; k < BLOCK / 64; ++k) b = fmaxf(b, wmax[k]);
+            if (b > 0.0f) atomicMax(&A.max_slots[(blockIdx.x % sps) * 32], __float_as_uint(b));
+        }
     }
 }
 
-// The loop test of PredictionCorrectionStep (:5376-5398) on the device: max
-// over the slots, densityErrorInPercent, the too-large latch and the number
-// of iterations the loop runs.  One wave.
-__global__ void k_loop_control(int t, const upf_params P, uint32_t* __restrict__ max_slots, Ctl* ctl) {
-    if (t >= __atomic_load_n(&ctl->want, __ATOMIC_RELAXED)) return;
-    const int l = threadIdx.x;
-    uint32_t b = max_slots[l * 32];
-    max_slots[l * 32] = 0u;
-    for (int o = 32; o > 0; o >>= 1) b = max(b, (uint32_t)__shfl_xor((int)b, o, 64));
-    if (l == 0) {
-        const float mx = __uint_as_float(b);
-        const float pct = fmax_ref(0.1f * mx - 100.0f, 0.0f);
-        int too_large = ctl->too_large;
-        if (pct < P.max_density_error_allowed) too_large = 0;
-        const int it = t + 1;
-        const bool next = it < P.min_loops || (too_large && it < P.max_loops);
-        ctl->too_large = too_large;
-        ctl->last_max = mx;
-        __atomic_store_n(&ctl->want, next ? it + 1 : it, __ATOMIC_RELAXED);
+// The loop test of PredictionCorrectionStep (:5376-5398) on the device, one
+// thread per scene: max over the scene's slots, densityErrorInPercent, the
+// too-large latch and the number of iterations the scene's loop runs; the
+// largest of those goes to ctl->want_any.
+__global__ __launch_bounds__(256) void k_loop_control(int t, StepArgs A, Ctl* ctl) {
+    if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
+    __shared__ int wmx[4];
+    const upf_params& P = A.P;
+    int best = 0;
+    for (int sc = threadIdx.x; sc < A.n_scenes; sc += blockDim.x) {
+        SceneCtl& C = A.sctl[sc];
+        if (t < C.want) {
+            uint32_t b = 0u;
+            for (int k = 0; k < A.slots_per_scene; ++k) {
+                uint32_t* slot = &A.max_slots[(sc * A.slots_per_scene + k) * 32];
+                b = max(b, *slot);
+                *slot = 0u;
+            }
+            const float mx = __uint_as_float(b);
+            const float pct = fmax_ref(0.1f * mx - 100.0f, 0.0f);
+            int too_large = C.too_large;
+            if (pct < P.max_density_error_allowed) too_large = 0;
+            const int it = t + 1;
+            const bool next = it < P.min_loops || (too_large && it < P.max_loops);
+            C.too_large = too_large;
+            C.last_max = mx;
+            C.want = next ? it + 1 : it;
+        }
+        best = max(best, C.want);
     }
+    for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o, 64));
+    if ((threadIdx.x & 63) == 0) wmx[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int b = wmx[0];
+        for (int k = 1; k < (int)(blockDim.x + 63) / 64; ++k) b = max(b, wmx[k]);
+        __atomic_store_n(&ctl->want_any, b, __ATOMIC_RELAXED);
+    }
+}
+
+// Start of a step's loop: every scene runs iteration 0 (the CPU loop starts
+// with density_error_too_large_ = true) unless its loop limits are zero.
+__global__ void k_loop_init(StepArgs A, Ctl* ctl) {
+    const upf_params& P = A.P;
+    const int w = (0 < P.min_loops || 0 < P.max_loops) ? 1 : 0;
+    for (int sc = blockIdx.x * blockDim.x + threadIdx.x; sc < A.n_scenes; sc += gstride()) {
+        A.sctl[sc].want = w;
+        A.sctl[sc].too_large = 1;
+        A.sctl[sc].last_max = 0.0f;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->want_any = w;
+}
+
+// Per-step record of every scene's iteration count and last max density
+// (one block), then the step counter advances.
+__global__ void k_record(StepArgs A, Ctl* ctl, int* __restrict__ k_out, float* __restrict__ r_out, int cap_steps) {
+    const uint32_t step = ctl->step;
+    if ((int)step < cap_steps) {
+        for (int sc = threadIdx.x; sc < A.n_scenes; sc += blockDim.x) {
+            k_out[(size_t)step * A.n_scenes + sc] = A.sctl[sc].want;
+            r_out[(size_t)step * A.n_scenes + sc] = A.sctl[sc].last_max;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ctl->step = step + 1u;
 }
 
 // ComputeCorrectivePressureForce (:5493-5522), then the prediction of the
@@ -535,15 +632,16 @@ __global__ void k_loop_control(int t, const upf_params P, uint32_t* __restrict__
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                  Ctl* ctl) {
-    if (t >= __atomic_load_n(&ctl->want, __ATOMIC_RELAXED)) return;
+    if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
     __shared__ float lpg[LUT_N];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lpg[k] = luts[LUT_N + k];
     __syncthreads();
     const upf_params& P = A.P;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
     const float4 xi = S.pos[i];
-    if (ptype(xi) != UPF_LIQUID) return;
+    if (ptype(xi) != UPF_LIQUID) continue;
+    const int sc = scene_at(A, i);
+    if (t >= A.sctl[sc].want) continue;
     const float m = P.initial_mass;
     const float T = P.support_radius_sq_limit;
     const float rho0 = P.rest_density;
@@ -589,7 +687,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     }
     const float4 Fp = make_float4(fx, fy, fz, 0.0f);
     W.fp[i] = Fp;
-    W.xs[i] = predict(xi, S.vel[i], W.force[i], Fp, A, ctl);
+    W.xs[i] = predict(xi, S.vel[i], W.force[i], Fp, A, dom_at(A, i), ctl);
+    }
 }
 
 // ------------------------------------------------------------ integration
@@ -597,16 +696,16 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
 // (:3940-3970) / BoundaryHandlingPureFluidOMP (:3972-3998): symplectic Euler,
 // box clamp with damping -0.5, new z-index key of every particle.
 __global__ void k_integrate(StepArgs A, Slots S, Work W, int pcisph, int pure_fluid_clamp, int has_bodies, Ctl* ctl) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n) return;
     const upf_params& P = A.P;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
     float4 x = S.pos[i];
     const int ti = ptype(x);
+    const Dom d = dom_at(A, i);
     if (ti == UPF_FROZEN) {
         bool ok;
-        W.keys[i] = dkey(x.x, x.y, x.z, P.scales, ok);
+        W.keys[i] = dkey(x.x, x.y, x.z, d.scales, ok);
         if (!ok) set_err(ctl, UPF_ERR_RANGE);
-        return;
+        continue;
     }
     float4 v = S.vel[i];
     if (ti == UPF_LIQUID) {
@@ -628,20 +727,21 @@ __global__ void k_integrate(StepArgs A, Slots S, Work W, int pcisph, int pure_fl
     const bool clamp = pure_fluid_clamp ? (ti == UPF_LIQUID) : (!has_bodies || S.body[i] < 0);
     if (clamp) {
         const float damping = -0.5f;
-        if (x.y < P.collision_min[1]) { x.y = P.collision_min[1]; v.y *= damping; }
-        if (x.y > P.collision_max[1]) { x.y = P.collision_max[1]; v.y *= damping; }
-        if (x.x < P.collision_min[0]) { x.x = P.collision_min[0]; v.x *= damping; }
-        if (x.x > P.collision_max[0]) { x.x = P.collision_max[0]; v.x *= damping; }
-        if (x.z < P.collision_min[2]) { x.z = P.collision_min[2]; v.z *= damping; }
-        if (x.z > P.collision_max[2]) { x.z = P.collision_max[2]; v.z *= damping; }
+        if (x.y < d.cmin[1]) { x.y = d.cmin[1]; v.y *= damping; }
+        if (x.y > d.cmax[1]) { x.y = d.cmax[1]; v.y *= damping; }
+        if (x.x < d.cmin[0]) { x.x = d.cmin[0]; v.x *= damping; }
+        if (x.x > d.cmax[0]) { x.x = d.cmax[0]; v.x *= damping; }
+        if (x.z < d.cmin[2]) { x.z = d.cmin[2]; v.z *= damping; }
+        if (x.z > d.cmax[2]) { x.z = d.cmax[2]; v.z *= damping; }
     }
     if (ti == UPF_LIQUID || clamp) {
         S.pos[i] = x;
         S.vel[i] = v;
     }
     bool ok;
-    W.keys[i] = dkey(x.x, x.y, x.z, P.scales, ok);
+    W.keys[i] = dkey(x.x, x.y, x.z, d.scales, ok);
     if (!ok) set_err(ctl, UPF_ERR_RANGE);
+    }
 }
 
 // ------------------------------------------------------------------- WCSPH
@@ -724,18 +824,21 @@ __global__ __launch_bounds__(BLOCK) void k_force_wcsph(StepArgs A, Slots S, Work
 // before the last rigid update (the reference recomputes keys before it moves
 // the rigid particles, src/cpu/UnifiedPhysics.cpp:5007-5022), so their query box
 // need not sit on their key's cell stencil: each lane walks its own box.
-__global__ __launch_bounds__(BLOCK) void k_nbr_build_box(int n, const float4* __restrict__ pos,
-                                                         const uint2* __restrict__ cells, upf_params P, float lut_factor,
-                                                         int owner_type, uint32_t* __restrict__ nbr, int stride,
-                                                         int* __restrict__ ncount, Ctl* ctl) {
+__global__ __launch_bounds__(BLOCK) void k_nbr_build_box(StepArgs A, const float4* __restrict__ pos,
+                                                         const uint2* __restrict__ cells, int owner_type,
+                                                         uint32_t* __restrict__ nbr, int* __restrict__ ncount,
+                                                         Ctl* ctl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= A.n) return;
     const float4 xi = pos[i];
     if (ptype(xi) != owner_type) return;
+    const upf_params& P = A.P;
+    const float lut_factor = A.lut_factor;
+    const int stride = A.stride;
     const float T = P.support_radius_sq_limit;
     const int cap = P.max_neighbors;
     int cnt = 0;
-    for_each_candidate(xi, P, cells, [&](int j) __attribute__((always_inline)) {
+    for_each_candidate(xi, P.support_radius, dom_at(A, i), cells, [&](int j) __attribute__((always_inline)) {
         if (j == i) return;
         const float4 xj = pos[j];
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -802,12 +905,8 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
     __syncthreads();
     const upf_params& P = A.P;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ctl->too_large = 1;
-        ctl->want = (0 < P.min_loops || 0 < P.max_loops) ? 1 : 0;
-        ctl->last_max = 0.0f;
-    }
     if (i >= A.n) return;
+    const Dom d = dom_at(A, i);
     const float4 xi = S.pos[i];
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
@@ -913,7 +1012,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
         float b[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const float mn = P.collision_min[a], mx = P.collision_max[a];
+            const float mn = d.cmin[a], mx = d.cmax[a];
             if (c[a] < mn + fd) {
                 const float v = (float)((double)((mn + fd - c[a]) * inv) * 2.0 * (double)fmax);
 #pragma unroll
@@ -928,7 +1027,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
         fx = fx + b[0]; fy = fy + b[1]; fz = fz + b[2];
         const float4 F = make_float4(fx, fy, fz, 0.0f);
         W.force[i] = F;
-        W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, ctl);
+        W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, d, ctl);
     } else {
         // CalculateBoundaryForcePerRigidParticleHost (:2071-2132)
         const float r = P.particle_radius, kb = P.spring_coefficient_boundary, cd = P.damping_coefficient;
@@ -937,7 +1036,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
         bool hit = false;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const float mn = P.collision_min[a], mx = P.collision_max[a];
+            const float mn = d.cmin[a], mx = d.cmax[a];
             if (c[a] - r < mn) {
                 hit = true;
                 const float v = mn + r - c[a];
@@ -1101,6 +1200,20 @@ __global__ void k_rigid_sync(int n, Slots S, const upf_body* __restrict__ bodies
     S.pos[i] = x;
 }
 
+// ------------------------------------------------------ batched sort
+// Second, stable pass of the batched sort: the scene of each key-sorted slot.
+__global__ void k_scene_keys(int n, const uint32_t* __restrict__ perm1, const int* __restrict__ scene_of,
+                             uint32_t* __restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) out[s] = (uint32_t)scene_of[perm1[s]];
+}
+
+__global__ void k_gather_keys(int n, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ keys,
+                              uint32_t* __restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) out[s] = keys[perm[s]];
+}
+
 // -------------------------------------------------------- id-order output
 __global__ void k_scatter_by_id(int n, const int* __restrict__ id, const float4* __restrict__ a4, float4* __restrict__ o4,
                                 const float* __restrict__ a1, float* __restrict__ o1) {
@@ -1188,6 +1301,21 @@ struct upf_sim {
     Work W;
     Ctl* ctl = nullptr;
     Ctl* h_ctl = nullptr;   // pinned mirror
+    SceneCtl* sctl = nullptr;
+    SceneCtl* h_sctl = nullptr;  // pinned mirror of scene 0
+    // scenes (a plain simulation is one scene with no per-slot arrays)
+    int n_scenes = 1;
+    int sps = MAX_SLOTS;          // max-reduction slots per scene
+    Dom* d_doms = nullptr;
+    int* d_scene_of = nullptr;
+    uint32_t* b_keys1 = nullptr;  // batched sort: key-sorted keys, their permutation, scene keys
+    uint32_t* b_perm1 = nullptr;
+    uint32_t* b_sk = nullptr;
+    uint32_t* b_sk2 = nullptr;
+    int* d_kstat = nullptr;       // per-step per-scene iteration counts and max densities
+    float* d_rstat = nullptr;
+    int stat_cap = 0;
+    int steps_done = 0;
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
     uint32_t ncells = 0;
@@ -1211,6 +1339,25 @@ struct upf_sim {
         A.dens_sq = P.rest_density * P.rest_density;
         A.n = n;
         A.stride = cap;
+        A.dom0.scales[0] = P.scales[0];
+        A.dom0.scales[1] = P.scales[1];
+        A.dom0.scales[2] = P.scales[2];
+        for (int a = 0; a < 3; ++a) {
+            A.dom0.cmin[a] = P.collision_min[a];
+            A.dom0.cmax[a] = P.collision_max[a];
+        }
+        A.dom0.block_shift = P.block_shift;
+        A.dom0.grid_dim = P.grid_dim;
+        A.dom0.cell_off = 0;
+        A.dom0.slot_off = 0;
+        A.dom0.n = n;
+        A.dom0.delta = P.density_error_factor;
+        A.doms = n_scenes > 1 ? d_doms : nullptr;
+        A.scene_of = n_scenes > 1 ? d_scene_of : nullptr;
+        A.sctl = sctl;
+        A.max_slots = W.max_slots;
+        A.n_scenes = n_scenes;
+        A.slots_per_scene = sps;
         return A;
     }
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -1221,6 +1368,8 @@ struct upf_sim {
 };
 
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+// grid of the persistent (grid-stride) kernels: enough waves to fill the chip
+static inline dim3 grid_persist(int n) { return dim3((unsigned)std::min((n + BLOCK - 1) / BLOCK, 8192)); }
 
 static int check_params(const upf_params* p) {
     if (!p) return UPF_ERR_ARG;
@@ -1251,7 +1400,17 @@ static int sort_and_bin(upf_sim* s) {
     const int n = s->n;
     Work& W = s->W;
     s->prof.begin(PC_SORT, s->st);
-    HIPCHK(radix_sort(W.keys, nullptr, W.keys_sorted, W.perm, n, 30, s->sort_tmp, s->st));
+    if (s->n_scenes == 1) {
+        HIPCHK(radix_sort(W.keys, nullptr, W.keys_sorted, W.perm, n, 30, s->sort_tmp, s->st));
+    } else {
+        // stable sort by (scene, key): keys first, then the scene as the most
+        // significant digit; scenes keep their slot ranges
+        HIPCHK(radix_sort(W.keys, nullptr, s->b_keys1, s->b_perm1, n, 30, s->sort_tmp, s->st));
+        hipLaunchKernelGGL(k_scene_keys, grid_for(n), dim3(BLOCK), 0, s->st, n, s->b_perm1, s->d_scene_of, s->b_sk);
+        const int sbits = s->n_scenes <= 256 ? 8 : 16;
+        HIPCHK(radix_sort(s->b_sk, s->b_perm1, s->b_sk2, W.perm, n, sbits, s->sort_tmp, s->st));
+        hipLaunchKernelGGL(k_gather_keys, grid_for(n), dim3(BLOCK), 0, s->st, n, W.perm, W.keys, W.keys_sorted);
+    }
     s->prof.end(s->st);
     const int nxt = 1 - s->cur;
     s->prof.begin(PC_GATHER, s->st);
@@ -1263,8 +1422,7 @@ static int sort_and_bin(upf_sim* s) {
     hipLaunchKernelGGL(k_fill_cells, dim3((s->ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, s->st, W.cells, s->ncells);
     s->prof.end(s->st);
     s->prof.begin(PC_CELLS, s->st);
-    hipLaunchKernelGGL(k_cell_bounds, grid_for(n), dim3(BLOCK), 0, s->st, n, W.keys_sorted, (int)(3 * s->P.block_shift),
-                       W.cells);
+    hipLaunchKernelGGL(k_cell_bounds, grid_for(n), dim3(BLOCK), 0, s->st, s->args(), W.keys_sorted, W.cells);
     s->prof.end(s->st);
     HIPCHK(hipGetLastError());
     return UPF_OK;
@@ -1272,6 +1430,7 @@ static int sort_and_bin(upf_sim* s) {
 
 static int read_ctl(upf_sim* s) {
     HIPCHK(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipMemcpyAsync(s->h_sctl, s->sctl, sizeof(SceneCtl), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     s->prof.collect();
     return UPF_OK;
@@ -1286,11 +1445,11 @@ static int err_code(uint32_t bits) {
 
 template <int MODE>
 static void enqueue_iteration(upf_sim* s, const StepArgs& A, int t) {
-    const dim3 g = grid_for(s->n);
+    const dim3 g = grid_persist(s->n);
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_DENSITY, s->st);
     hipLaunchKernelGGL(k_density<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    hipLaunchKernelGGL(k_loop_control, dim3(1), dim3(64), 0, s->st, t, s->P, s->W.max_slots, s->ctl);
+    hipLaunchKernelGGL(k_loop_control, dim3(1), dim3(256), 0, s->st, t, A, s->ctl);
     s->prof.end(s->st);
     s->prof.begin(PC_FORCE, s->st);
     hipLaunchKernelGGL(k_force<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
@@ -1309,55 +1468,77 @@ static int pcisph_loop(upf_sim* s, const StepArgs& A) {
     // trip about as much as three of them: speculate a few iterations ahead
     const int prev = s->k_hist.empty() ? lo : s->k_hist.back();
     const int ahead = std::min(std::min(std::max(lo, prev), lo + 4), maxl);
+    hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(256), 0, s->st, A, s->ctl);
     int t = 0;
     for (; t < ahead; ++t) enqueue_iteration<MODE>(s, A, t);
     for (;;) {
         int rc = read_ctl(s);
         if (rc) return rc;
-        const int want = s->h_ctl->want;  // final once want <= enqueued iterations
+        const int want = s->h_ctl->want_any;  // final once want <= enqueued iterations
         if (want <= t || (s->h_ctl->err & (1u << UPF_ERR_NAN))) break;
         const int stop = std::min(t + 4, maxl);
         for (; t < stop; ++t) enqueue_iteration<MODE>(s, A, t);
     }
-    s->k_hist.push_back(s->h_ctl->want);
-    s->rho_hist.push_back(s->h_ctl->last_max);
+    s->k_hist.push_back(s->h_ctl->want_any);
+    s->rho_hist.push_back(s->n_scenes == 1 ? s->h_sctl->last_max : 0.0f);
     return UPF_OK;
+}
+
+// The whole correction loop without a host decision: all max_loops
+// iterations are enqueued (for graph capture); those at or beyond every
+// scene's decision return at their first instruction.
+template <int MODE>
+static void pcisph_loop_device(upf_sim* s, const StepArgs& A) {
+    const int maxl = std::max(s->P.max_loops, 1);
+    hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(256), 0, s->st, A, s->ctl);
+    for (int t = 0; t < maxl; ++t) enqueue_iteration<MODE>(s, A, t);
 }
 
 static void build_lists(upf_sim* s, const StepArgs& A) {
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_NBR_BUILD, s->st);
-    hipLaunchKernelGGL(k_nbr_build, dim3((unsigned)((s->n + 63) / 64)), dim3(64), 0, s->st, s->n, S.pos,
-                       s->W.keys_sorted, s->W.cells, s->P, A.lut_factor, s->W.nbr, A.stride, s->W.ncount, s->ctl);
+    hipLaunchKernelGGL(k_nbr_build, dim3((unsigned)((s->n + 63) / 64)), dim3(64), 0, s->st, A, S.pos,
+                       s->W.keys_sorted, s->W.cells, s->W.nbr, s->W.ncount, s->ctl);
     s->prof.end(s->st);
 }
 
-static int step_pcisph_static(upf_sim* s) {
+// One PCISPH step with static boundaries (CpuParticlePhysicsStaticBoundariesPCISPH,
+// src/cpu/UnifiedPhysics.cpp:4220-4238).  device_loop: the loop decision stays
+// on the device and every iteration up to max_loops is enqueued (graph capture);
+// otherwise the host reads the decision back.
+static int step_pcisph_static(upf_sim* s, bool device_loop) {
     int rc = sort_and_bin(s);
     if (rc) return rc;
     const StepArgs A = s->args();
     Slots& S = s->S[s->cur];
     const dim3 g = grid_for(s->n);
+    const dim3 gp = grid_persist(s->n);
+    const bool akinci = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI;
     if (!s->psi_ready) {
         s->prof.begin(PC_PSI, s->st);
-        hipLaunchKernelGGL(k_psi, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.cells, s->P, s->luts, A.lut_factor, S.psi,
-                           s->ctl);
+        hipLaunchKernelGGL(k_psi, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, s->luts, S.psi, s->ctl);
         s->prof.end(s->st);
         s->psi_ready = true;
     }
     build_lists(s, A);
     s->prof.begin(PC_NBR_EXT, s->st);
-    if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI)
-        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+    if (akinci)
+        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, gp, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
     else
-        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, gp, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
-    rc = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? pcisph_loop<UPF_BOUNDARY_AKINCI>(s, A)
-                                                            : pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
-    if (rc) return rc;
+    if (device_loop) {
+        if (akinci) pcisph_loop_device<UPF_BOUNDARY_AKINCI>(s, A);
+        else pcisph_loop_device<UPF_BOUNDARY_CPU_REF>(s, A);
+    } else {
+        rc = akinci ? pcisph_loop<UPF_BOUNDARY_AKINCI>(s, A) : pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
+        if (rc) return rc;
+    }
     s->prof.begin(PC_INTEGRATE, s->st);
-    hipLaunchKernelGGL(k_integrate, g, dim3(BLOCK), 0, s->st, A, S, s->W, 1, 0, s->rigid ? 1 : 0, s->ctl);
+    hipLaunchKernelGGL(k_integrate, gp, dim3(BLOCK), 0, s->st, A, S, s->W, 1, 0, s->rigid ? 1 : 0, s->ctl);
     s->prof.end(s->st);
+    if (s->d_kstat)
+        hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, s->st, A, s->ctl, s->d_kstat, s->d_rstat, s->stat_cap);
     HIPCHK(hipGetLastError());
     return UPF_OK;
 }
@@ -1371,14 +1552,13 @@ static int step_pcisph_rigid(upf_sim* s) {
     const dim3 g = grid_for(s->n);
     build_lists(s, A);  // liquid owners
     s->prof.begin(PC_NBR_BUILD, s->st);
-    hipLaunchKernelGGL(k_nbr_build_box, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.cells, s->P, A.lut_factor,
-                       (int)UPF_RIGID, s->W.nbr, A.stride, s->W.ncount, s->ctl);
+    hipLaunchKernelGGL(k_nbr_build_box, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, (int)UPF_RIGID, s->W.nbr,
+                       s->W.ncount, s->ctl);
     s->prof.end(s->st);
     if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI) {
         s->prof.begin(PC_PSI, s->st);
         if (!s->psi_ready) {  // walls: once (they do not move)
-            hipLaunchKernelGGL(k_psi, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.cells, s->P, s->luts, A.lut_factor,
-                               S.psi, s->ctl);
+            hipLaunchKernelGGL(k_psi, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, s->luts, S.psi, s->ctl);
             s->psi_ready = true;
         }
         hipLaunchKernelGGL(k_psi_rigid, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.nbr, A.stride, s->W.ncount, s->P,
@@ -1395,7 +1575,7 @@ static int step_pcisph_rigid(upf_sim* s) {
                                                             : pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
     if (rc) return rc;
     s->prof.begin(PC_INTEGRATE, s->st);
-    hipLaunchKernelGGL(k_integrate, g, dim3(BLOCK), 0, s->st, A, S, s->W, 1, 1, 1, s->ctl);
+    hipLaunchKernelGGL(k_integrate, grid_persist(s->n), dim3(BLOCK), 0, s->st, A, S, s->W, 1, 1, 1, s->ctl);
     s->prof.end(s->st);
     // rigid bodies: members grouped by body in slot order, one thread per body
     s->prof.begin(PC_RIGID, s->st);
@@ -1425,7 +1605,8 @@ static int step_wcsph_static(upf_sim* s) {
     hipLaunchKernelGGL(k_force_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
     s->prof.begin(PC_INTEGRATE, s->st);
-    hipLaunchKernelGGL(k_integrate, g, dim3(BLOCK), 0, s->st, A, S, s->W, 0, 0, s->rigid ? 1 : 0, s->ctl);
+    hipLaunchKernelGGL(k_integrate, grid_persist(s->n), dim3(BLOCK), 0, s->st, A, S, s->W, 0, 0, s->rigid ? 1 : 0,
+                       s->ctl);
     s->prof.end(s->st);
     HIPCHK(hipGetLastError());
     rc = read_ctl(s);
@@ -1439,6 +1620,66 @@ extern "C" {
 
 const char* upf_version(void) { return "upf-mi355x 0.1 (gfx950)"; }
 
+// Device buffers of a simulation of `capacity` slots in `n_scenes` scenes
+// whose cell tables hold `ncells` entries together.
+static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
+    const upf_params* p = &s->P;
+    s->cap = capacity;
+    s->n_scenes = n_scenes;
+    s->sps = n_scenes == 1 ? MAX_SLOTS : 16;
+    s->ncells = ncells;
+    if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) return UPF_ERR_HIP;
+    if (alloc_slots(s, s->S[0], capacity) || alloc_slots(s, s->S[1], capacity)) return UPF_ERR_HIP;
+    Work& W = s->W;
+    const size_t nslots = (size_t)n_scenes * s->sps * 32;
+    HIPCHK(s->alloc(&W.keys, capacity));
+    HIPCHK(s->alloc(&W.keys_sorted, capacity));
+    HIPCHK(s->alloc(&W.perm, capacity));
+    HIPCHK(s->alloc(&W.force, capacity));
+    HIPCHK(s->alloc(&W.fp, capacity));
+    HIPCHK(s->alloc(&W.xs, capacity));
+    HIPCHK(s->alloc(&W.p, capacity));
+    HIPCHK(s->alloc(&W.rho, capacity));
+    HIPCHK(s->alloc(&W.pres, capacity));
+    HIPCHK(s->alloc(&W.pdiv, capacity));
+    HIPCHK(s->alloc(&W.max_slots, nslots));
+    HIPCHK(s->alloc(&W.ncount, capacity));
+    HIPCHK(s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors));
+    HIPCHK(s->alloc(&W.cells, ncells));
+    HIPCHK(s->alloc(&s->luts, 5 * LUT_N));
+    HIPCHK(s->alloc(&s->ctl, 1));
+    HIPCHK(s->alloc(&s->sctl, n_scenes));
+    void* tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, radix_sort_temp_bytes(capacity)));
+    s->sort_tmp = tmp;
+    s->allocs.push_back(tmp);
+    HIPCHK(hipHostMalloc((void**)&s->h_ctl, sizeof(Ctl), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&s->h_sctl, sizeof(SceneCtl), hipHostMallocDefault));
+    std::vector<float> L(5 * LUT_N);
+    build_luts(p->support_radius, L.data());
+    HIPCHK(hipMemcpy(s->luts, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(s->ctl, 0, sizeof(Ctl)));
+    HIPCHK(hipMemset(s->sctl, 0, (size_t)n_scenes * sizeof(SceneCtl)));
+    HIPCHK(hipMemset(W.max_slots, 0, nslots * sizeof(uint32_t)));
+    if (n_scenes > 1) {
+        HIPCHK(s->alloc(&s->d_doms, n_scenes));
+        HIPCHK(s->alloc(&s->d_scene_of, capacity));
+        HIPCHK(s->alloc(&s->b_keys1, capacity));
+        HIPCHK(s->alloc(&s->b_perm1, capacity));
+        HIPCHK(s->alloc(&s->b_sk, capacity));
+        HIPCHK(s->alloc(&s->b_sk2, capacity));
+    }
+    return UPF_OK;
+}
+
+// per-step statistics on the device (batched form and graph replay)
+static int stats_alloc(upf_sim* s, int cap_steps) {
+    s->stat_cap = cap_steps;
+    HIPCHK(s->alloc(&s->d_kstat, (size_t)cap_steps * s->n_scenes));
+    HIPCHK(s->alloc(&s->d_rstat, (size_t)cap_steps * s->n_scenes));
+    return UPF_OK;
+}
+
 int upf_create(const upf_params* p, int capacity, upf_sim** out) {
     // list entries hold the neighbour slot in 23 bits
     if (!out || capacity <= 0 || capacity >= (1 << (32 - NB_SLOT_SHIFT))) return UPF_ERR_ARG;
@@ -1448,35 +1689,7 @@ int upf_create(const upf_params* p, int capacity, upf_sim** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPF_ERR_NO_DEVICE;
     upf_sim* s = new upf_sim();
     s->P = *p;
-    s->cap = capacity;
-    rc = UPF_ERR_HIP;
-    do {
-        if (hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking) != hipSuccess) break;
-        if (alloc_slots(s, s->S[0], capacity) || alloc_slots(s, s->S[1], capacity)) break;
-        Work& W = s->W;
-        if (s->alloc(&W.keys, capacity) || s->alloc(&W.keys_sorted, capacity) || s->alloc(&W.perm, capacity) ||
-            s->alloc(&W.force, capacity) || s->alloc(&W.fp, capacity) || s->alloc(&W.xs, capacity) ||
-            s->alloc(&W.p, capacity) || s->alloc(&W.rho, capacity) || s->alloc(&W.pres, capacity) ||
-            s->alloc(&W.pdiv, capacity) || s->alloc(&W.max_slots, MAX_SLOTS * 32) ||
-            s->alloc(&W.ncount, capacity))
-            break;
-        if (s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors)) break;
-        s->ncells = p->grid_dim * p->grid_dim * p->grid_dim;
-        if (s->alloc(&W.cells, s->ncells)) break;
-        if (s->alloc(&s->luts, 5 * LUT_N)) break;
-        if (s->alloc(&s->ctl, 1)) break;
-        void* tmp = nullptr;
-        if (hipMalloc(&tmp, radix_sort_temp_bytes(capacity)) != hipSuccess) break;
-        s->sort_tmp = tmp;
-        s->allocs.push_back(tmp);
-        if (hipHostMalloc((void**)&s->h_ctl, sizeof(Ctl), hipHostMallocDefault) != hipSuccess) break;
-        std::vector<float> L(5 * LUT_N);
-        build_luts(p->support_radius, L.data());
-        if (hipMemcpy(s->luts, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) break;
-        if (hipMemset(s->ctl, 0, sizeof(Ctl)) != hipSuccess) break;
-        if (hipMemset(W.max_slots, 0, MAX_SLOTS * 32 * sizeof(uint32_t)) != hipSuccess) break;
-        rc = UPF_OK;
-    } while (0);
+    rc = sim_alloc(s, capacity, p->grid_dim * p->grid_dim * p->grid_dim, 1);
     if (rc != UPF_OK) {
         upf_destroy(s);
         return rc;
@@ -1492,6 +1705,7 @@ int upf_destroy(upf_sim* s) {
     if (s->d_bodies) (void)hipFree(s->d_bodies);
     if (s->d_body_start) (void)hipFree(s->d_body_start);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    if (s->h_sctl) (void)hipHostFree(s->h_sctl);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
     return UPF_OK;
@@ -1510,12 +1724,18 @@ int upf_set_params(upf_sim* s, const upf_params* p) {
     return UPF_OK;
 }
 
-int upf_upload_particles(upf_sim* s, int n, const float* pos4, const float* vel4, const int* type) {
+}  // extern "C"
+
+// Uploads n particles; scene_of / scene_params give every particle's scene and
+// the scenes' domains (null: one scene with the simulation's params).
+static int upload_common(upf_sim* s, int n, const float* pos4, const float* vel4, const int* type,
+                         const int* scene_of, const upf_params* scene_params) {
     if (!s || n <= 0 || n > s->cap || !pos4 || !type) return UPF_ERR_ARG;
     std::vector<float> pos((size_t)4 * n), vel((size_t)4 * n, 0.0f), x0((size_t)4 * n);
     std::vector<int> id(n), body(n, -1);
     std::vector<float> rho(n, s->P.rest_density), zero(n, 0.0f), psi(n, s->P.initial_mass / s->P.rest_density);
     for (int i = 0; i < n; ++i) {
+        const upf_params& Q = scene_of ? scene_params[scene_of[i]] : s->P;
         const int t = type[i];
         if (t != UPF_LIQUID && t != UPF_FROZEN && t != UPF_RIGID) return UPF_ERR_ARG;
         for (int a = 0; a < 3; ++a) {
@@ -1531,11 +1751,11 @@ int upf_upload_particles(upf_sim* s, int n, const float* pos4, const float* vel4
         vel[4 * i + 3] = 0.0f;
         id[i] = i;
         uint32_t k;
-        if (!host_key(s->P, pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2], k)) return UPF_ERR_RANGE;
+        if (!host_key(Q, pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2], k)) return UPF_ERR_RANGE;
         // the query box of every particle must stay inside the grid (beyond it the
         // reference's cell ranges overflow, src/cpu/zIndex.cpp:66)
         for (int a = 0; a < 3; ++a)
-            if (!(s->P.scales[a] * (pos4[4 * i + a] + s->P.support_radius) < 1024.0f)) return UPF_ERR_RANGE;
+            if (!(Q.scales[a] * (pos4[4 * i + a] + Q.support_radius) < 1024.0f)) return UPF_ERR_RANGE;
     }
     Slots& S = s->S[0];
     s->cur = 0;
@@ -1555,14 +1775,23 @@ int upf_upload_particles(upf_sim* s, int n, const float* pos4, const float* vel4
     HIPCHK(hipMemcpyAsync(s->W.p, zero.data(), b1, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipMemcpyAsync(s->W.rho, rho.data(), b1, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipMemsetAsync(s->ctl, 0, sizeof(Ctl), s->st));
-    hipLaunchKernelGGL(k_keys, grid_for(n), dim3(BLOCK), 0, s->st, n, S.pos, s->W.keys, s->P, s->ctl);
+    if (scene_of) HIPCHK(hipMemcpyAsync(s->d_scene_of, scene_of, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s->st));
+    hipLaunchKernelGGL(k_keys, grid_for(n), dim3(BLOCK), 0, s->st, s->args(), S.pos, s->W.keys, s->ctl);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->st));
     s->uploaded = true;
     s->psi_ready = false;
     s->k_hist.clear();
     s->rho_hist.clear();
+    s->steps_done = 0;
     return UPF_OK;
+}
+
+extern "C" {
+
+int upf_upload_particles(upf_sim* s, int n, const float* pos4, const float* vel4, const int* type) {
+    if (s && s->n_scenes != 1) return UPF_ERR_STATE;
+    return upload_common(s, n, pos4, vel4, type, nullptr, nullptr);
 }
 
 
@@ -1572,7 +1801,7 @@ int upf_step(upf_sim* s, int nsteps, int method) {
     if (method == UPF_PCISPH_RIGID && !s->rigid) return UPF_ERR_STATE;
     for (int k = 0; k < nsteps; ++k) {
         int rc;
-        if (method == UPF_PCISPH_STATIC) rc = step_pcisph_static(s);
+        if (method == UPF_PCISPH_STATIC) rc = step_pcisph_static(s, false);
         else if (method == UPF_WCSPH_STATIC) rc = step_wcsph_static(s);
         else if (method == UPF_PCISPH_RIGID) rc = step_pcisph_rigid(s);
         else return UPF_ERR_ARG;
@@ -1762,7 +1991,10 @@ int upf_keys(const upf_params* p, int n, const float* pos4, uint32_t* keys) {
         if (hipMalloc((void**)&ctl, sizeof(Ctl)) != hipSuccess) break;
         if (hipMemset(ctl, 0, sizeof(Ctl)) != hipSuccess) break;
         if (hipMemcpy(d_pos, pos4, (size_t)n * 16, hipMemcpyHostToDevice) != hipSuccess) break;
-        hipLaunchKernelGGL(k_keys, grid_for(n), dim3(BLOCK), 0, 0, n, d_pos, d_keys, *p, ctl);
+        upf_sim tmp;
+        tmp.P = *p;
+        tmp.n = n;
+        hipLaunchKernelGGL(k_keys, grid_for(n), dim3(BLOCK), 0, 0, tmp.args(), d_pos, d_keys, ctl);
         if (hipGetLastError() != hipSuccess) break;
         if (hipMemcpy(keys, d_keys, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) break;
         Ctl h;
@@ -1796,6 +2028,185 @@ int upf_radix_sort(int n, const uint32_t* keys_in, uint32_t* keys_out, int32_t* 
     (void)hipFree(dv);
     (void)hipFree(tmp);
     return rc;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched form
+struct upf_batch {
+    upf_sim* s = nullptr;
+    std::vector<upf_params> sp;  // per-scene params (domains)
+    std::vector<int> counts;
+    std::vector<int> scene_of;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};  // one step, by the slot buffer it starts from
+};
+
+// Captures one static-boundary PCISPH step starting from slot buffer `cur`
+// (the whole correction loop with the decision on the device).
+static int capture_step(upf_sim* s, int cur, hipGraphExec_t* out) {
+    const bool prof = s->prof.on;
+    s->prof.on = false;
+    s->cur = cur;
+    hipGraph_t g = nullptr;
+    int rc = UPF_OK;
+    if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = UPF_ERR_HIP;
+    if (!rc) {
+        rc = step_pcisph_static(s, true);
+        const hipError_t e = hipStreamEndCapture(s->st, &g);
+        if (!rc && e != hipSuccess) rc = UPF_ERR_HIP;
+    }
+    s->prof.on = prof;
+    if (!rc && hipGraphInstantiate(out, g, nullptr, nullptr, 0) != hipSuccess) rc = UPF_ERR_HIP;
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+static bool same_physics(const upf_params& a, const upf_params& b) {
+    upf_params x = b;
+    for (int k = 0; k < 3; ++k) {
+        x.vbox_min[k] = a.vbox_min[k];
+        x.vbox_max[k] = a.vbox_max[k];
+        x.scales[k] = a.scales[k];
+        x.collision_min[k] = a.collision_min[k];
+        x.collision_max[k] = a.collision_max[k];
+    }
+    x.block_size = a.block_size;
+    x.block_shift = a.block_shift;
+    x.grid_dim = a.grid_dim;
+    x.density_error_factor = a.density_error_factor;
+    return std::memcmp(&x, &a, sizeof(upf_params)) == 0;
+}
+
+extern "C" {
+
+int upf_batch_create(int n_scenes, const upf_params* scene_params, const int* counts, int max_record_steps,
+                     upf_batch** out) {
+    if (!out || n_scenes < 1 || n_scenes > 65535 || !scene_params || !counts || max_record_steps < 0)
+        return UPF_ERR_ARG;
+    long long total = 0;
+    uint64_t ncells = 0;
+    for (int k = 0; k < n_scenes; ++k) {
+        if (counts[k] <= 0) return UPF_ERR_ARG;
+        int rc = check_params(&scene_params[k]);
+        if (rc) return rc;
+        if (!same_physics(scene_params[0], scene_params[k])) return UPF_ERR_ARG;
+        total += counts[k];
+        const uint64_t g = scene_params[k].grid_dim;
+        ncells += g * g * g;
+    }
+    if (total >= (1 << (32 - NB_SLOT_SHIFT)) || ncells >= (1ull << 32)) return UPF_ERR_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPF_ERR_NO_DEVICE;
+    upf_batch* b = new upf_batch();
+    b->sp.assign(scene_params, scene_params + n_scenes);
+    b->counts.assign(counts, counts + n_scenes);
+    b->s = new upf_sim();
+    upf_sim* s = b->s;
+    s->P = scene_params[0];
+    int rc = sim_alloc(s, (int)total, (uint32_t)ncells, n_scenes);
+    if (!rc) rc = stats_alloc(s, std::max(max_record_steps, 1));
+    if (!rc) {
+        std::vector<Dom> doms(n_scenes);
+        uint32_t coff = 0;
+        int soff = 0;
+        for (int k = 0; k < n_scenes; ++k) {
+            const upf_params& Q = scene_params[k];
+            Dom& d = doms[k];
+            for (int a = 0; a < 3; ++a) {
+                d.scales[a] = Q.scales[a];
+                d.cmin[a] = Q.collision_min[a];
+                d.cmax[a] = Q.collision_max[a];
+            }
+            d.block_shift = Q.block_shift;
+            d.grid_dim = Q.grid_dim;
+            d.cell_off = coff;
+            d.slot_off = soff;
+            d.n = counts[k];
+            d.delta = Q.density_error_factor;
+            coff += Q.grid_dim * Q.grid_dim * Q.grid_dim;
+            b->scene_of.insert(b->scene_of.end(), (size_t)counts[k], k);
+            soff += counts[k];
+        }
+        if (hipMemcpy(s->d_doms, doms.data(), doms.size() * sizeof(Dom), hipMemcpyHostToDevice) != hipSuccess)
+            rc = UPF_ERR_HIP;
+    }
+    if (rc) {
+        upf_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return UPF_OK;
+}
+
+int upf_batch_destroy(upf_batch* b) {
+    if (!b) return UPF_ERR_ARG;
+    for (auto& g : b->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (b->s) upf_destroy(b->s);
+    delete b;
+    return UPF_OK;
+}
+
+int upf_batch_upload(upf_batch* b, const float* pos4, const float* vel4, const int* type) {
+    if (!b) return UPF_ERR_ARG;
+    return upload_common(b->s, (int)b->scene_of.size(), pos4, vel4, type, b->scene_of.data(), b->sp.data());
+}
+
+int upf_batch_step(upf_batch* b, int nsteps, int use_graph) {
+    if (!b || nsteps < 0) return UPF_ERR_ARG;
+    upf_sim* s = b->s;
+    if (!s->uploaded) return UPF_ERR_STATE;
+    int k = 0;
+    // the first step also computes the boundary volumes: eager
+    for (; k < nsteps && (!use_graph || !s->psi_ready); ++k) {
+        int rc = step_pcisph_static(s, false);
+        if (rc) return rc;
+        ++s->steps_done;
+        const int e = err_code(s->h_ctl->err);
+        if (e) return e;
+    }
+    if (k < nsteps && !b->gexec[0]) {
+        const int cur = s->cur;
+        int rc = capture_step(s, 0, &b->gexec[0]);
+        if (!rc) rc = capture_step(s, 1, &b->gexec[1]);
+        s->cur = cur;
+        if (rc) return rc;
+    }
+    for (; k < nsteps; ++k) {
+        HIPCHK(hipGraphLaunch(b->gexec[s->cur], s->st));
+        s->cur = 1 - s->cur;
+        ++s->steps_done;
+    }
+    int rc = read_ctl(s);
+    if (rc) return rc;
+    return err_code(s->h_ctl->err);
+}
+
+int upf_batch_download(upf_batch* b, float* pos4, float* vel4, float* rho) {
+    if (!b) return UPF_ERR_ARG;
+    return upf_download(b->s, pos4, vel4, rho, nullptr, nullptr, nullptr, nullptr);
+}
+
+// iterations / max_rho: [step][scene], up to max_steps steps; returns the
+// number of steps recorded
+int upf_batch_get_stats(upf_batch* b, int max_steps, int* iterations, float* max_rho) {
+    if (!b || max_steps < 0) return -UPF_ERR_ARG;
+    upf_sim* s = b->s;
+    if (read_ctl(s)) return -UPF_ERR_HIP;
+    const int m = std::min((int)s->h_ctl->step, s->stat_cap);
+    const int c = std::min(m, max_steps);
+    const size_t cnt = (size_t)c * s->n_scenes;
+    if (iterations && cnt && hipMemcpy(iterations, s->d_kstat, cnt * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return -UPF_ERR_HIP;
+    if (max_rho && cnt && hipMemcpy(max_rho, s->d_rstat, cnt * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return -UPF_ERR_HIP;
+    return m;
+}
+
+int upf_batch_sim(upf_batch* b, upf_sim** sim) {
+    if (!b || !sim) return UPF_ERR_ARG;
+    *sim = b->s;
+    return UPF_OK;
 }
 
 // Host-side mirror of the query-box enumeration, for tests: the cells (Morton
