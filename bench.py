@@ -87,7 +87,85 @@ def run_gpu(args):
         if cnt.value:
             prof[L.upf_profile_name(c).decode()] = (ms.value, cnt.value)
     sim.close()
-    return dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof)
+    out = dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof)
+    if not args.no_graph:
+        # the same scene as a batch of one: every step one hipGraph replay
+        b = upf.Batch([p], [n], max_record_steps=args.warmup + args.steps + 8)
+        b.upload(pos, typ)
+        b.step(args.warmup, use_graph=True)
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        b.step(args.steps, use_graph=True)
+        hip.hipDeviceSynchronize()
+        out["graph_seconds"] = time.perf_counter() - t0
+        Kg, _ = b.stats()
+        out["graph_K"] = Kg[-args.steps:, 0]
+        b.close()
+    return out
+
+
+def copy_calibration(nbytes=1 << 28):
+    """A device-to-device copy of known size (the runtime's copy kernel), so
+    that hardware byte counters can be scaled: FETCH_SIZE / WRITE_SIZE of this
+    dispatch should read nbytes each."""
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def run_batch(args):
+    """256 uneven dam breaks (fluid counts log-uniform in [2048, 65536], block
+    aspect ratios in [0.5, 2], own box and walls each): the batched form
+    (one launch per stage for all scenes, each step one hipGraph replay)
+    against the plain path (one simulation per scene, upf_step per scene),
+    same scenes, same number of steps."""
+    import unifiedparticleframeworkcuda_amd as upf
+    hip = C.CDLL("libamdhip64.so")
+    scenes = upf.batch_dam_breaks(args.batch_scenes, seed=0)
+    nf = sum(sc[3] for sc in scenes)
+    n = sum(len(sc[2]) for sc in scenes)
+    K = args.batch_steps
+    out = dict(scenes=len(scenes), fluid=nf, particles=n, steps=K)
+    b = upf.Batch.from_scenes(scenes, max_record_steps=2 * K + 8)
+    b.step(2, use_graph=True)  # eager first step (boundary volumes), graph capture + one replay
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    b.step(K, use_graph=True)
+    hip.hipDeviceSynchronize()
+    tg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.step(K, use_graph=False)
+    hip.hipDeviceSynchronize()
+    th = time.perf_counter() - t0
+    Ks, rs = b.stats()
+    b.close()
+    sims = []
+    for p, pos, typ, _ in scenes:
+        s = upf.Simulation(p, len(typ))
+        s.upload(pos, typ)
+        s.step(2, upf.UPF_PCISPH_STATIC)
+        sims.append(s)
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.step(K, upf.UPF_PCISPH_STATIC)
+    hip.hipDeviceSynchronize()
+    tp = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    out.update(graph_updates_per_s=nf * K / tg, host_loop_updates_per_s=nf * K / th, plain_updates_per_s=nf * K / tp,
+               graph_ms_per_step=tg / K * 1e3, plain_ms_per_step=tp / K * 1e3,
+               speedup_graph_vs_plain=tp / tg, iterations_mean=float(np.mean(Ks[2:2 + K])),
+               iterations_max=int(np.max(Ks[2:2 + K])),
+               max_density_error_pct=float(max(0.0, 0.1 * float(np.max(rs[2:2 + K])) - 100.0)))
+    return out
 
 
 def cpu_baseline(args, budget_s):
@@ -121,11 +199,19 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-max-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a 256 MiB device copy (scale for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one scene on one MI355X "
               "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
     g = run_gpu(args)
     n, nf, ncells = g["n"], g["nf"], g["ncells"]
     sec = g["seconds"]
@@ -161,8 +247,14 @@ def main():
         "step_roofline": {"achieved": total_bytes / sec / 1e9, "unit": "GB/s", "frac": total_bytes / sec / HBM_PEAK,
                           "bytes_per_update": total_bytes / (nf * args.steps)},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in g["prof"].items()},
+        "graph_replay": (None if "graph_seconds" not in g else
+                         {"ms_per_step": g["graph_seconds"] / args.steps * 1e3,
+                          "updates_per_s": nf * args.steps / g["graph_seconds"],
+                          "same_iterations": bool(np.array_equal(g["graph_K"], g["K"]))}),
         "roofline": roof,
     }
+    if not args.no_batch:
+        out["batched"] = run_batch(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, args.cpu_budget)
     print(json.dumps(out))

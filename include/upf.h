@@ -22,6 +22,7 @@
 #ifndef UPF_H_
 #define UPF_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -302,6 +303,31 @@ int upf_batch_download(upf_batch* b, float* pos4, float* vel4, float* rho);
 int upf_batch_get_stats(upf_batch* b, int max_steps, int* iterations, float* max_rho);
 /* The batch's simulation handle, for the profiling and download calls.    */
 int upf_batch_sim(upf_batch* b, upf_sim** sim);
+
+/* ------------------------------------------------- differentiable form */
+/* A rollout of `steps` static-boundary PCISPH steps with a fixed number of
+ * correction iterations and its adjoint (no reference counterpart).  The
+ * step is CpuParticlePhysicsStaticBoundariesPCISPH (src/cpu/UnifiedPhysics.cpp:4220)
+ * in the CPU boundary mode with the kernels in closed form (KernelM4,
+ * kernelPressureGrad, kernelViscosityLaplacian, src/cpu/WeightingKernel.cpp:75,165,190)
+ * instead of their 300-entry tables, so that it has a derivative.  Particles
+ * keep their upload order (types: liquid and frozen).  The forward pass keeps
+ * the state every `checkpoint_every` steps; the backward pass recomputes one
+ * segment at a time and runs it in reverse.                                */
+typedef struct upf_adjoint upf_adjoint;
+int upf_adjoint_create(const upf_params* p, int n, int steps, int iterations,
+                       int checkpoint_every, upf_adjoint** out);
+int upf_adjoint_destroy(upf_adjoint* a);
+/* x_final, v_final from x0, v0 (vel4 may be NULL: at rest). */
+int upf_adjoint_forward(upf_adjoint* a, const float* pos4, const float* vel4, const int* type,
+                        float* pos_out4, float* vel_out4);
+/* dL/dx0, dL/dv0 from dL/dx_final (and dL/dv_final, may be NULL) of the
+ * last forward call; outputs may be NULL.                                 */
+int upf_adjoint_backward(upf_adjoint* a, const float* dl_dpos4, const float* dl_dvel4,
+                         float* dl_dpos0_4, float* dl_dvel0_4);
+/* Device time of the last forward / backward call (HIP events on the
+ * rollout's stream) and the device memory the rollout holds.              */
+int upf_adjoint_timing(upf_adjoint* a, float* forward_ms, float* backward_ms, size_t* device_bytes);
 
 /* ------------------------------------------------------ profiling */
 /* Per-kernel-class device time measured with HIP events recorded on the

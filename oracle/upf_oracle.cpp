@@ -1185,3 +1185,389 @@ extern "C" int orc_rigid_step(float* st, int n, const float* ppos, const float* 
     for (int a = 0; a < 9; ++a) { st[19 + a] = b.R[a]; st[37 + a] = b.Iinv_world[a]; }
     return 0;
 }
+
+// ===================================================================
+// Differentiable PCISPH (no reference counterpart): the oracle of the HIP
+// adjoint.  The step is the static-boundary PCISPH step of the CPU path
+// (CpuParticlePhysicsStaticBoundariesPCISPH, UnifiedPhysics.cpp:4220-4238, in
+// its CPU boundary mode) with a fixed number of correction iterations, the
+// kernels evaluated in closed form -- the functions the reference tabulates
+// (KernelM4 WeightingKernel.cpp:75-97, kernelPressureGrad :165-180,
+// kernelViscosityLaplacian :190-200) -- instead of through the 300-entry
+// tables, whose step values have no useful derivative.  Particles keep their
+// upload order; the z-index sort only orders the neighbour lists.
+//
+// Forward, per step (fluid particles; walls fixed):
+//   F_i   = -sum_j (v_i - v_j) V0^2 mu L(r_ij) - m g e_y          (fluid j)
+//   iteration k = 0..K-1 (Fp = 0 before the first):
+//     v*_i = v_i + (F_i + Fp_i) dt/m;  z_i = x_i + v*_i dt;  x*_i = clamp(z_i)
+//     rho_i = m W(0) + sum_j [r*_ij < h] m W(r*_ij)               (all j)
+//     p_i  += delta max(rho_i - rho0, 0)
+//     Fp_i  = -(m^2/rho0^2) sum_j (p_i + p_j) G(r_ij) (x_i - x_j)  (p_wall = 0)
+//   v'_i = v_i + (F_i + Fp_i) dt/m;  y_i = x_i + v'_i dt;  x'_i = clamp(y_i),
+//   v'_i,a *= -0.5 on every clamped axis.
+// Backward: the exact reverse of the above (neighbour sets held fixed), in
+// gather form over the symmetric lists.
+namespace orc {
+
+template <class T> struct DKern {  // closed-form kernels and their radial derivatives
+    T h, a, cs, cv;
+    explicit DKern(T sr) : h(sr) {
+        a = T(2.546479089470325472) / (sr * sr * sr);                     // 8 / (pi h^3)
+        cs = T(-45) / (T(3.1415926535897932384626) * sr * sr * sr * sr * sr * sr);
+        cv = T(45) / (T(3.1415926535897932384626) * sr * sr * sr * sr * sr * sr);
+    }
+    T W(T r) const {
+        if (!(r < h)) return T(0);
+        const T s = r / h;
+        if (s < T(0.5)) return a * (T(1) - T(6) * s * s + T(6) * s * s * s);
+        const T t = T(1) - s;
+        return a * (T(2) * t * t * t);
+    }
+    T dW(T r) const {  // dW/dr
+        if (!(r < h)) return T(0);
+        const T s = r / h;
+        if (s < T(0.5)) return a / h * (T(-12) * s + T(18) * s * s);
+        const T t = T(1) - s;
+        return a / h * (T(-6) * t * t);
+    }
+    T G(T r) const {  // spiky gradient: grad W(d) = G(|d|) d
+        if (!(r < h) || r == T(0)) return T(0);
+        const T t = h - r;
+        return cs * t * t / r;
+    }
+    T dG(T r) const {
+        if (!(r < h) || r == T(0)) return T(0);
+        const T t = h - r;
+        return cs * (T(-2) * t / r - t * t / (r * r));
+    }
+    T L(T r) const { return r < h ? cv * (h - r) : T(0); }  // viscosity Laplacian
+    T dL(T r) const { return r < h ? -cv : T(0); }
+};
+
+template <class T> struct DiffSim {
+    upf_params P;
+    int n = 0, K = 5;
+    std::vector<V3<T>> x, v;
+    std::vector<int> type;
+    // tape of the last forward (per step)
+    struct StepTape {
+        std::vector<V3<T>> x, v, y;
+        std::vector<std::vector<int>> nb;
+        std::vector<std::vector<V3<T>>> z;   // unclamped predicted positions per iteration
+        std::vector<std::vector<T>> p, rho;  // pressure / density per iteration
+    };
+    std::vector<StepTape> tape;
+    int err = UPF_OK;
+
+    bool fluid(int i) const { return type[i] == UPF_LIQUID; }
+
+    // neighbour lists of the current positions, ascending in (key, id) order
+    void lists(std::vector<std::vector<int>>& nb) {
+        std::vector<uint32_t> key(n);
+        for (int i = 0; i < n; ++i) {
+            unsigned c[3];
+            bool ok = true;
+            for (int a = 0; a < 3; ++a) ok &= to_grid<float>(P.scales[a] * (float)x[i][a], c[a]);
+            if (!ok) err = UPF_ERR_RANGE;
+            key[i] = ztab().calc(c[0], c[1], c[2]);
+        }
+        std::vector<int> ord(n);
+        for (int i = 0; i < n; ++i) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int p, int q) { return key[p] < key[q]; });
+        std::vector<uint32_t> skey(n);
+        for (int s = 0; s < n; ++s) skey[s] = key[ord[s]];
+        nb.assign(n, std::vector<int>());
+        const T h = (T)P.support_radius;
+        const float hf = P.support_radius;
+#pragma omp parallel
+        {
+            std::vector<std::pair<uint32_t, uint32_t>> ranges;
+#pragma omp for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) {
+                if (!fluid(i)) continue;
+                unsigned mn[3], mx[3];
+                for (int a = 0; a < 3; ++a) {
+                    float lo = P.scales[a] * ((float)x[i][a] - hf);
+                    float hi = P.scales[a] * ((float)x[i][a] + hf);
+                    lo = MAXr<float>(lo, 0.0f);
+                    hi = MINr<float>(hi, 1024.0f);
+                    mn[a] = (unsigned)(unsigned short)lo;
+                    mx[a] = (unsigned)(unsigned short)hi;
+                }
+                box_query(P.block_size, mn, mx, ranges);
+                for (size_t k = 0; k < ranges.size(); ++k) {
+                    int lo = (int)(std::lower_bound(skey.begin(), skey.end(), ranges[k].first) - skey.begin());
+                    for (int s = lo; s < n && skey[s] <= ranges[k].second; ++s) {
+                        const int j = ord[s];
+                        if (j == i) continue;
+                        const V3<T> d = x[i] - x[j];
+                        if (d.length_sq() < h * h) nb[i].push_back(j);
+                    }
+                }
+                if ((int)nb[i].size() > P.max_neighbors) err = UPF_ERR_NBR_OVERFLOW;
+            }
+        }
+    }
+
+    void clamp_axis(T& c, int a) const {
+        if (c < (T)P.collision_min[a]) c = (T)P.collision_min[a];
+        if (c > (T)P.collision_max[a]) c = (T)P.collision_max[a];
+    }
+
+    void step(StepTape* tp) {
+        const DKern<T> kn((T)P.support_radius);
+        const T m = (T)P.initial_mass, rho0 = (T)P.rest_density, dt = (T)P.dt, q = dt / m;
+        const T cvis = (m / rho0) * (m / rho0) * (T)P.visc_const;
+        const T cp = (m * m) / (rho0 * rho0);
+        const T delta = (T)P.density_error_factor;
+        std::vector<std::vector<int>> nbl;
+        lists(nbl);
+        std::vector<V3<T>> F(n), Fp(n), xs(x), z(x);
+        std::vector<T> p(n, T(0)), rho(n, T(0));
+        if (tp) {
+            tp->x = x;
+            tp->v = v;
+            tp->z.assign(K, std::vector<V3<T>>());
+            tp->p.assign(K, std::vector<T>());
+            tp->rho.assign(K, std::vector<T>());
+        }
+#pragma omp parallel for schedule(dynamic, 100)
+        for (int i = 0; i < n; ++i) {
+            if (!fluid(i)) continue;
+            V3<T> f;
+            for (int j : nbl[i]) {
+                if (!fluid(j)) continue;
+                const T r = vsqrt<T>((x[i] - x[j]).length_sq());
+                f -= (v[i] - v[j]) * (cvis * kn.L(r));
+            }
+            f.y -= m * (T)P.gravity;
+            F[i] = f;
+        }
+        for (int k = 0; k < K; ++k) {
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) {
+                if (!fluid(i)) continue;
+                const V3<T> vs = v[i] + (F[i] + Fp[i]) * q;
+                z[i] = x[i] + vs * dt;
+                xs[i] = z[i];
+                for (int a = 0; a < 3; ++a) clamp_axis(xs[i][a], a);
+            }
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) {
+                if (!fluid(i)) continue;
+                T r_ = m * kn.W(T(0));
+                for (int j : nbl[i]) r_ += m * kn.W(vsqrt<T>((xs[i] - xs[j]).length_sq()));
+                rho[i] = r_;
+                p[i] += delta * MAXr<T>(r_ - rho0, T(0));
+            }
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) {
+                if (!fluid(i)) continue;
+                V3<T> f;
+                for (int j : nbl[i]) {
+                    const V3<T> d = x[i] - x[j];
+                    const T r = vsqrt<T>(d.length_sq());
+                    const T pj = fluid(j) ? p[j] : T(0);
+                    f -= d * (kn.G(r) * (p[i] + pj) * cp);
+                }
+                Fp[i] = f;
+            }
+            if (tp) { tp->z[k] = z; tp->p[k] = p; tp->rho[k] = rho; }
+        }
+        std::vector<V3<T>> y(x);
+        for (int i = 0; i < n; ++i) {
+            if (!fluid(i)) continue;
+            V3<T> vn = v[i] + (F[i] + Fp[i]) * q;
+            V3<T> yi = x[i] + vn * dt;
+            y[i] = yi;
+            for (int a = 0; a < 3; ++a) {
+                if (yi[a] < (T)P.collision_min[a]) { yi[a] = (T)P.collision_min[a]; vn[a] *= T(-0.5); }
+                else if (yi[a] > (T)P.collision_max[a]) { yi[a] = (T)P.collision_max[a]; vn[a] *= T(-0.5); }
+            }
+            x[i] = yi;
+            v[i] = vn;
+        }
+        if (tp) { tp->y = y; tp->nb.swap(nbl); }
+    }
+
+    void forward(int steps, bool record) {
+        tape.clear();
+        if (record) tape.resize(steps);
+        for (int s = 0; s < steps; ++s) step(record ? &tape[s] : nullptr);
+    }
+
+    bool clamped(T c, int a) const { return c < (T)P.collision_min[a] || c > (T)P.collision_max[a]; }
+
+    // reverse of one step: (gx, gv) hold dL/dx', dL/dv' on entry and dL/dx, dL/dv on exit
+    void step_back(const StepTape& tp, std::vector<V3<T>>& gx, std::vector<V3<T>>& gv) {
+        const DKern<T> kn((T)P.support_radius);
+        const T m = (T)P.initial_mass, rho0 = (T)P.rest_density, dt = (T)P.dt, q = dt / m;
+        const T cvis = (m / rho0) * (m / rho0) * (T)P.visc_const;
+        const T cp = (m * m) / (rho0 * rho0);
+        const T delta = (T)P.density_error_factor;
+        const std::vector<V3<T>>& X = tp.x;
+        const std::vector<V3<T>>& Vv = tp.v;
+        std::vector<V3<T>> gF(n), gFp(n), nx(n), nv(n);
+        std::vector<T> gp(n, T(0)), grho(n, T(0));
+        // integration
+        for (int i = 0; i < n; ++i) {
+            if (!fluid(i)) continue;
+            V3<T> gy, gvn;
+            for (int a = 0; a < 3; ++a) {
+                if (clamped(tp.y[i][a], a)) { gy[a] = T(0); gvn[a] = gv[i][a] * T(-0.5); }
+                else { gy[a] = gx[i][a]; gvn[a] = gv[i][a]; }
+            }
+            gvn += gy * dt;
+            nx[i] = gy;
+            nv[i] = gvn;
+            gF[i] = gvn * q;
+            gFp[i] = gF[i];
+        }
+        for (int k = K - 1; k >= 0; --k) {
+            const std::vector<T>& p = tp.p[k];
+            const std::vector<T>& rho = tp.rho[k];
+            // pressure force: dL/dp, dL/dx
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) {
+                if (!fluid(i)) continue;
+                T acc = T(0);
+                V3<T> gxi;
+                for (int j : tp.nb[i]) {
+                    const V3<T> d = X[i] - X[j];
+                    const T r = vsqrt<T>(d.length_sq());
+                    const V3<T> w = fluid(j) ? gFp[i] - gFp[j] : gFp[i];
+                    const T G = kn.G(r);
+                    acc -= cp * G * d.dot(w);
+                    // H(d) w = G w + (G'/r) d (d . w)
+                    const T pp = p[i] + (fluid(j) ? p[j] : T(0));
+                    const T s = r > T(0) ? kn.dG(r) / r * d.dot(w) : T(0);
+                    gxi -= (w * G + d * s) * (cp * pp);
+                }
+                gp[i] += acc;
+                nx[i] += gxi;
+            }
+            // pressure update -> density
+            for (int i = 0; i < n; ++i)
+                grho[i] = (fluid(i) && rho[i] > rho0) ? delta * gp[i] : T(0);
+            // density -> predicted positions -> prediction
+            std::vector<V3<T>> xs(n);
+            for (int i = 0; i < n; ++i) {
+                xs[i] = tp.z[k][i];
+                if (fluid(i)) for (int a = 0; a < 3; ++a) clamp_axis(xs[i][a], a);
+                else xs[i] = X[i];
+            }
+            std::vector<V3<T>> gfp_prev(n);
+#pragma omp parallel for schedule(dynamic, 100)
+            for (int i = 0; i < n; ++i) {
+                if (!fluid(i)) continue;
+                V3<T> gxs;
+                for (int j : tp.nb[i]) {
+                    const V3<T> d = xs[i] - xs[j];
+                    const T r = vsqrt<T>(d.length_sq());
+                    if (!(r < kn.h) || r == T(0)) continue;
+                    const T gr = grho[i] + (fluid(j) ? grho[j] : T(0));
+                    gxs += d * (m * kn.dW(r) / r * gr);
+                }
+                V3<T> gz;
+                for (int a = 0; a < 3; ++a) gz[a] = clamped(tp.z[k][i][a], a) ? T(0) : gxs[a];
+                nx[i] += gz;
+                const V3<T> gvs = gz * dt;
+                nv[i] += gvs;
+                gF[i] += gvs * q;
+                gfp_prev[i] = gvs * q;
+            }
+            gFp.swap(gfp_prev);
+        }
+        // external (viscosity) force
+#pragma omp parallel for schedule(dynamic, 100)
+        for (int i = 0; i < n; ++i) {
+            if (!fluid(i)) continue;
+            V3<T> gvi, gxi;
+            for (int j : tp.nb[i]) {
+                if (!fluid(j)) continue;
+                const V3<T> d = X[i] - X[j];
+                const T r = vsqrt<T>(d.length_sq());
+                const V3<T> w = gF[i] - gF[j];
+                gvi -= w * (cvis * kn.L(r));
+                if (r > T(0)) gxi -= d * (cvis * kn.dL(r) / r * (Vv[i] - Vv[j]).dot(w));
+            }
+            nv[i] += gvi;
+            nx[i] += gxi;
+        }
+        gx.swap(nx);
+        gv.swap(nv);
+    }
+};
+
+}  // namespace orc
+
+struct orc_diff {
+    int prec;
+    orc::DiffSim<float> f;
+    orc::DiffSim<double> d;
+};
+
+template <class T>
+static int diff_run(orc::DiffSim<T>& s, const upf_params* p, int n, const double* pos3, const double* vel3,
+                    const int* type, int steps, int K, double* out_pos3, double* out_vel3, const double* gpos3,
+                    const double* gvel3, double* g_pos0, double* g_vel0) {
+    s.P = *p;
+    s.n = n;
+    s.K = K;
+    s.x.resize(n);
+    s.v.resize(n);
+    s.type.assign(type, type + n);
+    for (int i = 0; i < n; ++i) {
+        s.x[i] = orc::V3<T>((T)pos3[3 * i], (T)pos3[3 * i + 1], (T)pos3[3 * i + 2]);
+        s.v[i] = vel3 ? orc::V3<T>((T)vel3[3 * i], (T)vel3[3 * i + 1], (T)vel3[3 * i + 2]) : orc::V3<T>();
+    }
+    const bool grad = gpos3 != nullptr;
+    s.forward(steps, grad);
+    if (s.err) return s.err;
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            if (out_pos3) out_pos3[3 * i + a] = (double)s.x[i][a];
+            if (out_vel3) out_vel3[3 * i + a] = (double)s.v[i][a];
+        }
+    if (!grad) return UPF_OK;
+    std::vector<orc::V3<T>> gx(n), gv(n);
+    for (int i = 0; i < n; ++i) {
+        gx[i] = orc::V3<T>((T)gpos3[3 * i], (T)gpos3[3 * i + 1], (T)gpos3[3 * i + 2]);
+        if (gvel3) gv[i] = orc::V3<T>((T)gvel3[3 * i], (T)gvel3[3 * i + 1], (T)gvel3[3 * i + 2]);
+    }
+    for (int st = steps - 1; st >= 0; --st) s.step_back(s.tape[st], gx, gv);
+    for (int i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            if (g_pos0) g_pos0[3 * i + a] = s.fluid(i) ? (double)gx[i][a] : 0.0;
+            if (g_vel0) g_vel0[3 * i + a] = s.fluid(i) ? (double)gv[i][a] : 0.0;
+        }
+    return UPF_OK;
+}
+
+// Rollout of `steps` differentiable steps with K iterations each (precision
+// 32 or 64); with gpos3 != NULL also the gradient of a loss L with
+// dL/dx_final = gpos3 (and dL/dv_final = gvel3, may be NULL) with respect to
+// the initial positions and velocities.
+extern "C" int orc_diff_run(const upf_params* p, int precision, int n, const double* pos3, const double* vel3,
+                            const int* type, int steps, int K, double* out_pos3, double* out_vel3,
+                            const double* gpos3, const double* gvel3, double* g_pos0, double* g_vel0) {
+    orc_diff s;
+    if (precision == 64)
+        return diff_run(s.d, p, n, pos3, vel3, type, steps, K, out_pos3, out_vel3, gpos3, gvel3, g_pos0, g_vel0);
+    return diff_run(s.f, p, n, pos3, vel3, type, steps, K, out_pos3, out_vel3, gpos3, gvel3, g_pos0, g_vel0);
+}
+
+// The closed-form kernels (for the tests that pin them against the reference's functions)
+extern "C" int orc_diff_kernels(float h, int n, const double* r, double* W, double* dW, double* G, double* dG,
+                                double* L) {
+    const orc::DKern<double> k((double)h);
+    for (int i = 0; i < n; ++i) {
+        W[i] = k.W(r[i]);
+        dW[i] = k.dW(r[i]);
+        G[i] = k.G(r[i]);
+        dG[i] = k.dG(r[i]);
+        L[i] = k.L(r[i]);
+    }
+    return 0;
+}

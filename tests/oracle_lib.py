@@ -41,6 +41,9 @@ def _bind(lib):
     lib.orc_sim_stats.argtypes = [C.c_void_p, C.c_int, pi, pf]
     lib.orc_sim_neighbors.argtypes = [C.c_void_p, pi, pi, C.c_int]
     lib.orc_rigid_step.argtypes = [pf, C.c_int, pf, pf, pf, f, f, f, C.c_int]
+    pd = C.POINTER(C.c_double)
+    lib.orc_diff_run.argtypes = [pp, C.c_int, C.c_int, pd, pd, pi, C.c_int, C.c_int, pd, pd, pd, pd, pd, pd]
+    lib.orc_diff_kernels.argtypes = [f, C.c_int, pd, pd, pd, pd, pd, pd]
     return lib
 
 
@@ -146,3 +149,27 @@ class OracleSim:
             self.lib.orc_sim_destroy(self.h)
         except Exception:
             pass
+
+
+def _pd(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def diff_run(p, pos, vel, types, steps, K, gpos=None, gvel=None, precision=64, omp=True):
+    """Differentiable PCISPH rollout on the oracle: returns (x_final, v_final)
+    or, with gpos (dL/dx_final, n x 3), (x_final, v_final, dL/dx0, dL/dv0)."""
+    n = len(types)
+    x = np.ascontiguousarray(np.asarray(pos, np.float64)[:, :3])
+    v = None if vel is None else np.ascontiguousarray(np.asarray(vel, np.float64)[:, :3])
+    t = np.ascontiguousarray(types, np.int32)
+    xo = np.zeros((n, 3))
+    vo = np.zeros((n, 3))
+    gp = None if gpos is None else np.ascontiguousarray(gpos, np.float64)
+    gv = None if gvel is None else np.ascontiguousarray(gvel, np.float64)
+    g0 = np.zeros((n, 3)) if gp is not None else None
+    gv0 = np.zeros((n, 3)) if gp is not None else None
+    rc = oracle(omp).orc_diff_run(C.byref(p), precision, n, _pd(x), _pd(v), ptr(t, C.c_int), steps, K, _pd(xo),
+                                  _pd(vo), _pd(gp), _pd(gv), _pd(g0), _pd(gv0))
+    if rc != 0:
+        raise RuntimeError("orc_diff_run failed: %d" % rc)
+    return (xo, vo) if gp is None else (xo, vo, g0, gv0)

@@ -90,3 +90,28 @@ def test_batch_rejects_mixed_physics(gpu):
     p1.visc_const = p1.visc_const * 2
     with pytest.raises(upf.UPFError):
         upf.Batch([p0, p1], [len(scenes[0][2]), len(scenes[1][2])])
+
+
+@pytest.mark.parametrize("mode", [upf.UPF_BOUNDARY_CPU_REF, upf.UPF_BOUNDARY_AKINCI])
+def test_single_scene_graph_equals_plain(gpu, mode):
+    """A plain scene replayed as a hipGraph (a batch of one) steps exactly like upf_step."""
+    p, pos, typ, nf = upf.standard_dam_break(14, 10, 12, seed=3)
+    p.boundary_pressure_mode = mode
+    print("single-scene graph: %d particles mode=%d" % (len(typ), mode), flush=True)
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    sim.step(7, upf.UPF_PCISPH_STATIC)
+    g = sim.state()
+    Kp, rp = sim.stats()
+    sim.close()
+    b = upf.Batch([p], [len(typ)])
+    b.upload(pos, typ)
+    b.step(3, use_graph=True)
+    b.step(4, use_graph=True)
+    st = b.state()
+    K, r = b.stats()
+    b.close()
+    assert np.array_equal(st["pos"][:, :3], g["pos"][:, :3])
+    assert np.array_equal(st["vel"][:, :3], g["vel"][:, :3])
+    assert list(K[:, 0]) == list(Kp)
+    assert np.array_equal(r[:, 0], rp)

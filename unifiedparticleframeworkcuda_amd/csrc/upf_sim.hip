@@ -560,43 +560,35 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     }
 }
 
-// The loop test of PredictionCorrectionStep (:5376-5398) on the device, one
-// thread per scene: max over the scene's slots, densityErrorInPercent, the
-// too-large latch and the number of iterations the scene's loop runs; the
-// largest of those goes to ctl->want_any.
-__global__ __launch_bounds__(256) void k_loop_control(int t, StepArgs A, Ctl* ctl) {
-    if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
-    __shared__ int wmx[4];
+// The loop test of PredictionCorrectionStep (:5376-5398) on the device for
+// scene sc after iteration t's densities (one wave): max over the scene's
+// slots (which it clears for the next iteration), densityErrorInPercent, the
+// too-large latch and the number of iterations the scene's loop runs; that
+// number also raises ctl->want_any.  Every scene's count only grows during a
+// step, so want_any needs no reset between iterations.
+__device__ inline void loop_test(const StepArgs& A, int sc, int t, int lane, Ctl* ctl) {
     const upf_params& P = A.P;
-    int best = 0;
-    for (int sc = threadIdx.x; sc < A.n_scenes; sc += blockDim.x) {
-        SceneCtl& C = A.sctl[sc];
-        if (t < C.want) {
-            uint32_t b = 0u;
-            for (int k = 0; k < A.slots_per_scene; ++k) {
-                uint32_t* slot = &A.max_slots[(sc * A.slots_per_scene + k) * 32];
-                b = max(b, *slot);
-                *slot = 0u;
-            }
-            const float mx = __uint_as_float(b);
-            const float pct = fmax_ref(0.1f * mx - 100.0f, 0.0f);
-            int too_large = C.too_large;
-            if (pct < P.max_density_error_allowed) too_large = 0;
-            const int it = t + 1;
-            const bool next = it < P.min_loops || (too_large && it < P.max_loops);
-            C.too_large = too_large;
-            C.last_max = mx;
-            C.want = next ? it + 1 : it;
-        }
-        best = max(best, C.want);
+    SceneCtl& C = A.sctl[sc];
+    if (t >= C.want) return;
+    uint32_t b = 0u;
+    if (lane < A.slots_per_scene) {
+        uint32_t* slot = &A.max_slots[(sc * A.slots_per_scene + lane) * 32];
+        b = *slot;
+        *slot = 0u;
     }
-    for (int o = 32; o > 0; o >>= 1) best = max(best, __shfl_xor(best, o, 64));
-    if ((threadIdx.x & 63) == 0) wmx[threadIdx.x >> 6] = best;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int b = wmx[0];
-        for (int k = 1; k < (int)(blockDim.x + 63) / 64; ++k) b = max(b, wmx[k]);
-        __atomic_store_n(&ctl->want_any, b, __ATOMIC_RELAXED);
+    for (int o = 32; o > 0; o >>= 1) b = max(b, (uint32_t)__shfl_xor((int)b, o, 64));
+    if (lane == 0) {
+        const float mx = __uint_as_float(b);
+        const float pct = fmax_ref(0.1f * mx - 100.0f, 0.0f);
+        int too_large = C.too_large;
+        if (pct < P.max_density_error_allowed) too_large = 0;
+        const int it = t + 1;
+        const bool next = it < P.min_loops || (too_large && it < P.max_loops);
+        const int want = next ? it + 1 : it;
+        C.too_large = too_large;
+        C.last_max = mx;
+        C.want = want;
+        atomicMax(&ctl->want_any, want);
     }
 }
 
@@ -628,11 +620,19 @@ __global__ void k_record(StepArgs A, Ctl* ctl, int* __restrict__ k_out, float* _
 }
 
 // ComputeCorrectivePressureForce (:5493-5522), then the prediction of the
-// next iteration for the same particle.
+// next iteration for the same particle; the loop test of the iteration runs
+// first (the force pass does not depend on it).
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                  Ctl* ctl) {
     if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
+    // the loop test of iteration t, one wave per scene: it only ever raises a
+    // scene's count past t, so the particle loop below reads either value
+    {
+        const int waves = gridDim.x * (BLOCK / 64);
+        for (int sc = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); sc < A.n_scenes; sc += waves)
+            loop_test(A, sc, t, threadIdx.x & 63, ctl);
+    }
     __shared__ float lpg[LUT_N];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lpg[k] = luts[LUT_N + k];
     __syncthreads();
@@ -1449,7 +1449,6 @@ static void enqueue_iteration(upf_sim* s, const StepArgs& A, int t) {
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_DENSITY, s->st);
     hipLaunchKernelGGL(k_density<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    hipLaunchKernelGGL(k_loop_control, dim3(1), dim3(256), 0, s->st, t, A, s->ctl);
     s->prof.end(s->st);
     s->prof.begin(PC_FORCE, s->st);
     hipLaunchKernelGGL(k_force<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
@@ -2127,7 +2126,9 @@ int upf_batch_create(int n_scenes, const upf_params* scene_params, const int* co
             b->scene_of.insert(b->scene_of.end(), (size_t)counts[k], k);
             soff += counts[k];
         }
-        if (hipMemcpy(s->d_doms, doms.data(), doms.size() * sizeof(Dom), hipMemcpyHostToDevice) != hipSuccess)
+        // one scene: its domain travels in the kernel arguments
+        if (n_scenes > 1 &&
+            hipMemcpy(s->d_doms, doms.data(), doms.size() * sizeof(Dom), hipMemcpyHostToDevice) != hipSuccess)
             rc = UPF_ERR_HIP;
     }
     if (rc) {
@@ -2149,7 +2150,9 @@ int upf_batch_destroy(upf_batch* b) {
 
 int upf_batch_upload(upf_batch* b, const float* pos4, const float* vel4, const int* type) {
     if (!b) return UPF_ERR_ARG;
-    return upload_common(b->s, (int)b->scene_of.size(), pos4, vel4, type, b->scene_of.data(), b->sp.data());
+    const bool many = b->s->n_scenes > 1;
+    return upload_common(b->s, (int)b->scene_of.size(), pos4, vel4, type, many ? b->scene_of.data() : nullptr,
+                         many ? b->sp.data() : nullptr);
 }
 
 int upf_batch_step(upf_batch* b, int nsteps, int use_graph) {
