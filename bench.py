@@ -168,6 +168,30 @@ def run_batch(args):
     return out
 
 
+def run_adjoint(args):
+    """The gradient workload: 65,536-particle dam break (64x32x32 lattice),
+    64 steps of 5 iterations, checkpoint every 8 steps; gradient of the mean
+    squared distance to a target cloud with respect to the initial
+    velocities.  Device time of the forward and of the backward pass."""
+    from unifiedparticleframeworkcuda_amd import adjoint as adj
+    p, pos, typ, vel, target, f = adj.differentiable_dam_break()
+    r = adj.Rollout(p, pos, typ, 64, 5, checkpoint_every=8)
+    best = None
+    for _ in range(3):
+        x, _ = r.forward(vel)
+        g = np.zeros_like(x)
+        g[f] = 2.0 * (x[f] - target[f]) / f.sum()
+        r.backward(g)
+        fwd, bwd, mem = r.timing()
+        if best is None or fwd + bwd < best[0] + best[1]:
+            best = (fwd, bwd, mem)
+    r.close()
+    fwd, bwd, mem = best
+    return dict(particles=len(typ), fluid=int(f.sum()), steps=64, iterations=5, checkpoint_every=8,
+                forward_ms=fwd, backward_ms=bwd, backward_over_forward=bwd / fwd, device_mb=mem / 1e6,
+                forward_updates_per_s=int(f.sum()) * 64 / (fwd / 1e3))
+
+
 def cpu_baseline(args, budget_s):
     """The oracle (OpenMP C++ port of the reference CPU path) on the same
     workload: steps until the time budget is used (at least one)."""
@@ -203,6 +227,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (scale for hardware byte counters)")
     args = ap.parse_args()
@@ -255,6 +280,8 @@ def main():
     }
     if not args.no_batch:
         out["batched"] = run_batch(args)
+    if not args.no_adjoint:
+        out["adjoint"] = run_adjoint(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, args.cpu_budget)
     print(json.dumps(out))

@@ -310,7 +310,10 @@ int upf_batch_sim(upf_batch* b, upf_sim** sim);
  * step is CpuParticlePhysicsStaticBoundariesPCISPH (src/cpu/UnifiedPhysics.cpp:4220)
  * in the CPU boundary mode with the kernels in closed form (KernelM4,
  * kernelPressureGrad, kernelViscosityLaplacian, src/cpu/WeightingKernel.cpp:75,165,190)
- * instead of their 300-entry tables, so that it has a derivative.  Particles
+ * instead of their 300-entry tables, the pressure clamp max(rho - rho0, 0)
+ * smoothed to tau log(1 + exp((rho - rho0) / tau)), tau = 1 kg/m^3, and the
+ * box handled by position clamp + wall penalty force (no velocity
+ * reflection), so that the rollout is differentiable.  Particles
  * keep their upload order (types: liquid and frozen).  The forward pass keeps
  * the state every `checkpoint_every` steps; the backward pass recomputes one
  * segment at a time and runs it in reverse.                                */

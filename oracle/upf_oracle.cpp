@@ -1198,14 +1198,26 @@ extern "C" int orc_rigid_step(float* st, int n, const float* ppos, const float* 
 // upload order; the z-index sort only orders the neighbour lists.
 //
 // Forward, per step (fluid particles; walls fixed):
-//   F_i   = -sum_j (v_i - v_j) V0^2 mu L(r_ij) - m g e_y          (fluid j)
+//   F_i   = -sum_j (v_i - v_j) V0^2 mu L(r_ij) - m g e_y + B(x_i)  (fluid j)
+//   with B the wall penalty force of CalculateBoundaryForcePerLiquidParticleHost
+//   (:2032-2069): per axis k_b (lo + d_f - x) within d_f of the lower face of
+//   the collision box, -k_b (x + d_f - hi) within d_f of the upper face,
+//   k_b = 2 F_max / d_f;
 //   iteration k = 0..K-1 (Fp = 0 before the first):
 //     v*_i = v_i + (F_i + Fp_i) dt/m;  z_i = x_i + v*_i dt;  x*_i = clamp(z_i)
 //     rho_i = m W(0) + sum_j [r*_ij < h] m W(r*_ij)               (all j)
-//     p_i  += delta max(rho_i - rho0, 0)
+//     p_i  += delta sp(rho_i - rho0)
 //     Fp_i  = -(m^2/rho0^2) sum_j (p_i + p_j) G(r_ij) (x_i - x_j)  (p_wall = 0)
-//   v'_i = v_i + (F_i + Fp_i) dt/m;  y_i = x_i + v'_i dt;  x'_i = clamp(y_i),
-//   v'_i,a *= -0.5 on every clamped axis.
+//   with sp(s) = tau log(1 + exp(s / tau)), tau = 1 kg/m^3 (0.1 % of rho0),
+//   a smooth form of the CPU path's max(s, 0) (:5477-5478): at the kink,
+//   where a settling fluid keeps many particles, rounding alone decides the
+//   branch, and FP32 and FP64 gradients of a 64-step rollout differ by O(1);
+//   v'_i = v_i + (F_i + Fp_i) dt/m;  y_i = x_i + v'_i dt;  x'_i = clamp(y_i).
+// The CPU path's wall response (clamp plus velocity reflection with damping
+// -0.5, :4000-4044) jumps when a particle crosses a face, which no derivative
+// or finite difference can follow; this step keeps the position clamp, drops
+// the reflection and lets the penalty force push particles back, so that the
+// rollout is a continuous function of its inputs.
 // Backward: the exact reverse of the above (neighbour sets held fixed), in
 // gather form over the symmetric lists.
 namespace orc {
@@ -1261,6 +1273,37 @@ template <class T> struct DiffSim {
     int err = UPF_OK;
 
     bool fluid(int i) const { return type[i] == UPF_LIQUID; }
+
+    static constexpr double kTau = 1.0;  // width of the smooth pressure clamp, kg/m^3
+    static T softplus(T s) {
+        const T sx = s / T(kTau);
+        return sx > T(30) ? s : T(kTau) * std::log1p(std::exp(sx));
+    }
+    static T softplus_grad(T s) { return T(1) / (T(1) + std::exp(-s / T(kTau))); }
+
+    T kb() const { return T(2) * (T)P.max_boundary_force / (T)P.force_distance; }
+    // wall penalty force on a liquid particle at c
+    V3<T> wall_force(const V3<T>& c) const {
+        V3<T> f;
+        const T fd = (T)P.force_distance, k = kb();
+        for (int a = 0; a < 3; ++a) {
+            const T mn = (T)P.collision_min[a], mx = (T)P.collision_max[a];
+            if (c[a] < mn + fd) f[a] = f[a] + (mn + fd - c[a]) * k;
+            if (c[a] > mx - fd) f[a] = f[a] - (c[a] + fd - mx) * k;
+        }
+        return f;
+    }
+    // d(wall force)/dx per axis (diagonal)
+    V3<T> wall_force_grad(const V3<T>& c) const {
+        V3<T> g;
+        const T fd = (T)P.force_distance, k = kb();
+        for (int a = 0; a < 3; ++a) {
+            const T mn = (T)P.collision_min[a], mx = (T)P.collision_max[a];
+            if (c[a] < mn + fd) g[a] = g[a] - k;
+            if (c[a] > mx - fd) g[a] = g[a] - k;
+        }
+        return g;
+    }
 
     // neighbour lists of the current positions, ascending in (key, id) order
     void lists(std::vector<std::vector<int>>& nb) {
@@ -1342,7 +1385,7 @@ template <class T> struct DiffSim {
                 f -= (v[i] - v[j]) * (cvis * kn.L(r));
             }
             f.y -= m * (T)P.gravity;
-            F[i] = f;
+            F[i] = f + wall_force(x[i]);
         }
         for (int k = 0; k < K; ++k) {
 #pragma omp parallel for schedule(dynamic, 100)
@@ -1359,7 +1402,7 @@ template <class T> struct DiffSim {
                 T r_ = m * kn.W(T(0));
                 for (int j : nbl[i]) r_ += m * kn.W(vsqrt<T>((xs[i] - xs[j]).length_sq()));
                 rho[i] = r_;
-                p[i] += delta * MAXr<T>(r_ - rho0, T(0));
+                p[i] += delta * softplus(r_ - rho0);
             }
 #pragma omp parallel for schedule(dynamic, 100)
             for (int i = 0; i < n; ++i) {
@@ -1378,13 +1421,10 @@ template <class T> struct DiffSim {
         std::vector<V3<T>> y(x);
         for (int i = 0; i < n; ++i) {
             if (!fluid(i)) continue;
-            V3<T> vn = v[i] + (F[i] + Fp[i]) * q;
+            const V3<T> vn = v[i] + (F[i] + Fp[i]) * q;
             V3<T> yi = x[i] + vn * dt;
             y[i] = yi;
-            for (int a = 0; a < 3; ++a) {
-                if (yi[a] < (T)P.collision_min[a]) { yi[a] = (T)P.collision_min[a]; vn[a] *= T(-0.5); }
-                else if (yi[a] > (T)P.collision_max[a]) { yi[a] = (T)P.collision_max[a]; vn[a] *= T(-0.5); }
-            }
+            for (int a = 0; a < 3; ++a) clamp_axis(yi[a], a);
             x[i] = yi;
             v[i] = vn;
         }
@@ -1413,11 +1453,8 @@ template <class T> struct DiffSim {
         // integration
         for (int i = 0; i < n; ++i) {
             if (!fluid(i)) continue;
-            V3<T> gy, gvn;
-            for (int a = 0; a < 3; ++a) {
-                if (clamped(tp.y[i][a], a)) { gy[a] = T(0); gvn[a] = gv[i][a] * T(-0.5); }
-                else { gy[a] = gx[i][a]; gvn[a] = gv[i][a]; }
-            }
+            V3<T> gy, gvn = gv[i];
+            for (int a = 0; a < 3; ++a) gy[a] = clamped(tp.y[i][a], a) ? T(0) : gx[i][a];
             gvn += gy * dt;
             nx[i] = gy;
             nv[i] = gvn;
@@ -1449,7 +1486,7 @@ template <class T> struct DiffSim {
             }
             // pressure update -> density
             for (int i = 0; i < n; ++i)
-                grho[i] = (fluid(i) && rho[i] > rho0) ? delta * gp[i] : T(0);
+                grho[i] = fluid(i) ? delta * gp[i] * softplus_grad(rho[i] - rho0) : T(0);
             // density -> predicted positions -> prediction
             std::vector<V3<T>> xs(n);
             for (int i = 0; i < n; ++i) {
@@ -1492,6 +1529,9 @@ template <class T> struct DiffSim {
                 gvi -= w * (cvis * kn.L(r));
                 if (r > T(0)) gxi -= d * (cvis * kn.dL(r) / r * (Vv[i] - Vv[j]).dot(w));
             }
+            // wall penalty force
+            const V3<T> wg = wall_force_grad(X[i]);
+            gxi += gF[i] * wg;
             nv[i] += gvi;
             nx[i] += gxi;
         }

@@ -6,14 +6,24 @@
 // src/cpu/WeightingKernel.cpp:75-97, kernelPressureGrad :165-180,
 // kernelViscosityLaplacian :190-200) rather than through the 300-entry tables,
 // whose step values have no useful derivative:
-//   F_i   = -sum_j (v_i - v_j) V0^2 mu L(r_ij) - m g e_y          (fluid j)
+//   F_i   = -sum_j (v_i - v_j) V0^2 mu L(r_ij) - m g e_y + B(x_i)  (fluid j)
+//   with B the wall penalty force (CalculateBoundaryForcePerLiquidParticleHost,
+//   src/cpu/UnifiedPhysics.cpp:2032-2069), k_b = 2 F_max / d_f per unit depth
+//   within d_f of a face of the collision box;
 //   iteration k (Fp = 0 before the first):
 //     v*_i = v_i + (F_i + Fp_i) dt/m;  z_i = x_i + v*_i dt;  x*_i = clamp(z_i)
 //     rho_i = m W(0) + sum_j [r*_ij < h] m W(r*_ij)
-//     p_i  += delta max(rho_i - rho0, 0)
+//     p_i  += delta sp(rho_i - rho0),  sp(s) = tau log(1 + exp(s / tau))
 //     Fp_i  = -(m^2/rho0^2) sum_j (p_i + p_j) G(r_ij) (x_i - x_j)   (p_wall = 0)
-//   v'_i = v_i + (F_i + Fp_i) dt/m;  y_i = x_i + v'_i dt;  x'_i = clamp(y_i),
-//   v'_i,a *= -0.5 on every clamped axis.
+//   v'_i = v_i + (F_i + Fp_i) dt/m;  y_i = x_i + v'_i dt;  x'_i = clamp(y_i).
+// sp with tau = 1 kg/m^3 (0.1 % of rho0) is a smooth form of the CPU path's
+// max(s, 0) (:5477-5478): a settling fluid keeps many particles at the kink,
+// where rounding alone decides the branch and FP32 and FP64 gradients of a
+// 64-step rollout differ by O(1).
+// The CPU path's velocity reflection at the box (:4000-4044) jumps when a
+// particle crosses a face; here the position clamp stays, the reflection is
+// dropped and the penalty force pushes particles back, so that the rollout is
+// a continuous function of its inputs.
 // Particles keep their upload order; each step sorts z-index keys only to
 // build the cell table and neighbour lists (ascending sorted order, so every
 // sum runs in the order of the CPU restatement).
@@ -23,7 +33,7 @@
 //   dL/dp_i  += -(m^2/rho0^2) sum_j G (x_ij . (gFp_i - gFp_j))
 //   dL/dx_i  += -(m^2/rho0^2) sum_j (p_i + p_j) H(x_ij) (gFp_i - gFp_j),
 //               H(d) = G I + (G'/r) d d^T
-//   dL/drho_i = delta [rho_i > rho0] dL/dp_i
+//   dL/drho_i = delta sp'(rho_i - rho0) dL/dp_i
 //   dL/dx*_i  = sum_j m W'(r*)/r* x*_ij (dL/drho_i + dL/drho_j)
 // and the elementwise reverses of the prediction, the clamps and the
 // integration.  Memory: the state every `ckpt` steps is kept; the backward
@@ -81,10 +91,31 @@ struct Kern {  // closed-form kernels and their radial derivatives, FP32
     __device__ float dL(float r) const { return r < h ? -cv : 0.0f; }
 };
 
+constexpr float kTau = 1.0f;  // width of the smooth pressure clamp, kg/m^3
+
+__device__ inline float softplus(float s) {
+    const float sx = s / kTau;
+    return sx > 30.0f ? s : kTau * log1pf(expf(sx));
+}
+__device__ inline float softplus_grad(float s) { return 1.0f / (1.0f + expf(-s / kTau)); }
+
 struct Consts {
     float m, rho0, dt, q, cvis, cp, delta, g, h;
+    float fd, kb;  // wall penalty: depth and stiffness 2 F_max / d_f
     float cmin[3], cmax[3];
 };
+
+// wall penalty force on a liquid particle at (x, y, z) and its diagonal derivative
+__device__ inline void wall_force(const Consts& C, float x, float y, float z, float* f, float* g) {
+    const float c[3] = {x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        f[a] = 0.0f;
+        g[a] = 0.0f;
+        if (c[a] < C.cmin[a] + C.fd) { f[a] = f[a] + (C.cmin[a] + C.fd - c[a]) * C.kb; g[a] = g[a] - C.kb; }
+        if (c[a] > C.cmax[a] - C.fd) { f[a] = f[a] - (c[a] + C.fd - C.cmax[a]) * C.kb; g[a] = g[a] - C.kb; }
+    }
+}
 
 __device__ inline bool is_fluid(const float4& x) { return __float_as_int(x.w) == UPF_LIQUID; }
 __device__ inline float len3(float dx, float dy, float dz) { return sqrtf(dx * dx + dy * dy + dz * dz); }
@@ -174,7 +205,9 @@ __global__ void k_ext(int n, const float4* __restrict__ x, const float4* __restr
         fz = fz - (vi.z - vj.z) * s;
     }
     fy = fy - C.m * C.g;
-    F[i] = make_float4(fx, fy, fz, 0.0f);
+    float wf[3], wg[3];
+    wall_force(C, xi.x, xi.y, xi.z, wf, wg);
+    F[i] = make_float4(fx + wf[0], fy + wf[1], fz + wf[2], 0.0f);
 }
 
 __device__ inline float clampc(float c, float mn, float mx) {
@@ -214,7 +247,7 @@ __global__ void k_density(int n, const float4* __restrict__ xs, const uint32_t* 
         const float4 xj = xs[nb[(size_t)k * n + i]];
         r_ = r_ + C.m * kn.W(len3(xi.x - xj.x, xi.y - xj.y, xi.z - xj.z));
     }
-    const float pi = p[i] + C.delta * fmax_ref(r_ - C.rho0, 0.0f);
+    const float pi = p[i] + C.delta * softplus(r_ - C.rho0);
     p[i] = pi;
     if (p_tape) { p_tape[i] = pi; rho_tape[i] = r_; }
 }
@@ -250,17 +283,14 @@ __global__ void k_integrate(int n, float4* __restrict__ x, float4* __restrict__ 
     const float4 xi = x[i];
     if (!is_fluid(xi)) { if (y_tape) y_tape[i] = xi; return; }
     const float4 vi = v[i], f = F[i], fp = Fp[i];
-    float vn[3] = {vi.x + (f.x + fp.x) * C.q, vi.y + (f.y + fp.y) * C.q, vi.z + (f.z + fp.z) * C.q};
+    const float vn[3] = {vi.x + (f.x + fp.x) * C.q, vi.y + (f.y + fp.y) * C.q, vi.z + (f.z + fp.z) * C.q};
     const float xo[3] = {xi.x, xi.y, xi.z};
     float y[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) y[a] = xo[a] + vn[a] * C.dt;
     if (y_tape) y_tape[i] = make_float4(y[0], y[1], y[2], xi.w);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        if (y[a] < C.cmin[a]) { y[a] = C.cmin[a]; vn[a] *= -0.5f; }
-        else if (y[a] > C.cmax[a]) { y[a] = C.cmax[a]; vn[a] *= -0.5f; }
-    }
+    for (int a = 0; a < 3; ++a) y[a] = clampc(y[a], C.cmin[a], C.cmax[a]);
     x[i] = make_float4(y[0], y[1], y[2], xi.w);
     v[i] = make_float4(vn[0], vn[1], vn[2], 0.0f);
 }
@@ -282,8 +312,8 @@ __global__ void kr_integrate(int n, const float4* __restrict__ y, const float4* 
     float gy[3], gvn[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        if (outside(ax(yi, k), k, C)) { gy[k] = 0.0f; gvn[k] = ax(b, k) * -0.5f; }
-        else { gy[k] = ax(a, k); gvn[k] = ax(b, k); }
+        gy[k] = outside(ax(yi, k), k, C) ? 0.0f : ax(a, k);
+        gvn[k] = ax(b, k);
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) gvn[k] = gvn[k] + gy[k] * C.dt;
@@ -344,7 +374,7 @@ __global__ void kr_density(int n, const float4* __restrict__ x, const float4* __
     const float4 zi = z[i];
     const float xsi[3] = {clampc(zi.x, C.cmin[0], C.cmax[0]), clampc(zi.y, C.cmin[1], C.cmax[1]),
                           clampc(zi.z, C.cmin[2], C.cmax[2])};
-    const float gri = rho[i] > C.rho0 ? C.delta * gp[i] : 0.0f;
+    const float gri = C.delta * gp[i] * softplus_grad(rho[i] - C.rho0);
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
     const int c = cnt[i];
     for (int k = 0; k < c; ++k) {
@@ -357,7 +387,7 @@ __global__ void kr_density(int n, const float4* __restrict__ x, const float4* __
             xsj[0] = clampc(zj.x, C.cmin[0], C.cmax[0]);
             xsj[1] = clampc(zj.y, C.cmin[1], C.cmax[1]);
             xsj[2] = clampc(zj.z, C.cmin[2], C.cmax[2]);
-            grj = rho[j] > C.rho0 ? C.delta * gp[j] : 0.0f;
+            grj = C.delta * gp[j] * softplus_grad(rho[j] - C.rho0);
         } else {
             xsj[0] = xj.x; xsj[1] = xj.y; xsj[2] = xj.z;
         }
@@ -420,6 +450,11 @@ __global__ void kr_ext(int n, const float4* __restrict__ x, const float4* __rest
             gxz = gxz - d.z * s;
         }
     }
+    float wf[3], wg[3];
+    wall_force(C, xi.x, xi.y, xi.z, wf, wg);
+    gxx = gxx + gi.x * wg[0];
+    gxy = gxy + gi.y * wg[1];
+    gxz = gxz + gi.z * wg[2];
     float4 ov = nv[i];
     ov.x = ov.x + gvx; ov.y = ov.y + gvy; ov.z = ov.z + gvz;
     nv[i] = ov;
@@ -485,6 +520,8 @@ struct upf_adjoint {
         C.delta = P.density_error_factor;
         C.g = P.gravity;
         C.h = P.support_radius;
+        C.fd = P.force_distance;
+        C.kb = 2.0f * P.max_boundary_force / P.force_distance;
         for (int a = 0; a < 3; ++a) {
             C.cmin[a] = P.collision_min[a];
             C.cmax[a] = P.collision_max[a];
