@@ -57,6 +57,11 @@ def run_gpu(args):
     p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz)
     n = len(typ)
     sim = upf.Simulation(p, n)
+    tuning = None
+    if not args.no_tune:
+        from unifiedparticleframeworkcuda_amd import autotune
+        tuning, tune_ms = autotune.tune_simulation(sim, pos, typ, "pcisph_dam_break_%dx%dx%d" % (nx, ny, nz),
+                                                   use_cache=not args.retune)
     sim.upload(pos, typ)
     sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
     L = upf.lib()
@@ -87,10 +92,12 @@ def run_gpu(args):
         if cnt.value:
             prof[L.upf_profile_name(c).decode()] = (ms.value, cnt.value)
     sim.close()
-    out = dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof)
+    out = dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof, tuning=tuning)
     if not args.no_graph:
         # the same scene as a batch of one: every step one hipGraph replay
         b = upf.Batch([p], [n], max_record_steps=args.warmup + args.steps + 8)
+        if tuning:
+            b.set_tuning(tuning)
         b.upload(pos, typ)
         b.step(args.warmup, use_graph=True)
         hip.hipDeviceSynchronize()
@@ -228,6 +235,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
+    ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (scale for hardware byte counters)")
     args = ap.parse_args()
@@ -272,6 +281,7 @@ def main():
         "step_roofline": {"achieved": total_bytes / sec / 1e9, "unit": "GB/s", "frac": total_bytes / sec / HBM_PEAK,
                           "bytes_per_update": total_bytes / (nf * args.steps)},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in g["prof"].items()},
+        "tuning": g["tuning"],
         "graph_replay": (None if "graph_seconds" not in g else
                          {"ms_per_step": g["graph_seconds"] / args.steps * 1e3,
                           "updates_per_s": nf * args.steps / g["graph_seconds"],

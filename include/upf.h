@@ -332,6 +332,21 @@ int upf_adjoint_backward(upf_adjoint* a, const float* dl_dpos4, const float* dl_
  * rollout's stream) and the device memory the rollout holds.              */
 int upf_adjoint_timing(upf_adjoint* a, float* forward_ms, float* backward_ms, size_t* device_bytes);
 
+/* ------------------------------------------------------ launch tuning */
+/* Launch shapes that change speed only, never results (no reference
+ * counterpart; the reference fixes them at compile time).  Defaults
+ * {256, 8192, 4, 64, 4}.  A batch is tuned through its upf_batch_sim handle
+ * before its first graph capture.                                          */
+typedef struct upf_tuning {
+    int loop_block;   /* threads per block of the grid-stride passes: 64, 128, 256     */
+    int grid_cap;     /* blocks of the grid-stride passes at most (>= 1)               */
+    int nbr_rounds;   /* list build: candidates per lane per chunk, 2, 4 or 8 (x 64)    */
+    int nbr_stage;    /* list build: entries per list staged in LDS, 32 or 64          */
+    int speculate;    /* host loop: PCISPH iterations enqueued past the known count    */
+} upf_tuning;
+int upf_set_tuning(upf_sim* s, const upf_tuning* t);
+int upf_get_tuning(const upf_sim* s, upf_tuning* t);
+
 /* ------------------------------------------------------ profiling */
 /* Per-kernel-class device time measured with HIP events recorded on the
  * simulation's own stream around every launch (the reference's

@@ -1,0 +1,70 @@
+"""Launch tuning changes speed only: every tuning gives bit-identical steps
+(including list-build staging overflow), invalid shapes are rejected, and the
+autotuner runs, applies and caches its choice."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import unifiedparticleframeworkcuda_amd as upf
+from unifiedparticleframeworkcuda_amd import autotune
+from unifiedparticleframeworkcuda_amd._abi import Tuning
+
+pytestmark = pytest.mark.gpu
+
+
+def _compressed_scene():
+    # lattice at 0.9 spacing: ~45 neighbours per particle, past a 32-entry stage
+    p = upf.make_params(0.01, 0.001)
+    pos, typ, nf = upf.dam_break(p, 14, 10, 12, (0.28, 0.2, 0.2), lattice_rel=0.9)
+    upf.density_error_factor(p)
+    return p, pos, typ
+
+
+def _run(p, pos, typ, tuning, steps=5):
+    sim = upf.Simulation(p, len(typ))
+    sim.set_tuning(tuning)
+    sim.upload(pos, typ)
+    sim.step(steps, upf.UPF_PCISPH_STATIC)
+    st = sim.state()
+    K, _ = sim.stats()
+    sim.close()
+    return st, K
+
+
+@pytest.mark.parametrize("tuning", [
+    dict(loop_block=64, grid_cap=16, nbr_rounds=2, nbr_stage=32, speculate=1),
+    dict(loop_block=128, grid_cap=2048, nbr_rounds=8, nbr_stage=32, speculate=6),
+    dict(loop_block=256, grid_cap=3, nbr_rounds=4, nbr_stage=64, speculate=2),
+])
+def test_tunings_give_identical_steps(gpu, tuning):
+    p, pos, typ = _compressed_scene()
+    print("tuning %s, %d particles" % (tuning, len(typ)), flush=True)
+    ref, Kr = _run(p, pos, typ, autotune.DEFAULT)
+    got, Kg = _run(p, pos, typ, tuning)
+    assert list(Kr) == list(Kg)
+    for k in ("pos", "vel", "rho", "p", "force"):
+        assert np.array_equal(ref[k], got[k]), k
+
+
+def test_invalid_tuning_rejected(gpu):
+    p, pos, typ = _compressed_scene()
+    sim = upf.Simulation(p, len(typ))
+    for d in (dict(autotune.DEFAULT, loop_block=96), dict(autotune.DEFAULT, nbr_rounds=3),
+              dict(autotune.DEFAULT, nbr_stage=16), dict(autotune.DEFAULT, grid_cap=0)):
+        with pytest.raises(upf.UPFError):
+            sim.set_tuning(d)
+    assert sim.tuning().as_dict() == autotune.DEFAULT
+    sim.close()
+
+
+def test_autotuner_applies_and_caches(gpu, tmp_path):
+    p, pos, typ, nf = upf.standard_dam_break(16, 12, 16)
+    sim = upf.Simulation(p, len(typ))
+    path = str(tmp_path / "tuning.json")
+    best, ms = autotune.tune_simulation(sim, pos, typ, "dam_break_test", steps=2, warmup=1, path=path)
+    assert sim.tuning().as_dict() == best and ms > 0
+    assert autotune.load("dam_break_test", len(typ), path) == best
+    again, ms2 = autotune.tune_simulation(sim, pos, typ, "dam_break_test", path=path)
+    assert again == best and ms2 is None  # from the cache
+    sim.close()

@@ -12,7 +12,7 @@ import os
 
 import numpy as np
 
-from ._abi import (Params, DamBreakDesc, RigidSceneDesc, Body, ptr, ERROR_NAMES,  # noqa: F401
+from ._abi import (Params, DamBreakDesc, RigidSceneDesc, Body, Tuning, ptr, ERROR_NAMES,  # noqa: F401
                    UPF_OK, UPF_LIQUID, UPF_FROZEN, UPF_RIGID, UPF_PCISPH_STATIC,
                    UPF_WCSPH_STATIC, UPF_PCISPH_RIGID, UPF_BOUNDARY_CPU_REF, UPF_BOUNDARY_AKINCI)
 
@@ -66,6 +66,8 @@ def lib():
         L.upf_batch_download.argtypes = [vp, pf, pf, pf]
         L.upf_batch_get_stats.argtypes = [vp, C.c_int, pi, pf]
         L.upf_batch_sim.argtypes = [vp, C.POINTER(vp)]
+        L.upf_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+        L.upf_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
         L.upf_profile_enable.argtypes = [vp, C.c_int]
         L.upf_profile_reset.argtypes = [vp]
         L.upf_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
@@ -207,6 +209,15 @@ class Simulation:
 
     def step(self, nsteps=1, method=UPF_PCISPH_STATIC):
         check("upf_step", lib().upf_step(self.h, int(nsteps), int(method)))
+
+    def set_tuning(self, tuning):
+        t = tuning if isinstance(tuning, Tuning) else Tuning.from_dict(tuning)
+        check("upf_set_tuning", lib().upf_set_tuning(self.h, C.byref(t)))
+
+    def tuning(self):
+        t = Tuning()
+        check("upf_get_tuning", lib().upf_get_tuning(self.h, C.byref(t)))
+        return t
 
     def state(self):
         n = self.n
@@ -353,6 +364,11 @@ class Batch:
         s = C.c_void_p()
         check("upf_batch_sim", lib().upf_batch_sim(self.h, C.byref(s)))
         return s
+
+    def set_tuning(self, tuning):
+        """Launch shapes (before the first graph capture, which bakes them in)."""
+        t = tuning if isinstance(tuning, Tuning) else Tuning.from_dict(tuning)
+        check("upf_set_tuning", lib().upf_set_tuning(self.sim_handle(), C.byref(t)))
 
     def close(self):
         if self.h:

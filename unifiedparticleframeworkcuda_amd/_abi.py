@@ -133,6 +133,22 @@ class Body(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    """upf_tuning: launch shapes that change speed only."""
+    _fields_ = [("loop_block", C.c_int), ("grid_cap", C.c_int), ("nbr_rounds", C.c_int),
+                ("nbr_stage", C.c_int), ("speculate", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for k, _ in cls._fields_:
+            setattr(t, k, int(d[k]))
+        return t
+
+
 def ptr(a, ctype=C.c_float):
     """Pointer to a numpy array's data (None passes NULL)."""
     if a is None:

@@ -1,0 +1,110 @@
+"""Autotuner of the launch shapes (upf_set_tuning): block size and grid cap
+of the grid-stride passes, candidates per lane and LDS staging depth of the
+neighbour-list build, and how far the host loop enqueues PCISPH iterations
+ahead.  None of them changes results (every per-particle sum keeps its
+order), so candidates are compared on the same steps from the same initial
+state.  The best choice is kept per (workload, particle count) in a JSON
+cache: $UPF_TUNING_CACHE, else ~/.cache/upf/tuning.json.
+"""
+import ctypes as C
+import json
+import os
+import time
+
+from . import UPF_PCISPH_STATIC
+from ._abi import Tuning
+
+DEFAULT = dict(loop_block=256, grid_cap=8192, nbr_rounds=4, nbr_stage=64, speculate=4)
+CANDIDATES = dict(loop_block=(64, 128, 256), grid_cap=(2048, 4096, 8192, 16384), nbr_rounds=(2, 4, 8),
+                  nbr_stage=(32, 64), speculate=(2, 4, 6))
+
+
+def cache_path():
+    return os.environ.get("UPF_TUNING_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "upf", "tuning.json"))
+
+
+def cache_key(workload, n):
+    return "%s:%d" % (workload, int(n))
+
+
+def load(workload, n, path=None):
+    """The cached tuning (dict) for (workload, n), or None."""
+    path = path or cache_path()
+    try:
+        with open(path) as f:
+            entry = json.load(f).get(cache_key(workload, n))
+    except (OSError, ValueError):
+        return None
+    return None if entry is None else dict(entry["tuning"])
+
+
+def store(workload, n, tuning, ms_per_step, trials, path=None):
+    path = path or cache_path()
+    try:
+        with open(path) as f:
+            data = json.load(f)
+    except (OSError, ValueError):
+        data = {}
+    data[cache_key(workload, n)] = {"tuning": dict(tuning), "ms_per_step": ms_per_step, "trials": trials}
+    d = os.path.dirname(path)
+    if d:
+        os.makedirs(d, exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def coordinate_search(measure, start=None, candidates=None, sweeps=1):
+    """Minimise measure(tuning dict) -> seconds one parameter at a time,
+    starting from `start`; returns (best dict, best time, number of trials)."""
+    cand = candidates or CANDIDATES
+    best = dict(start or DEFAULT)
+    seen = {}
+
+    def run(t):
+        k = tuple(sorted(t.items()))
+        if k not in seen:
+            seen[k] = measure(dict(t))
+        return seen[k]
+
+    best_t = run(best)
+    for _ in range(sweeps):
+        for name, values in cand.items():
+            for v in values:
+                t = dict(best)
+                t[name] = v
+                tt = run(t)
+                if tt < best_t:
+                    best, best_t = t, tt
+    return best, best_t, len(seen)
+
+
+def tune_simulation(sim, pos, types, workload, steps=4, warmup=2, method=UPF_PCISPH_STATIC, use_cache=True,
+                    path=None):
+    """Tunes `sim` (a Simulation) on the scene (pos, types) and applies the
+    result; the scene is uploaded afresh for every trial, so the caller
+    re-uploads its own state afterwards.  Returns (tuning dict, ms/step)."""
+    n = len(types)
+    if use_cache:
+        t = load(workload, n, path)
+        if t is not None:
+            sim.set_tuning(Tuning.from_dict(t))
+            return t, None
+    hip = C.CDLL("libamdhip64.so")
+
+    def measure(t):
+        sim.set_tuning(Tuning.from_dict(t))
+        sim.upload(pos, types)
+        sim.step(warmup, method)
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        sim.step(steps, method)
+        hip.hipDeviceSynchronize()
+        return time.perf_counter() - t0
+
+    best, sec, trials = coordinate_search(measure)
+    sim.set_tuning(Tuning.from_dict(best))
+    ms = sec / steps * 1e3
+    store(workload, n, best, ms, trials, path)
+    return best, ms

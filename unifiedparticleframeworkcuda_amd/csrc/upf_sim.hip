@@ -194,8 +194,6 @@ __device__ inline float4 predict(const float4& x, const float4& v, const float4&
 // kernel values from it without recomputing the distance.
 constexpr int NB_SLOT_SHIFT = 9;
 constexpr uint32_t NB_BIN_MASK = 511u;
-constexpr int NB_ROUNDS = 4;              // candidates per lane per chunk
-constexpr int NB_CHUNK = 64 * NB_ROUNDS;
 
 __device__ inline uint32_t compact10(uint32_t v) {  // inverse of dspread10
     v &= 0x09249249u;
@@ -217,12 +215,19 @@ __device__ inline uint32_t lut_bin(float dist, float factor) {
 // candidates come in ascending slot order), then tests the candidates 64 at
 // a time against each particle of the cell: the candidate's cell must lie in
 // the particle's query box (the reference's BoxQuery) and its distance must be
-// below h.  Ballots keep each list in ascending order.
+// below h.  Ballots keep each list in ascending order.  ROUNDS x 64
+// candidates are held in registers per chunk.  The lists of the wave's 64
+// slots are staged in LDS (row k, one column per slot, padded against bank
+// conflicts) and leave in full 256-byte rows of the column-major layout;
+// entries past the first STAGE of a list are written directly.
+template <int ROUNDS, int STAGE>
 __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                                   uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
+    constexpr int NB_CHUNK = 64 * ROUNDS;
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
     __shared__ int pcnt[64];
+    __shared__ uint32_t stage[STAGE * 65];
     const upf_params& P = A.P;
     const float lut_factor = A.lut_factor;
     const int n = A.n;
@@ -284,12 +289,12 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
         const int M = (int)r_pref[nr];
         for (int c0 = 0; c0 < M; c0 += NB_CHUNK) {
             // this lane's candidates of the chunk
-            float4 cx[NB_ROUNDS];
-            int cj[NB_ROUNDS];
-            uint32_t ccell[NB_ROUNDS];
+            float4 cx[ROUNDS];
+            int cj[ROUNDS];
+            uint32_t ccell[ROUNDS];
             int k = 0;  // range of the lane's candidate; rises with the round
 #pragma unroll
-            for (int r = 0; r < NB_ROUNDS; ++r) {
+            for (int r = 0; r < ROUNDS; ++r) {
                 const int ci = c0 + r * 64 + lane;
                 cj[r] = -1;
                 if (ci < M) {
@@ -316,7 +321,7 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                 }
                 int cnt = pcnt[p - base];
 #pragma unroll
-                for (int r = 0; r < NB_ROUNDS; ++r) {
+                for (int r = 0; r < ROUNDS; ++r) {
                     bool pass = false;
                     float d2 = 0.0f;
                     if (cj[r] >= 0 && cj[r] != p) {
@@ -330,9 +335,9 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     const uint64_t m = __ballot(pass);
                     if (pass) {
                         const int slot = cnt + __popcll(m & lt);
-                        if (slot < cap)
-                            nbr[(size_t)slot * stride + p] =
-                                ((uint32_t)cj[r] << NB_SLOT_SHIFT) | lut_bin(sqrtf(d2), lut_factor);
+                        const uint32_t e = ((uint32_t)cj[r] << NB_SLOT_SHIFT) | lut_bin(sqrtf(d2), lut_factor);
+                        if (slot < STAGE) stage[slot * 65 + (p - base)] = e;
+                        else if (slot < cap) nbr[(size_t)slot * stride + p] = e;
                     }
                     cnt += __popcll(m);
                 }
@@ -345,11 +350,16 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
     }
     __syncthreads();
     const int i = base + lane;
+    const int cnt = i < wend ? pcnt[lane] : 0;
     if (i < wend) {
-        const int cnt = pcnt[lane];
         ncount[i] = cnt < cap ? cnt : cap;
         if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
     }
+    const int mine = min(min(cnt, cap), STAGE);
+    int rows = mine;
+    for (int o = 32; o > 0; o >>= 1) rows = max(rows, __shfl_xor(rows, o, 64));
+    for (int k = 0; k < rows; ++k)
+        if (k < mine) nbr[(size_t)k * stride + i] = stage[k * 65 + lane];
 }
 
 // CalculateExternalForcesStaticBoundariesPCISPH (:5198-5250) over the lists +
@@ -463,7 +473,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                                                    Ctl* ctl) {
     if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
     __shared__ float lw[LUT_N];
-    __shared__ float wmax[BLOCK / 64];
+    __shared__ float wmax[BLOCK / 64];  // blockDim.x <= BLOCK
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lw[k] = luts[k];
     __syncthreads();
     const upf_params& P = A.P;
@@ -542,7 +552,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
         if (uniform) {
             const float m = wave_max(mine);
             if (lane == 0)
-                atomicMax(&A.max_slots[(first * sps + ((blockIdx.x * 4 + (threadIdx.x >> 6)) % sps)) * 32],
+                atomicMax(&A.max_slots[(first * sps + ((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % sps)) * 32],
                           __float_as_uint(m));
         } else if (sc >= 0) {
             atomicMax(&A.max_slots[(sc * sps + lane % sps) * 32], __float_as_uint(mine));
@@ -554,7 +564,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
         __syncthreads();
         if (threadIdx.x == 0) {
             float b = wmax[0];
-            for (int k = 1; k < BLOCK / 64; ++k) b = fmaxf(b, wmax[k]);
+            for (int k = 1; k < (int)(blockDim.x >> 6); ++k) b = fmaxf(b, wmax[k]);
             if (b > 0.0f) atomicMax(&A.max_slots[(blockIdx.x % sps) * 32], __float_as_uint(b));
         }
     }
@@ -629,8 +639,9 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     // the loop test of iteration t, one wave per scene: it only ever raises a
     // scene's count past t, so the particle loop below reads either value
     {
-        const int waves = gridDim.x * (BLOCK / 64);
-        for (int sc = blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); sc < A.n_scenes; sc += waves)
+        const int wpb = blockDim.x >> 6;
+        const int waves = gridDim.x * wpb;
+        for (int sc = blockIdx.x * wpb + (threadIdx.x >> 6); sc < A.n_scenes; sc += waves)
             loop_test(A, sc, t, threadIdx.x & 63, ctl);
     }
     __shared__ float lpg[LUT_N];
@@ -1316,6 +1327,7 @@ struct upf_sim {
     float* d_rstat = nullptr;
     int stat_cap = 0;
     int steps_done = 0;
+    upf_tuning tune = {256, 8192, 4, 64, 4};
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
     uint32_t ncells = 0;
@@ -1368,8 +1380,12 @@ struct upf_sim {
 };
 
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
-// grid of the persistent (grid-stride) kernels: enough waves to fill the chip
-static inline dim3 grid_persist(int n) { return dim3((unsigned)std::min((n + BLOCK - 1) / BLOCK, 8192)); }
+// block and grid of the persistent (grid-stride) kernels, from the tuning
+static inline dim3 block_loop(const upf_sim* s) { return dim3((unsigned)s->tune.loop_block); }
+static inline dim3 grid_persist(const upf_sim* s, int n) {
+    const int b = s->tune.loop_block;
+    return dim3((unsigned)std::max(1, std::min((n + b - 1) / b, s->tune.grid_cap)));
+}
 
 static int check_params(const upf_params* p) {
     if (!p) return UPF_ERR_ARG;
@@ -1445,13 +1461,13 @@ static int err_code(uint32_t bits) {
 
 template <int MODE>
 static void enqueue_iteration(upf_sim* s, const StepArgs& A, int t) {
-    const dim3 g = grid_persist(s->n);
+    const dim3 g = grid_persist(s, s->n), b = block_loop(s);
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_DENSITY, s->st);
-    hipLaunchKernelGGL(k_density<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    hipLaunchKernelGGL(k_density<MODE>, g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
     s->prof.begin(PC_FORCE, s->st);
-    hipLaunchKernelGGL(k_force<MODE>, g, dim3(BLOCK), 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    hipLaunchKernelGGL(k_force<MODE>, g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
 }
 
@@ -1466,7 +1482,8 @@ static int pcisph_loop(upf_sim* s, const StepArgs& A) {
     // a skipped iteration costs three near-empty launches, a read-back round
     // trip about as much as three of them: speculate a few iterations ahead
     const int prev = s->k_hist.empty() ? lo : s->k_hist.back();
-    const int ahead = std::min(std::min(std::max(lo, prev), lo + 4), maxl);
+    const int spec = std::max(1, s->tune.speculate);
+    const int ahead = std::min(std::min(std::max(lo, prev), lo + spec), maxl);
     hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(256), 0, s->st, A, s->ctl);
     int t = 0;
     for (; t < ahead; ++t) enqueue_iteration<MODE>(s, A, t);
@@ -1475,7 +1492,7 @@ static int pcisph_loop(upf_sim* s, const StepArgs& A) {
         if (rc) return rc;
         const int want = s->h_ctl->want_any;  // final once want <= enqueued iterations
         if (want <= t || (s->h_ctl->err & (1u << UPF_ERR_NAN))) break;
-        const int stop = std::min(t + 4, maxl);
+        const int stop = std::min(t + spec, maxl);
         for (; t < stop; ++t) enqueue_iteration<MODE>(s, A, t);
     }
     s->k_hist.push_back(s->h_ctl->want_any);
@@ -1496,8 +1513,21 @@ static void pcisph_loop_device(upf_sim* s, const StepArgs& A) {
 static void build_lists(upf_sim* s, const StepArgs& A) {
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_NBR_BUILD, s->st);
-    hipLaunchKernelGGL(k_nbr_build, dim3((unsigned)((s->n + 63) / 64)), dim3(64), 0, s->st, A, S.pos,
-                       s->W.keys_sorted, s->W.cells, s->W.nbr, s->W.ncount, s->ctl);
+    const dim3 g((unsigned)((s->n + 63) / 64));
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, g, dim3(64), 0, s->st, A, S.pos, s->W.keys_sorted, s->W.cells, s->W.nbr, s->W.ncount,
+                           s->ctl);
+    };
+    const int r = s->tune.nbr_rounds;
+    if (s->tune.nbr_stage == 32) {
+        if (r == 2) launch(k_nbr_build<2, 32>);
+        else if (r == 8) launch(k_nbr_build<8, 32>);
+        else launch(k_nbr_build<4, 32>);
+    } else {
+        if (r == 2) launch(k_nbr_build<2, 64>);
+        else if (r == 8) launch(k_nbr_build<8, 64>);
+        else launch(k_nbr_build<4, 64>);
+    }
     s->prof.end(s->st);
 }
 
@@ -1511,7 +1541,7 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
     const StepArgs A = s->args();
     Slots& S = s->S[s->cur];
     const dim3 g = grid_for(s->n);
-    const dim3 gp = grid_persist(s->n);
+    const dim3 gp = grid_persist(s, s->n), bl = block_loop(s);
     const bool akinci = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI;
     if (!s->psi_ready) {
         s->prof.begin(PC_PSI, s->st);
@@ -1522,9 +1552,9 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
     build_lists(s, A);
     s->prof.begin(PC_NBR_EXT, s->st);
     if (akinci)
-        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, gp, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
     else
-        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, gp, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
     if (device_loop) {
         if (akinci) pcisph_loop_device<UPF_BOUNDARY_AKINCI>(s, A);
@@ -1534,7 +1564,7 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
         if (rc) return rc;
     }
     s->prof.begin(PC_INTEGRATE, s->st);
-    hipLaunchKernelGGL(k_integrate, gp, dim3(BLOCK), 0, s->st, A, S, s->W, 1, 0, s->rigid ? 1 : 0, s->ctl);
+    hipLaunchKernelGGL(k_integrate, gp, bl, 0, s->st, A, S, s->W, 1, 0, s->rigid ? 1 : 0, s->ctl);
     s->prof.end(s->st);
     if (s->d_kstat)
         hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, s->st, A, s->ctl, s->d_kstat, s->d_rstat, s->stat_cap);
@@ -1574,7 +1604,7 @@ static int step_pcisph_rigid(upf_sim* s) {
                                                             : pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
     if (rc) return rc;
     s->prof.begin(PC_INTEGRATE, s->st);
-    hipLaunchKernelGGL(k_integrate, grid_persist(s->n), dim3(BLOCK), 0, s->st, A, S, s->W, 1, 1, 1, s->ctl);
+    hipLaunchKernelGGL(k_integrate, grid_persist(s, s->n), block_loop(s), 0, s->st, A, S, s->W, 1, 1, 1, s->ctl);
     s->prof.end(s->st);
     // rigid bodies: members grouped by body in slot order, one thread per body
     s->prof.begin(PC_RIGID, s->st);
@@ -1604,8 +1634,8 @@ static int step_wcsph_static(upf_sim* s) {
     hipLaunchKernelGGL(k_force_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
     s->prof.begin(PC_INTEGRATE, s->st);
-    hipLaunchKernelGGL(k_integrate, grid_persist(s->n), dim3(BLOCK), 0, s->st, A, S, s->W, 0, 0, s->rigid ? 1 : 0,
-                       s->ctl);
+    hipLaunchKernelGGL(k_integrate, grid_persist(s, s->n), block_loop(s), 0, s->st, A, S, s->W, 0, 0,
+                       s->rigid ? 1 : 0, s->ctl);
     s->prof.end(s->st);
     HIPCHK(hipGetLastError());
     rc = read_ctl(s);
@@ -1811,6 +1841,22 @@ int upf_step(upf_sim* s, int nsteps, int method) {
     }
     HIPCHK(hipStreamSynchronize(s->st));
     s->prof.collect();
+    return UPF_OK;
+}
+
+int upf_set_tuning(upf_sim* s, const upf_tuning* t) {
+    if (!s || !t) return UPF_ERR_ARG;
+    if (t->loop_block != 64 && t->loop_block != 128 && t->loop_block != 256) return UPF_ERR_ARG;
+    if (t->grid_cap < 1 || (t->nbr_rounds != 2 && t->nbr_rounds != 4 && t->nbr_rounds != 8) ||
+        (t->nbr_stage != 32 && t->nbr_stage != 64) || t->speculate < 1 || t->speculate > 64)
+        return UPF_ERR_ARG;
+    s->tune = *t;
+    return UPF_OK;
+}
+
+int upf_get_tuning(const upf_sim* s, upf_tuning* t) {
+    if (!s || !t) return UPF_ERR_ARG;
+    *t = s->tune;
     return UPF_OK;
 }
 
