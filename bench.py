@@ -140,7 +140,15 @@ def run_batch(args):
     n = sum(len(sc[2]) for sc in scenes)
     K = args.batch_steps
     out = dict(scenes=len(scenes), fluid=nf, particles=n, steps=K)
-    b = upf.Batch.from_scenes(scenes, max_record_steps=2 * K + 8)
+    b = upf.Batch.from_scenes(scenes, max_record_steps=2 * K + 64)
+    if not args.no_tune:
+        from unifiedparticleframeworkcuda_amd import autotune
+        allpos = np.concatenate([sc[1] for sc in scenes])
+        alltyp = np.concatenate([sc[2] for sc in scenes])
+        bt, _ = autotune.tune_batch(b, allpos, alltyp, "batch_%d_scenes_seed0" % len(scenes),
+                                    use_cache=not args.retune)
+        out["tuning"] = bt
+        b.upload(allpos, alltyp)
     b.step(2, use_graph=True)  # eager first step (boundary volumes), graph capture + one replay
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
@@ -199,6 +207,62 @@ def run_adjoint(args):
                 forward_updates_per_s=int(f.sum()) * 64 / (fwd / 1e3))
 
 
+def _timed_steps(sim, steps, method):
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    sim.step(steps, method)
+    hip.hipDeviceSynchronize()
+    return time.perf_counter() - t0
+
+
+def run_wcsph(args):
+    """The WCSPH variant on the same 1M inputs: Tait EOS, stiffness 1000,
+    gamma 7, at the CFL step 0.4 h / c = 0.253 ms; the 1 ms step of the
+    PCISPH runs is also tried (it violates the CFL condition)."""
+    import unifiedparticleframeworkcuda_amd as upf
+    nx, ny, nz = args.lattice
+    out = {}
+    for dt, steps in ((0.000253, 20), (0.001, 20)):
+        p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz, dt=dt)
+        p.gas_const_wcsph = 1000.0
+        sim = upf.Simulation(p, len(typ))
+        sim.upload(pos, typ)
+        key = "dt_%gms" % (dt * 1e3)
+        try:
+            sim.step(3, upf.UPF_WCSPH_STATIC)
+            sec = _timed_steps(sim, steps, upf.UPF_WCSPH_STATIC)
+            st = sim.state()
+            f = typ == upf.UPF_LIQUID
+            out[key] = {"updates_per_s": nf * steps / sec, "ms_per_step": sec / steps * 1e3, "stable": True,
+                        "max_speed": float(np.max(np.linalg.norm(st["vel"][f, :3], axis=1))),
+                        "max_density_error_pct": float(max(0.0, (np.max(st["rho"][f]) - 1000.0) / 10.0))}
+        except upf.UPFError as e:
+            out[key] = {"stable": False, "error": str(e)}
+        sim.close()
+    return out
+
+
+def run_rigid(args):
+    """Rigid-fluid coupling: 524,288 fluid particles (64 x 128 x 64 block),
+    64 seeded boxes and spheres sampled at half spacing, Akinci boundary
+    volumes, surface tension 0.05, adhesion 0.01."""
+    import unifiedparticleframeworkcuda_amd as upf
+    p, pos, typ, bid, rel, bodies, counts = upf.standard_rigid_scene(64, 128, 64, n_bodies=64)
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    sim.upload_bodies(bodies, bid, rel)
+    sim.step(3, upf.UPF_PCISPH_RIGID)
+    steps = 10
+    sec = _timed_steps(sim, steps, upf.UPF_PCISPH_RIGID)
+    K, rmax = sim.stats()
+    sim.close()
+    nf, nr, nw = counts
+    return {"fluid": nf, "rigid_particles": nr, "wall": nw, "bodies": 64, "steps": steps,
+            "updates_per_s": nf * steps / sec, "ms_per_step": sec / steps * 1e3,
+            "iterations_mean": float(np.mean(K[-steps:]))}
+
+
 def cpu_baseline(args, budget_s):
     """The oracle (OpenMP C++ port of the reference CPU path) on the same
     workload: steps until the time budget is used (at least one)."""
@@ -235,6 +299,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-wcsph", action="store_true")
+    ap.add_argument("--no-rigid", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -292,6 +358,10 @@ def main():
         out["batched"] = run_batch(args)
     if not args.no_adjoint:
         out["adjoint"] = run_adjoint(args)
+    if not args.no_wcsph:
+        out["wcsph"] = run_wcsph(args)
+    if not args.no_rigid:
+        out["rigid"] = run_rigid(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, args.cpu_budget)
     print(json.dumps(out))

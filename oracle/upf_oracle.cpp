@@ -399,6 +399,7 @@ template <class T> class Sim {
                 }
             }
             p.force.y -= (T)P.initial_mass * (T)P.gravity;
+            if (P.add_boundary_force) p.force += box_force_static(p.pos);
         }
     }
 
@@ -442,6 +443,24 @@ template <class T> class Sim {
                 f += e * (T)((x[a] + fd - mx) * inv * 2.0 * fmax);
             }
         }
+        return f;
+    }
+
+    // BoxBoundaryForce (:3794-3842): the optional wall force of the static
+    // drivers (addBoundaryForce); twice as strong at the floor as elsewhere
+    V3<T> box_force_static(const V3<T>& x) {
+        V3<T> f;
+        const T fd = (T)P.force_distance;
+        const T inv = (T)(1.0 / (double)P.force_distance);
+        const T F = (T)P.max_boundary_force;
+        const T mn[3] = {(T)P.collision_min[0], (T)P.collision_min[1], (T)P.collision_min[2]};
+        const T mx[3] = {(T)P.collision_max[0], (T)P.collision_max[1], (T)P.collision_max[2]};
+        if (x.y < mn[1] + fd) f.y += ((mn[1] + fd - x.y) * inv * T(2)) * F;
+        else if (x.y > mx[1] - fd) f.y += ((x.y - (mx[1] - fd)) * inv) * F * T(-1);
+        if (x.x < mn[0] + fd) f.x += ((mn[0] + fd - x.x) * inv) * F;
+        if (x.x > mx[0] - fd) f.x += ((x.x - (mx[0] - fd)) * inv) * F * T(-1);
+        if (x.z < mn[2] + fd) f.z += ((mn[2] + fd - x.z) * inv) * F;
+        if (x.z > mx[2] - fd) f.z += ((x.z - (mx[2] - fd)) * inv) * F * T(-1);
         return f;
     }
 
@@ -693,7 +712,7 @@ template <class T> class Sim {
                 }
             }
         }
-        if (P.add_boundary_force) force += box_boundary_force(p.pos);
+        if (P.add_boundary_force) force += box_force_static(p.pos);
         p.force = force;
         p.force.y -= (T)P.initial_mass * (T)P.gravity;
     }

@@ -124,6 +124,20 @@ def test_pcisph_one_step_bit_exact(gpu, mode):
     assert np.array_equal(kg, ko) and np.array_equal(rg, ro)
 
 
+def test_pcisph_steps_with_box_boundary_force_bit_exact(gpu):
+    """addBoundaryForce (BoxBoundaryForce, src/cpu/UnifiedPhysics.cpp:3794-3842) on."""
+    p, pos, typ, nf = scenes.dam_break(16)
+    p.add_boundary_force = 1
+    f = typ == upf.UPF_LIQUID
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    sim.step(4, upf.UPF_PCISPH_STATIC)
+    o = ol.OracleSim(p, pos, None, typ)
+    o.step(4, upf.UPF_PCISPH_STATIC)
+    _compare(sim, o, f)
+    assert np.array_equal(sim.stats()[0], o.stats()[0])
+
+
 def test_pcisph_config0_steps_bit_exact(gpu):
     """Config 0 (32^3 fluid + walls), 10 steps: state, iteration counts and
     max densities identical to the oracle."""
@@ -147,8 +161,10 @@ def test_pcisph_config0_steps_bit_exact(gpu):
     assert np.array_equal(keys, st["keys"]) and np.array_equal(ids, st["ids"])
 
 
-def test_wcsph_steps(gpu):
+@pytest.mark.parametrize("box_force", [0, 1])
+def test_wcsph_steps(gpu, box_force):
     p, pos, typ, nf = scenes.dam_break(16)
+    p.add_boundary_force = box_force
     p.gas_const_wcsph = 1000.0
     p.dt = 0.000253
     f = typ == upf.UPF_LIQUID

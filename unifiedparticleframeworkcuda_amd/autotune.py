@@ -108,3 +108,35 @@ def tune_simulation(sim, pos, types, workload, steps=4, warmup=2, method=UPF_PCI
     ms = sec / steps * 1e3
     store(workload, n, best, ms, trials, path)
     return best, ms
+
+
+def tune_batch(batch, pos, types, workload, steps=3, use_graph=True, use_cache=True, path=None):
+    """Tunes a Batch on its scenes (pos, types concatenated); every trial
+    re-uploads them, runs the eager first step and a graph capture, then
+    times `steps` graph-replayed steps.  The caller re-uploads afterwards."""
+    n = len(types)
+    cand = dict(CANDIDATES)
+    if use_graph:
+        cand.pop("speculate")  # the graph never speculates
+    if use_cache:
+        t = load(workload, n, path)
+        if t is not None:
+            batch.set_tuning(Tuning.from_dict(t))
+            return t, None
+    hip = C.CDLL("libamdhip64.so")
+
+    def measure(t):
+        batch.set_tuning(Tuning.from_dict(t))
+        batch.upload(pos, types)
+        batch.step(2, use_graph=use_graph)
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        batch.step(steps, use_graph=use_graph)
+        hip.hipDeviceSynchronize()
+        return time.perf_counter() - t0
+
+    best, sec, trials = coordinate_search(measure, candidates=cand)
+    batch.set_tuning(Tuning.from_dict(best))
+    ms = sec / steps * 1e3
+    store(workload, n, best, ms, trials, path)
+    return best, ms

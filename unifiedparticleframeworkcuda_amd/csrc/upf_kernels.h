@@ -12,6 +12,7 @@ namespace upf {
 
 constexpr int LUT_N = 301;
 constexpr int BLOCK = 256;
+constexpr int NB_ESC = 27;     // escapes per delta-coded list at most (one per gap between stencil cells)
 
 // Loop control of the PCISPH correction loop, kept on the device.
 constexpr int MAX_SLOTS = 64;  // the max reduction spreads its atomics over 64 cache lines
@@ -69,6 +70,12 @@ struct Work {           // per-step arrays, slot order
     float* pres;        // WCSPH pressure
     uint32_t* nbr;      // neighbour lists, column-major: nbr[k * stride + slot]
     int* ncount;
+    // the same lists for the correction loop, delta-coded: first neighbour
+    // slot in nbf, then 16-bit slot increments (0 = escape: the increment is
+    // the particle's next entry of nbe)
+    uint32_t* nbf;
+    uint16_t* nbw;
+    uint32_t* nbe;
     uint2* cells;       // dense cell table indexed by cell Morton code: [first, end)
 };
 

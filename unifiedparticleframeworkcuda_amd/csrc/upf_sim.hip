@@ -167,6 +167,25 @@ __global__ void k_psi(StepArgs A, const float4* __restrict__ pos, const uint2* _
 }
 
 
+// BoxBoundaryForce (src/cpu/UnifiedPhysics.cpp:3794-3842): the optional wall
+// force of the static drivers (addBoundaryForce), twice as strong at the floor
+__device__ inline void box_force_static(const upf_params& P, const Dom& d, const float4& x, float& fx, float& fy,
+                                        float& fz) {
+    const float fd = P.force_distance;
+    const float inv = (float)(1.0 / (double)P.force_distance);
+    const float F = P.max_boundary_force;
+    float bx = 0.0f, by = 0.0f, bz = 0.0f;
+    if (x.y < d.cmin[1] + fd) by = by + ((d.cmin[1] + fd - x.y) * inv * 2.0f) * F;
+    else if (x.y > d.cmax[1] - fd) by = by + ((x.y - (d.cmax[1] - fd)) * inv) * F * -1.0f;
+    if (x.x < d.cmin[0] + fd) bx = bx + ((d.cmin[0] + fd - x.x) * inv) * F;
+    if (x.x > d.cmax[0] - fd) bx = bx + ((x.x - (d.cmax[0] - fd)) * inv) * F * -1.0f;
+    if (x.z < d.cmin[2] + fd) bz = bz + ((d.cmin[2] + fd - x.z) * inv) * F;
+    if (x.z > d.cmax[2] - fd) bz = bz + ((x.z - (d.cmax[2] - fd)) * inv) * F * -1.0f;
+    fx = fx + bx;
+    fy = fy + by;
+    fz = fz + bz;
+}
+
 // Predicted position + CollisionHandling (src/cpu/UnifiedPhysics.cpp:5426-5454, :5739-5757)
 __device__ inline float4 predict(const float4& x, const float4& v, const float4& F, const float4& Fp,
                                  const StepArgs& A, const Dom& d, Ctl* ctl) {
@@ -208,6 +227,53 @@ __device__ inline uint32_t lut_bin(float dist, float factor) {
     const int idx = (int)(dist * factor * 300.0f);
     return idx >= 300 ? 300u : (uint32_t)idx;
 }
+
+// Delta coding of a list for the passes that re-read it every iteration:
+// the list is ascending, so consecutive slots mostly differ by less than
+// 2^16; larger jumps (between stencil cells far apart in Morton order) go to
+// a per-particle escape column.  Halves the bytes of the re-read lists.
+struct ListEncoder {
+    uint32_t* first;
+    uint16_t* w;
+    uint32_t* e;
+    size_t st;
+    uint32_t prev = 0;
+    int esc = 0;
+    __device__ ListEncoder(const Work& W, int i, size_t stride)
+        : first(W.nbf + i), w(W.nbw + i), e(W.nbe + i), st(stride) {}
+    __device__ void put(int k, uint32_t j, Ctl* ctl) {
+        if (k == 0) {
+            *first = j;
+        } else {
+            const uint32_t d = j - prev;
+            if (d <= 65535u) {
+                w[(size_t)k * st] = (uint16_t)d;
+            } else {
+                w[(size_t)k * st] = 0;
+                if (esc < NB_ESC) e[(size_t)esc * st] = d;
+                else set_err(ctl, UPF_ERR_NBR_OVERFLOW);
+                ++esc;
+            }
+        }
+        prev = j;
+    }
+};
+
+struct ListDecoder {
+    const uint16_t* w;
+    const uint32_t* e;
+    size_t st;
+    uint32_t j;
+    int esc = 0;
+    __device__ ListDecoder(const Work& W, int i, size_t stride)
+        : w(W.nbw + i), e(W.nbe + i), st(stride), j(W.nbf[i]) {}
+    __device__ uint32_t word(int k) const { return k > 0 ? (uint32_t)w[(size_t)k * st] : 0u; }
+    // slot of entry k given its word (entries decoded in order)
+    __device__ int next(int k, uint32_t wd) {
+        if (k > 0) j += wd ? wd : e[(size_t)(esc++) * st];
+        return (int)j;
+    }
+};
 
 // GetNeighbors (src/cpu/UnifiedPhysics.cpp:2242-2295) for every liquid slot,
 // one wave per 64 consecutive slots.  For each cell met in its slots the wave
@@ -430,6 +496,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
             fz = fz + (s3 * wv) * -1.0f;
         }
     };
+    ListEncoder enc(W, i, st);
     int k = 0;
     for (; k + 4 <= cnt; k += 4) {
         uint32_t e[4];
@@ -439,6 +506,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = (int)(e[u] >> NB_SLOT_SHIFT);
+            enc.put(k + u, (uint32_t)j, ctl);
             xj[u] = S.pos[j];
             vj[u] = S.vel[j];
         }
@@ -448,12 +516,15 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     for (; k < cnt; ++k) {
         const uint32_t e = L[k * st];
         const int j = (int)(e >> NB_SLOT_SHIFT);
+        enc.put(k, (uint32_t)j, ctl);
         pair(e, S.pos[j], S.vel[j]);
     }
     fy = fy - P.initial_mass * P.gravity;
+    const Dom d = dom_at(A, i);
+    if (P.add_boundary_force) box_force_static(P, d, xi, fx, fy, fz);
     const float4 F = make_float4(fx, fy, fz, 0.0f);
     W.force[i] = F;
-    W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, dom_at(A, i), ctl);
+    W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, d, ctl);
     }
 }
 
@@ -493,16 +564,19 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 const float T = P.support_radius_sq_limit;
                 float rho = P.kernel_self * m;
                 const int cnt = W.ncount[i];
-                const uint32_t* L = W.nbr + i;
                 const size_t st = (size_t)A.stride;
+                ListDecoder dec(W, i, st);
                 int k = 0;
                 // four neighbours per round: their loads are issued together,
                 // the sum still runs in list order
                 for (; k + 4 <= cnt; k += 4) {
                     int j[4];
+                    uint32_t wd[4];
                     float4 xj[4];
     #pragma unroll
-                    for (int u = 0; u < 4; ++u) j[u] = (int)(L[(k + u) * st] >> NB_SLOT_SHIFT);
+                    for (int u = 0; u < 4; ++u) wd[u] = dec.word(k + u);
+    #pragma unroll
+                    for (int u = 0; u < 4; ++u) j[u] = dec.next(k + u, wd[u]);
     #pragma unroll
                     for (int u = 0; u < 4; ++u) xj[u] = W.xs[j[u]];
     #pragma unroll
@@ -518,7 +592,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                     }
                 }
                 for (; k < cnt; ++k) {
-                    const int j = (int)(L[k * st] >> NB_SLOT_SHIFT);
+                    const int j = dec.next(k, dec.word(k));
                     const float4 xj = W.xs[j];
                     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                     const float d2 = dx * dx + dy * dy + dz * dz;
@@ -659,42 +733,44 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     const float pi_d = W.p[i] / A.dens_sq;
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
-    const uint32_t* L = W.nbr + i;
     const size_t st = (size_t)A.stride;
+    ListDecoder dec(W, i, st);
     // every listed pair is closer than h at these (current) positions, so the
-    // reference's `dist < h` test holds; the table bin comes with the entry
-    auto pair = [&](uint32_t e, const float4& xj, float pdj) __attribute__((always_inline)) {
+    // reference's `dist < h` test holds; the table bin is the one the list
+    // build computed from the same positions
+    auto pair = [&](int j, const float4& xj, float pdj) __attribute__((always_inline)) {
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float g = lpg[e & NB_BIN_MASK];
+        const float g = lpg[lut_bin(sqrtf(dx * dx + dy * dy + dz * dz), A.lut_factor)];
         // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
         // precomputed per particle by the density pass (same division)
         const float grad = pi_d + pdj;
         float mj = m;
-        if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = rho0 * S.psi[(int)(e >> NB_SLOT_SHIFT)];
+        if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = rho0 * S.psi[j];
         fx = fx - (((dx * g) * grad) * m) * mj;
         fy = fy - (((dy * g) * grad) * m) * mj;
         fz = fz - (((dz * g) * grad) * m) * mj;
     };
     int k = 0;
     for (; k + 4 <= cnt; k += 4) {
-        uint32_t e[4];
+        uint32_t wd[4];
+        int j[4];
         float4 xj[4];
         float pd[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) e[u] = L[(k + u) * st];
+        for (int u = 0; u < 4; ++u) wd[u] = dec.word(k + u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = dec.next(k + u, wd[u]);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            const int j = (int)(e[u] >> NB_SLOT_SHIFT);
-            xj[u] = S.pos[j];
-            pd[u] = W.pdiv[j];
+            xj[u] = S.pos[j[u]];
+            pd[u] = W.pdiv[j[u]];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pair(e[u], xj[u], pd[u]);
+        for (int u = 0; u < 4; ++u) pair(j[u], xj[u], pd[u]);
     }
     for (; k < cnt; ++k) {
-        const uint32_t e = L[k * st];
-        const int j = (int)(e >> NB_SLOT_SHIFT);
-        pair(e, S.pos[j], W.pdiv[j]);
+        const int j = dec.next(k, dec.word(k));
+        pair(j, S.pos[j], W.pdiv[j]);
     }
     const float4 Fp = make_float4(fx, fy, fz, 0.0f);
     W.fp[i] = Fp;
@@ -826,6 +902,7 @@ __global__ __launch_bounds__(BLOCK) void k_force_wcsph(StepArgs A, Slots S, Work
         fy = fy - vy * pVol * nVol * P.visc_const * kv;
         fz = fz - vz * pVol * nVol * P.visc_const * kv;
     }
+    if (P.add_boundary_force) box_force_static(P, dom_at(A, i), xi, fx, fy, fz);
     fy = fy - P.initial_mass * P.gravity;
     W.force[i] = make_float4(fx, fy, fz, 0.0f);
 }
@@ -941,9 +1018,11 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
     const float own_p = MODE == UPF_BOUNDARY_CPU_REF ? S.p_prev[i] : S.p_last[i];
     const float own_rho = S.rho_last[i];
     const float own_psi = S.psi[i];
+    ListEncoder enc(W, i, (size_t)A.stride);
     for (int k = 0; k < cnt; ++k) {
         const uint32_t e = W.nbr[(size_t)k * A.stride + i];
         const int j = (int)(e >> NB_SLOT_SHIFT);
+        enc.put(k, (uint32_t)j, ctl);
         const float4 xj = S.pos[j];
         const int tj = ptype(xj);
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -1393,7 +1472,6 @@ static int check_params(const upf_params* p) {
     if (p->block_size < 2 || p->grid_dim == 0 || p->grid_dim > 512) return UPF_ERR_ARG;
     if (p->vbox_min[0] != 0.0f || p->vbox_min[1] != 0.0f || p->vbox_min[2] != 0.0f) return UPF_ERR_ARG;
     if (p->max_neighbors <= 0 || p->max_neighbors > 4096) return UPF_ERR_ARG;
-    if (p->add_boundary_force) return UPF_ERR_ARG;  // not used by the static-boundary drivers here
     return UPF_OK;
 }
 
@@ -1674,6 +1752,9 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.max_slots, nslots));
     HIPCHK(s->alloc(&W.ncount, capacity));
     HIPCHK(s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors));
+    HIPCHK(s->alloc(&W.nbw, (size_t)capacity * p->max_neighbors));
+    HIPCHK(s->alloc(&W.nbf, capacity));
+    HIPCHK(s->alloc(&W.nbe, (size_t)capacity * NB_ESC));
     HIPCHK(s->alloc(&W.cells, ncells));
     HIPCHK(s->alloc(&s->luts, 5 * LUT_N));
     HIPCHK(s->alloc(&s->ctl, 1));
@@ -2084,6 +2165,7 @@ struct upf_batch {
     std::vector<int> counts;
     std::vector<int> scene_of;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};  // one step, by the slot buffer it starts from
+    upf_tuning captured{};                          // launch shapes baked into gexec
 };
 
 // Captures one static-boundary PCISPH step starting from slot buffer `cur`
@@ -2214,7 +2296,14 @@ int upf_batch_step(upf_batch* b, int nsteps, int use_graph) {
         const int e = err_code(s->h_ctl->err);
         if (e) return e;
     }
+    if (b->gexec[0] && std::memcmp(&b->captured, &s->tune, sizeof(upf_tuning)) != 0) {
+        for (auto& g : b->gexec) {  // the tuning changed since the capture
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    }
     if (k < nsteps && !b->gexec[0]) {
+        b->captured = s->tune;
         const int cur = s->cur;
         int rc = capture_step(s, 0, &b->gexec[0]);
         if (!rc) rc = capture_step(s, 1, &b->gexec[1]);
