@@ -51,6 +51,31 @@ def class_bytes(name, n, nf, ncells):
     return table.get(name)
 
 
+# kernel symbol prefix of each profile class (for the committed PMC summary)
+CLASS_KERNEL = {"pressure_force_predict": "upf::k_force<", "predicted_density": "upf::k_density<",
+                "neighbour_lists": "upf::k_nbr_build<", "external_force_predict": "upf::k_ext_static<",
+                "integrate": "upf::k_integrate", "gather": "upf::k_gather", "radix_sort": "upf::rs_scatter"}
+PROFILE = os.path.join(ROOT, "profiles", "pcisph_dam_break_1m_kernels.md")
+
+
+def measured_traffic(cls):
+    """HBM bytes per launch of the class's kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size),
+    or None when the summary has no row for it."""
+    pre = CLASS_KERNEL.get(cls)
+    if pre is None or not os.path.exists(PROFILE):
+        return None
+    in_table = False
+    for line in open(PROFILE):
+        if line.startswith("| kernel | dispatches"):
+            in_table = True
+            continue
+        if in_table and line.startswith("| " + pre):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            return float(cells[4]) * 1e6
+    return None
+
+
 def run_gpu(args):
     import unifiedparticleframeworkcuda_amd as upf
     nx, ny, nz = args.lattice
@@ -324,7 +349,8 @@ def main():
     b = class_bytes(name, n, nf, ncells)
     roof = dict(bound="hbm", kernel=name, launches=cnt, avg_us=avg_s * 1e6,
                 achieved=(b / avg_s / 1e9) if b else None, peak=HBM_PEAK / 1e9, unit="GB/s",
-                frac=(b / avg_s / HBM_PEAK) if b else None, traffic=None)
+                frac=(b / avg_s / HBM_PEAK) if b else None, traffic=measured_traffic(name),
+                algorithmic_bytes=b)
     out = {
         "metric": "PCISPH particle-updates/s @1M; batched vs plain at <=1% density error",
         "value": value,

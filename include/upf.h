@@ -267,6 +267,13 @@ int upf_sort_and_bin(upf_sim* s);
 int upf_download_sorted(upf_sim* s, uint32_t* keys, int32_t* ids,
                         uint32_t* cells, int* n_cells);
 
+/* GetNeighbors (src/cpu/UnifiedPhysics.cpp:2242-2295) of every liquid particle
+ * on the current state, after sorting it as upf_sort_and_bin does: counts
+ * per sorted slot and the lists (neighbour slots, ascending) concatenated
+ * into flat (up to cap entries).  Returns the number of entries, or a
+ * negative UPF_ERR_* code.                                                 */
+int upf_download_neighbors(upf_sim* s, int* counts, int32_t* flat, int cap);
+
 /* Device helpers used by the parity tests: keys of arbitrary positions and
  * a stable sort of arbitrary (key) arrays, both fully on the device.       */
 int upf_keys(const upf_params* p, int n, const float* pos4, uint32_t* keys);

@@ -138,6 +138,30 @@ def test_pcisph_steps_with_box_boundary_force_bit_exact(gpu):
     assert np.array_equal(sim.stats()[0], o.stats()[0])
 
 
+@pytest.mark.parametrize("steps", [0, 3])
+def test_neighbour_lists_equal_oracle(gpu, steps):
+    """GetNeighbors of every liquid particle: same slots, same (ascending) order."""
+    p, pos, typ, nf = scenes.dam_break(16)
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    o = ol.OracleSim(p, pos, None, typ)
+    if steps:
+        sim.step(steps, upf.UPF_PCISPH_STATIC)
+        o.step(steps, upf.UPF_PCISPH_STATIC)
+    cg, fg = sim.neighbors()
+    co, fo = o.neighbors()
+    st = o.state()
+    liquid = np.zeros(len(typ), bool)
+    liquid[np.arange(len(typ))] = typ[st["ids"]] == upf.UPF_LIQUID  # oracle slot order
+    keys_g, ids_g, _ = sim.sorted_state()
+    assert np.array_equal(ids_g, st["ids"])  # same slot order on both sides
+    og = np.concatenate([[0], np.cumsum(cg)])
+    oo = np.concatenate([[0], np.cumsum(co)])
+    for i in np.flatnonzero(liquid):
+        assert np.array_equal(fg[og[i]:og[i + 1]], fo[oo[i]:oo[i + 1]]), i
+    assert np.all(cg[~liquid] == 0)
+
+
 def test_pcisph_config0_steps_bit_exact(gpu):
     """Config 0 (32^3 fluid + walls), 10 steps: state, iteration counts and
     max densities identical to the oracle."""

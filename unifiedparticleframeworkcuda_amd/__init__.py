@@ -55,6 +55,7 @@ def lib():
         L.upf_get_step_stats.argtypes = [vp, C.c_int, pi, pf]
         L.upf_sort_and_bin.argtypes = [vp]
         L.upf_download_sorted.argtypes = [vp, pu, pi, pu, pi]
+        L.upf_download_neighbors.argtypes = [vp, pi, pi, C.c_int]
         L.upf_keys.argtypes = [pp, C.c_int, pf, pu]
         L.upf_radix_sort.argtypes = [C.c_int, pu, pu, pi]
         L.upf_debug_box_cells.argtypes = [pp, C.c_float, C.c_float, C.c_float, pu, C.c_int]
@@ -242,6 +243,18 @@ class Simulation:
 
     def sort_and_bin(self):
         check("upf_sort_and_bin", lib().upf_sort_and_bin(self.h))
+
+    def neighbors(self, cap=None):
+        """(counts per sorted slot, concatenated lists) of the current state,
+        after sorting it (non-liquid particles have empty lists)."""
+        n = self.n
+        cap = cap or n * self.p.max_neighbors
+        counts = np.zeros(n, np.int32)
+        flat = np.zeros(cap, np.int32)
+        w = lib().upf_download_neighbors(self.h, ptr(counts, C.c_int), ptr(flat, C.c_int), cap)
+        if w < 0:
+            raise UPFError("upf_download_neighbors", -w)
+        return counts, flat[:w]
 
     def sorted_state(self):
         n = self.n

@@ -270,7 +270,12 @@ struct ListDecoder {
     __device__ uint32_t word(int k) const { return k > 0 ? (uint32_t)w[(size_t)k * st] : 0u; }
     // slot of entry k given its word (entries decoded in order)
     __device__ int next(int k, uint32_t wd) {
-        if (k > 0) j += wd ? wd : e[(size_t)(esc++) * st];
+        if (k > 0) {
+            // more escapes than NB_ESC cannot occur (one per gap between the
+            // <= 27 stencil cells); the encoder flags it, the clamp keeps the
+            // read inside the column
+            j += wd ? wd : e[(size_t)min(esc++, NB_ESC - 1) * st];
+        }
         return (int)j;
     }
 };
@@ -1978,6 +1983,30 @@ int upf_sort_and_bin(upf_sim* s) {
     HIPCHK(hipMemcpyAsync(s->W.rho, S.rho_last, (size_t)s->n * 4, hipMemcpyDeviceToDevice, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     return UPF_OK;
+}
+
+int upf_download_neighbors(upf_sim* s, int* counts, int32_t* flat, int cap) {
+    if (!s || !counts) return -UPF_ERR_ARG;
+    if (!s->uploaded || s->n_scenes != 1) return -UPF_ERR_STATE;
+    int rc = upf_sort_and_bin(s);
+    if (rc) return -rc;
+    build_lists(s, s->args());
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->st) != hipSuccess) return -UPF_ERR_HIP;
+    const int n = s->n, mx = s->P.max_neighbors;
+    std::vector<int> cnt(n);
+    std::vector<uint32_t> L((size_t)mx * s->cap);
+    if (hipMemcpy(cnt.data(), s->W.ncount, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) return -UPF_ERR_HIP;
+    if (hipMemcpy(L.data(), s->W.nbr, L.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return -UPF_ERR_HIP;
+    int w = 0;
+    for (int i = 0; i < n; ++i) {
+        counts[i] = cnt[i];
+        for (int k = 0; k < cnt[i]; ++k, ++w)
+            if (flat && w < cap) flat[w] = (int32_t)(L[(size_t)k * s->cap + i] >> NB_SLOT_SHIFT);
+    }
+    Ctl h;
+    if (hipMemcpy(&h, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost) != hipSuccess) return -UPF_ERR_HIP;
+    const int e = err_code(h.err);
+    return e ? -e : w;
 }
 
 int upf_download_sorted(upf_sim* s, uint32_t* keys, int32_t* ids, uint32_t* cells, int* n_cells) {
