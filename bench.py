@@ -85,8 +85,12 @@ def run_gpu(args):
     tuning = None
     if not args.no_tune:
         from unifiedparticleframeworkcuda_amd import autotune
+        trials = []
         tuning, tune_ms = autotune.tune_simulation(sim, pos, typ, "pcisph_dam_break_%dx%dx%d" % (nx, ny, nz),
-                                                   use_cache=not args.retune)
+                                                   use_cache=not args.retune, log=trials)
+        if args.show_trials:
+            for t, sec in trials:
+                print("tuning trial %s: %.3f ms" % (t, sec * 1e3), file=sys.stderr)
     sim.upload(pos, typ)
     sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
     L = upf.lib()
@@ -328,6 +332,7 @@ def main():
     ap.add_argument("--no-rigid", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
+    ap.add_argument("--show-trials", action="store_true", help="print every tuning trial to stderr")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a 256 MiB device copy (scale for hardware byte counters)")
     args = ap.parse_args()

@@ -342,14 +342,16 @@ int upf_adjoint_timing(upf_adjoint* a, float* forward_ms, float* backward_ms, si
 /* ------------------------------------------------------ launch tuning */
 /* Launch shapes that change speed only, never results (no reference
  * counterpart; the reference fixes them at compile time).  Defaults
- * {256, 8192, 4, 64, 4}.  A batch is tuned through its upf_batch_sim handle
- * before its first graph capture.                                          */
+ * {256, 8192, 4, 0, 4, 1, 4}.  A batch recaptures its step graph when its
+ * tuning (set through its upf_batch_sim handle) changed.                   */
 typedef struct upf_tuning {
     int loop_block;   /* threads per block of the grid-stride passes: 64, 128, 256     */
     int grid_cap;     /* blocks of the grid-stride passes at most (>= 1)               */
     int nbr_rounds;   /* list build: candidates per lane per chunk, 2, 4 or 8 (x 64)    */
-    int nbr_stage;    /* list build: entries per list staged in LDS, 32 or 64          */
+    int nbr_stage;    /* list build: entries per list staged in LDS: 0, 32 or 64       */
     int speculate;    /* host loop: PCISPH iterations enqueued past the known count    */
+    int loop_list;    /* correction loop reads: 0 packed 32-bit lists, 1 delta-coded   */
+    int loop_width;   /* correction loop: neighbours whose loads are issued together, 4 or 8 */
 } upf_tuning;
 int upf_set_tuning(upf_sim* s, const upf_tuning* t);
 int upf_get_tuning(const upf_sim* s, upf_tuning* t);

@@ -1,7 +1,7 @@
 """Autotuner of the launch shapes (upf_set_tuning): block size and grid cap
 of the grid-stride passes, candidates per lane and LDS staging depth of the
-neighbour-list build, and how far the host loop enqueues PCISPH iterations
-ahead.  None of them changes results (every per-particle sum keeps its
+neighbour-list build, how far the host loop enqueues PCISPH iterations
+ahead, and the list format and load width of the correction loop.  None of them changes results (every per-particle sum keeps its
 order), so candidates are compared on the same steps from the same initial
 state.  The best choice is kept per (workload, particle count) in a JSON
 cache: $UPF_TUNING_CACHE, else ~/.cache/upf/tuning.json.
@@ -14,9 +14,10 @@ import time
 from . import UPF_PCISPH_STATIC
 from ._abi import Tuning
 
-DEFAULT = dict(loop_block=256, grid_cap=8192, nbr_rounds=4, nbr_stage=64, speculate=4)
+DEFAULT = dict(loop_block=256, grid_cap=8192, nbr_rounds=4, nbr_stage=0, speculate=4, loop_list=1, loop_width=4)
 CANDIDATES = dict(loop_block=(64, 128, 256), grid_cap=(2048, 4096, 8192, 16384), nbr_rounds=(2, 4, 8),
-                  nbr_stage=(32, 64), speculate=(2, 4, 6))
+                  nbr_stage=(0, 32, 64), speculate=(2, 4, 6), loop_list=(0, 1),
+                  loop_width=(4, 8))
 
 
 def cache_path():
@@ -55,9 +56,10 @@ def store(workload, n, tuning, ms_per_step, trials, path=None):
     os.replace(tmp, path)
 
 
-def coordinate_search(measure, start=None, candidates=None, sweeps=1):
+def coordinate_search(measure, start=None, candidates=None, sweeps=1, log=None):
     """Minimise measure(tuning dict) -> seconds one parameter at a time,
-    starting from `start`; returns (best dict, best time, number of trials)."""
+    starting from `start`; returns (best dict, best time, number of trials).
+    `log`, a list, receives (tuning, seconds) of every trial."""
     cand = candidates or CANDIDATES
     best = dict(start or DEFAULT)
     seen = {}
@@ -66,6 +68,8 @@ def coordinate_search(measure, start=None, candidates=None, sweeps=1):
         k = tuple(sorted(t.items()))
         if k not in seen:
             seen[k] = measure(dict(t))
+            if log is not None:
+                log.append((dict(t), seen[k]))
         return seen[k]
 
     best_t = run(best)
@@ -80,10 +84,12 @@ def coordinate_search(measure, start=None, candidates=None, sweeps=1):
     return best, best_t, len(seen)
 
 
-def tune_simulation(sim, pos, types, workload, steps=4, warmup=2, method=UPF_PCISPH_STATIC, use_cache=True,
-                    path=None):
+def tune_simulation(sim, pos, types, workload, steps=4, warmup=1, settle=20, method=UPF_PCISPH_STATIC,
+                    use_cache=True, path=None, log=None):
     """Tunes `sim` (a Simulation) on the scene (pos, types) and applies the
-    result; the scene is uploaded afresh for every trial, so the caller
+    result.  The scene first runs `settle` steps (a resting start needs few
+    correction iterations, a moving fluid many more); every trial uploads that
+    settled state afresh and times `steps` steps after `warmup`, so the caller
     re-uploads its own state afterwards.  Returns (tuning dict, ms/step)."""
     n = len(types)
     if use_cache:
@@ -92,10 +98,17 @@ def tune_simulation(sim, pos, types, workload, steps=4, warmup=2, method=UPF_PCI
             sim.set_tuning(Tuning.from_dict(t))
             return t, None
     hip = C.CDLL("libamdhip64.so")
+    sim.set_tuning(Tuning.from_dict(DEFAULT))
+    sim.upload(pos, types)
+    vel = None
+    if settle > 0:
+        sim.step(settle, method)
+        st = sim.state()
+        pos, vel = st["pos"], st["vel"]
 
     def measure(t):
         sim.set_tuning(Tuning.from_dict(t))
-        sim.upload(pos, types)
+        sim.upload(pos, types, vel)
         sim.step(warmup, method)
         hip.hipDeviceSynchronize()
         t0 = time.perf_counter()
@@ -103,14 +116,14 @@ def tune_simulation(sim, pos, types, workload, steps=4, warmup=2, method=UPF_PCI
         hip.hipDeviceSynchronize()
         return time.perf_counter() - t0
 
-    best, sec, trials = coordinate_search(measure)
+    best, sec, trials = coordinate_search(measure, log=log)
     sim.set_tuning(Tuning.from_dict(best))
     ms = sec / steps * 1e3
     store(workload, n, best, ms, trials, path)
     return best, ms
 
 
-def tune_batch(batch, pos, types, workload, steps=3, use_graph=True, use_cache=True, path=None):
+def tune_batch(batch, pos, types, workload, steps=3, use_graph=True, use_cache=True, path=None, log=None):
     """Tunes a Batch on its scenes (pos, types concatenated); every trial
     re-uploads them, runs the eager first step and a graph capture, then
     times `steps` graph-replayed steps.  The caller re-uploads afterwards."""
@@ -135,7 +148,7 @@ def tune_batch(batch, pos, types, workload, steps=3, use_graph=True, use_cache=T
         hip.hipDeviceSynchronize()
         return time.perf_counter() - t0
 
-    best, sec, trials = coordinate_search(measure, candidates=cand)
+    best, sec, trials = coordinate_search(measure, candidates=cand, log=log)
     batch.set_tuning(Tuning.from_dict(best))
     ms = sec / steps * 1e3
     store(workload, n, best, ms, trials, path)

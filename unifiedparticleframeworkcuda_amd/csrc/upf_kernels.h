@@ -71,10 +71,11 @@ struct Work {           // per-step arrays, slot order
     uint32_t* nbr;      // neighbour lists, column-major: nbr[k * stride + slot]
     int* ncount;
     // the same lists for the correction loop, delta-coded: first neighbour
-    // slot in nbf, then 16-bit slot increments (0 = escape: the increment is
-    // the particle's next entry of nbe)
+    // slot in nbf, then 16-bit slot increments, eight per 16-byte word of
+    // nbw (row r = entries 8r..8r+7, column-major); increment 0 = escape:
+    // the increment is the particle's next entry of nbe
     uint32_t* nbf;
-    uint16_t* nbw;
+    uint4* nbw;
     uint32_t* nbe;
     uint2* cells;       // dense cell table indexed by cell Morton code: [first, end)
 };

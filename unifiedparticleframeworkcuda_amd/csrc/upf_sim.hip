@@ -55,6 +55,7 @@ struct StepArgs {
     uint32_t* max_slots; // per-scene max slots (x 32 uints of padding)
     int n_scenes;
     int slots_per_scene;
+    int code_lists;      // the correction loop reads delta-coded lists (the external-force pass writes them)
 };
 
 __device__ inline int scene_at(const StepArgs& A, int i) { return A.scene_of ? A.scene_of[i] : 0; }
@@ -231,46 +232,68 @@ __device__ inline uint32_t lut_bin(float dist, float factor) {
 // Delta coding of a list for the passes that re-read it every iteration:
 // the list is ascending, so consecutive slots mostly differ by less than
 // 2^16; larger jumps (between stencil cells far apart in Morton order) go to
-// a per-particle escape column.  Halves the bytes of the re-read lists.
+// a per-particle escape column.  Eight 16-bit increments share one 16-byte
+// load, so a reader fetches eight entries per memory round trip.
 struct ListEncoder {
     uint32_t* first;
-    uint16_t* w;
+    uint4* rows;
     uint32_t* e;
     size_t st;
     uint32_t prev = 0;
     int esc = 0;
+    uint64_t lo = 0, hi = 0;  // the row being filled: a 128-bit shift register of 16-bit words
+    int nw = 0, row = 0;
     __device__ ListEncoder(const Work& W, int i, size_t stride)
-        : first(W.nbf + i), w(W.nbw + i), e(W.nbe + i), st(stride) {}
+        : first(W.nbf + i), rows(W.nbw + i), e(W.nbe + i), st(stride) {}
+    __device__ void push(uint32_t w) {
+        lo = (lo >> 16) | (hi << 48);
+        hi = (hi >> 16) | ((uint64_t)w << 48);
+        if (++nw == 8) {
+            rows[(size_t)row * st] = make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+            ++row;
+            nw = 0;
+        }
+    }
     __device__ void put(int k, uint32_t j, Ctl* ctl) {
+        uint32_t w = 0;
         if (k == 0) {
             *first = j;
         } else {
             const uint32_t d = j - prev;
             if (d <= 65535u) {
-                w[(size_t)k * st] = (uint16_t)d;
+                w = d;
             } else {
-                w[(size_t)k * st] = 0;
                 if (esc < NB_ESC) e[(size_t)esc * st] = d;
                 else set_err(ctl, UPF_ERR_NBR_OVERFLOW);
                 ++esc;
             }
         }
         prev = j;
+        push(w);
+    }
+    __device__ void finish() {
+        while (nw != 0) push(0u);
     }
 };
 
 struct ListDecoder {
-    const uint16_t* w;
+    const uint4* rows;
     const uint32_t* e;
     size_t st;
     uint32_t j;
     int esc = 0;
     __device__ ListDecoder(const Work& W, int i, size_t stride)
-        : w(W.nbw + i), e(W.nbe + i), st(stride), j(W.nbf[i]) {}
-    __device__ uint32_t word(int k) const { return k > 0 ? (uint32_t)w[(size_t)k * st] : 0u; }
-    // slot of entry k given its word (entries decoded in order)
-    __device__ int next(int k, uint32_t wd) {
-        if (k > 0) {
+        : rows(W.nbw + i), e(W.nbe + i), st(stride), j(W.nbf[i]) {}
+    // the 16-byte word of entries k0 .. k0 + 7 (k0 a multiple of 8)
+    __device__ void load(int k0, uint64_t& lo, uint64_t& hi) const {
+        const uint4 r = rows[(size_t)(k0 >> 3) * st];
+        lo = (uint64_t)r.x | ((uint64_t)r.y << 32);
+        hi = (uint64_t)r.z | ((uint64_t)r.w << 32);
+    }
+    // slot of entry k0 + u (entries decoded in order)
+    __device__ int next(int k0, int u, uint64_t lo, uint64_t hi) {
+        if (k0 + u > 0) {
+            const uint32_t wd = (uint32_t)(((u < 4 ? lo : hi) >> (16 * (u & 3))) & 0xFFFFu);
             // more escapes than NB_ESC cannot occur (one per gap between the
             // <= 27 stencil cells); the encoder flags it, the clamp keeps the
             // read inside the column
@@ -279,6 +302,61 @@ struct ListDecoder {
         return (int)j;
     }
 };
+
+// Visits the list of slot i in order: load(j) for G neighbours is issued
+// together, then use(j, entry, value) runs for them in list order.  CODED:
+// the delta-coded lists (entry = 0; eight entries per list load), else the
+// packed 32-bit lists (entry = (slot << 9) | bin).
+template <int CODED, int G, class Load, class Use>
+__device__ __forceinline__ void for_each_listed(const Work& W, int i, size_t st, int cnt, Load load, Use use) {
+    using V = decltype(load(0));
+    if (CODED) {
+        ListDecoder dec(W, i, st);
+        int k0 = 0;
+        // full rows: the next row's load is issued before this row's gathers
+        uint64_t lo = 0, hi = 0;
+        if (cnt > 0) dec.load(0, lo, hi);
+        for (; k0 + 8 <= cnt; k0 += 8) {
+            uint64_t nlo = 0, nhi = 0;
+            if (k0 + 8 < cnt) dec.load(k0 + 8, nlo, nhi);
+            int j[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) j[u] = dec.next(k0, u, lo, hi);
+#pragma unroll
+            for (int h = 0; h < 8; h += G) {
+                V v[G];
+#pragma unroll
+                for (int u = 0; u < G; ++u) v[u] = load(j[h + u]);
+#pragma unroll
+                for (int u = 0; u < G; ++u) use(j[h + u], 0u, v[u]);
+            }
+            lo = nlo;
+            hi = nhi;
+        }
+        // last, partial row
+        for (int u = 0; k0 + u < cnt; ++u) {
+            const int j = dec.next(k0, u, lo, hi);
+            use(j, 0u, load(j));
+        }
+    } else {
+        const uint32_t* L = W.nbr + i;
+        int k0 = 0;
+        for (; k0 + G <= cnt; k0 += G) {
+            uint32_t e[G];
+            V v[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) e[u] = L[(size_t)(k0 + u) * st];
+#pragma unroll
+            for (int u = 0; u < G; ++u) v[u] = load((int)(e[u] >> NB_SLOT_SHIFT));
+#pragma unroll
+            for (int u = 0; u < G; ++u) use((int)(e[u] >> NB_SLOT_SHIFT), e[u], v[u]);
+        }
+        for (; k0 < cnt; ++k0) {
+            const uint32_t e = L[(size_t)k0 * st];
+            use((int)(e >> NB_SLOT_SHIFT), e, load((int)(e >> NB_SLOT_SHIFT)));
+        }
+    }
+}
 
 // GetNeighbors (src/cpu/UnifiedPhysics.cpp:2242-2295) for every liquid slot,
 // one wave per 64 consecutive slots.  For each cell met in its slots the wave
@@ -290,7 +368,8 @@ struct ListDecoder {
 // candidates are held in registers per chunk.  The lists of the wave's 64
 // slots are staged in LDS (row k, one column per slot, padded against bank
 // conflicts) and leave in full 256-byte rows of the column-major layout;
-// entries past the first STAGE of a list are written directly.
+// entries past the first STAGE of a list are written directly (STAGE 0: all,
+// which keeps the LDS free for more waves).
 template <int ROUNDS, int STAGE>
 __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
@@ -298,7 +377,7 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
     constexpr int NB_CHUNK = 64 * ROUNDS;
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
     __shared__ int pcnt[64];
-    __shared__ uint32_t stage[STAGE * 65];
+    __shared__ uint32_t stage[(STAGE > 0 ? STAGE : 1) * 65];
     const upf_params& P = A.P;
     const float lut_factor = A.lut_factor;
     const int n = A.n;
@@ -502,6 +581,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
         }
     };
     ListEncoder enc(W, i, st);
+    const bool code = A.code_lists != 0;
     int k = 0;
     for (; k + 4 <= cnt; k += 4) {
         uint32_t e[4];
@@ -511,7 +591,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = (int)(e[u] >> NB_SLOT_SHIFT);
-            enc.put(k + u, (uint32_t)j, ctl);
+            if (code) enc.put(k + u, (uint32_t)j, ctl);
             xj[u] = S.pos[j];
             vj[u] = S.vel[j];
         }
@@ -521,9 +601,10 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     for (; k < cnt; ++k) {
         const uint32_t e = L[k * st];
         const int j = (int)(e >> NB_SLOT_SHIFT);
-        enc.put(k, (uint32_t)j, ctl);
+        if (code) enc.put(k, (uint32_t)j, ctl);
         pair(e, S.pos[j], S.vel[j]);
     }
+    if (code) enc.finish();
     fy = fy - P.initial_mass * P.gravity;
     const Dom d = dom_at(A, i);
     if (P.add_boundary_force) box_force_static(P, d, xi, fx, fy, fz);
@@ -544,7 +625,7 @@ __device__ inline float wave_max(float v) {
 // (atomics spread over cache lines).  Persistent grid-stride kernel; iteration
 // t of a scene runs only while t < its want, and the kernel returns at once
 // when no scene runs iteration t.
-template <int MODE>
+template <int MODE, int CODED, int G>
 __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                    Ctl* ctl) {
     if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
@@ -570,44 +651,20 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 float rho = P.kernel_self * m;
                 const int cnt = W.ncount[i];
                 const size_t st = (size_t)A.stride;
-                ListDecoder dec(W, i, st);
-                int k = 0;
-                // four neighbours per round: their loads are issued together,
+                // G neighbours per round: their loads are issued together,
                 // the sum still runs in list order
-                for (; k + 4 <= cnt; k += 4) {
-                    int j[4];
-                    uint32_t wd[4];
-                    float4 xj[4];
-    #pragma unroll
-                    for (int u = 0; u < 4; ++u) wd[u] = dec.word(k + u);
-    #pragma unroll
-                    for (int u = 0; u < 4; ++u) j[u] = dec.next(k + u, wd[u]);
-    #pragma unroll
-                    for (int u = 0; u < 4; ++u) xj[u] = W.xs[j[u]];
-    #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const float dx = xi.x - xj[u].x, dy = xi.y - xj[u].y, dz = xi.z - xj[u].z;
+                for_each_listed<CODED, G>(
+                    W, i, st, cnt, [&](int j) { return W.xs[j]; },
+                    [&](int j, uint32_t, const float4& xj) __attribute__((always_inline)) {
+                        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                         const float d2 = dx * dx + dy * dy + dz * dz;
                         if (d2 < T) {
                             const float w = lut_at(lw, sqrtf(d2), A.lut_factor);
                             float mj = m;
-                            if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj[u]) != UPF_LIQUID) mj = P.rest_density * S.psi[j[u]];
+                            if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
                             rho = rho + w * mj;
                         }
-                    }
-                }
-                for (; k < cnt; ++k) {
-                    const int j = dec.next(k, dec.word(k));
-                    const float4 xj = W.xs[j];
-                    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    if (d2 < T) {
-                        const float w = lut_at(lw, sqrtf(d2), A.lut_factor);
-                        float mj = m;
-                        if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
-                        rho = rho + w * mj;
-                    }
-                }
+                    });
                 const float err = fmax_ref(rho - P.rest_density, 0.0f);
                 const float delta = A.scene_of ? A.doms[sc].delta : P.density_error_factor;
                 const float p = W.p[i] + fmax_ref(err * delta, 0.0f);
@@ -711,7 +768,12 @@ __global__ void k_record(StepArgs A, Ctl* ctl, int* __restrict__ k_out, float* _
 // ComputeCorrectivePressureForce (:5493-5522), then the prediction of the
 // next iteration for the same particle; the loop test of the iteration runs
 // first (the force pass does not depend on it).
-template <int MODE>
+struct PosPd {
+    float4 x;
+    float pd;
+};
+
+template <int MODE, int CODED, int G>
 __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                  Ctl* ctl) {
     if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
@@ -739,44 +801,26 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
     const size_t st = (size_t)A.stride;
-    ListDecoder dec(W, i, st);
     // every listed pair is closer than h at these (current) positions, so the
     // reference's `dist < h` test holds; the table bin is the one the list
-    // build computed from the same positions
-    auto pair = [&](int j, const float4& xj, float pdj) __attribute__((always_inline)) {
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float g = lpg[lut_bin(sqrtf(dx * dx + dy * dy + dz * dz), A.lut_factor)];
-        // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
-        // precomputed per particle by the density pass (same division)
-        const float grad = pi_d + pdj;
-        float mj = m;
-        if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = rho0 * S.psi[j];
-        fx = fx - (((dx * g) * grad) * m) * mj;
-        fy = fy - (((dy * g) * grad) * m) * mj;
-        fz = fz - (((dz * g) * grad) * m) * mj;
-    };
-    int k = 0;
-    for (; k + 4 <= cnt; k += 4) {
-        uint32_t wd[4];
-        int j[4];
-        float4 xj[4];
-        float pd[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) wd[u] = dec.word(k + u);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) j[u] = dec.next(k + u, wd[u]);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            xj[u] = S.pos[j[u]];
-            pd[u] = W.pdiv[j[u]];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) pair(j[u], xj[u], pd[u]);
-    }
-    for (; k < cnt; ++k) {
-        const int j = dec.next(k, dec.word(k));
-        pair(j, S.pos[j], W.pdiv[j]);
-    }
+    // build computed from the same positions (carried by the packed entry,
+    // recomputed by the same expression for the coded lists)
+    for_each_listed<CODED, G>(
+        W, i, st, cnt, [&](int j) { return PosPd{S.pos[j], W.pdiv[j]}; },
+        [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
+            const float4& xj = q.x;
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const uint32_t bin = CODED ? lut_bin(sqrtf(dx * dx + dy * dy + dz * dz), A.lut_factor) : (e & NB_BIN_MASK);
+            const float g = lpg[bin];
+            // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
+            // precomputed per particle by the density pass (same division)
+            const float grad = pi_d + q.pd;
+            float mj = m;
+            if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = rho0 * S.psi[j];
+            fx = fx - (((dx * g) * grad) * m) * mj;
+            fy = fy - (((dy * g) * grad) * m) * mj;
+            fz = fz - (((dz * g) * grad) * m) * mj;
+        });
     const float4 Fp = make_float4(fx, fy, fz, 0.0f);
     W.fp[i] = Fp;
     W.xs[i] = predict(xi, S.vel[i], W.force[i], Fp, A, dom_at(A, i), ctl);
@@ -1024,10 +1068,11 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
     const float own_rho = S.rho_last[i];
     const float own_psi = S.psi[i];
     ListEncoder enc(W, i, (size_t)A.stride);
+    const bool code = A.code_lists != 0;
     for (int k = 0; k < cnt; ++k) {
         const uint32_t e = W.nbr[(size_t)k * A.stride + i];
         const int j = (int)(e >> NB_SLOT_SHIFT);
-        enc.put(k, (uint32_t)j, ctl);
+        if (code) enc.put(k, (uint32_t)j, ctl);
         const float4 xj = S.pos[j];
         const int tj = ptype(xj);
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -1098,6 +1143,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
             }
         }
     }
+    if (code) enc.finish();
     if (ti == UPF_LIQUID) {
         fy = fy - P.initial_mass * P.gravity;
         // CalculateBoundaryForcePerLiquidParticleHost (:2032-2069)
@@ -1196,17 +1242,37 @@ __global__ void k_rigid_bodies(int nb, upf_body* __restrict__ bodies, const int*
     for (int k = b + 1; k <= nb; ++k) {
         if (start[k] >= 0) { s1 = start[k]; break; }
     }
-    for (int k = s0; s0 >= 0 && k < s1; ++k) {
-        const int i = (int)members[k];
-        const float4 f = force[i];
-        const float4 x = pos[i];
-        F[0] = F[0] + f.x; F[1] = F[1] + f.y; F[2] = F[2] + f.z;
-        const float r0 = x.x - B.pos[0], r1 = x.y - B.pos[1], r2 = x.z - B.pos[2];
-        Tq[0] = Tq[0] + (r1 * f.z - r2 * f.y);
-        Tq[1] = Tq[1] + (r2 * f.x - r0 * f.z);
-        Tq[2] = Tq[2] + (r0 * f.y - r1 * f.x);
-        const float gy = -rel[i].w * P.gravity;
-        F[0] = F[0] + 0.0f; F[1] = F[1] + gy; F[2] = F[2] + 0.0f;
+    // eight members per round: their loads are issued together, the sums
+    // still run in slot order
+    for (int k0 = s0; s0 >= 0 && k0 < s1; k0 += 8) {
+        const int mk = s1 - k0;
+        int ii[8];
+        float4 ff[8], xx[8];
+        float ww[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (u < mk) ii[u] = (int)members[k0 + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (u < mk) {
+                ff[u] = force[ii[u]];
+                xx[u] = pos[ii[u]];
+                ww[u] = rel[ii[u]].w;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (u >= mk) break;
+            const float4 f = ff[u];
+            const float4 x = xx[u];
+            F[0] = F[0] + f.x; F[1] = F[1] + f.y; F[2] = F[2] + f.z;
+            const float r0 = x.x - B.pos[0], r1 = x.y - B.pos[1], r2 = x.z - B.pos[2];
+            Tq[0] = Tq[0] + (r1 * f.z - r2 * f.y);
+            Tq[1] = Tq[1] + (r2 * f.x - r0 * f.z);
+            Tq[2] = Tq[2] + (r0 * f.y - r1 * f.x);
+            const float gy = -ww[u] * P.gravity;
+            F[0] = F[0] + 0.0f; F[1] = F[1] + gy; F[2] = F[2] + 0.0f;
+        }
     }
     const float dt = P.dt;
     const float terminal = B.mass * P.terminal_speed;
@@ -1411,7 +1477,7 @@ struct upf_sim {
     float* d_rstat = nullptr;
     int stat_cap = 0;
     int steps_done = 0;
-    upf_tuning tune = {256, 8192, 4, 64, 4};
+    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 4};
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
     uint32_t ncells = 0;
@@ -1454,6 +1520,7 @@ struct upf_sim {
         A.max_slots = W.max_slots;
         A.n_scenes = n_scenes;
         A.slots_per_scene = sps;
+        A.code_lists = tune.loop_list;
         return A;
     }
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -1547,10 +1614,17 @@ static void enqueue_iteration(upf_sim* s, const StepArgs& A, int t) {
     const dim3 g = grid_persist(s, s->n), b = block_loop(s);
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_DENSITY, s->st);
-    hipLaunchKernelGGL(k_density<MODE>, g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    const int lw = s->tune.loop_width, ll = s->tune.loop_list;
+    if (ll && lw == 8) hipLaunchKernelGGL((k_density<MODE, 1, 8>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    else if (ll) hipLaunchKernelGGL((k_density<MODE, 1, 4>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    else if (lw == 8) hipLaunchKernelGGL((k_density<MODE, 0, 8>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    else hipLaunchKernelGGL((k_density<MODE, 0, 4>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
     s->prof.begin(PC_FORCE, s->st);
-    hipLaunchKernelGGL(k_force<MODE>, g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    if (ll && lw == 8) hipLaunchKernelGGL((k_force<MODE, 1, 8>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    else if (ll) hipLaunchKernelGGL((k_force<MODE, 1, 4>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    else if (lw == 8) hipLaunchKernelGGL((k_force<MODE, 0, 8>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
+    else hipLaunchKernelGGL((k_force<MODE, 0, 4>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
 }
 
@@ -1602,7 +1676,11 @@ static void build_lists(upf_sim* s, const StepArgs& A) {
                            s->ctl);
     };
     const int r = s->tune.nbr_rounds;
-    if (s->tune.nbr_stage == 32) {
+    if (s->tune.nbr_stage == 0) {
+        if (r == 2) launch(k_nbr_build<2, 0>);
+        else if (r == 8) launch(k_nbr_build<8, 0>);
+        else launch(k_nbr_build<4, 0>);
+    } else if (s->tune.nbr_stage == 32) {
         if (r == 2) launch(k_nbr_build<2, 32>);
         else if (r == 8) launch(k_nbr_build<8, 32>);
         else launch(k_nbr_build<4, 32>);
@@ -1757,7 +1835,7 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.max_slots, nslots));
     HIPCHK(s->alloc(&W.ncount, capacity));
     HIPCHK(s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors));
-    HIPCHK(s->alloc(&W.nbw, (size_t)capacity * p->max_neighbors));
+    HIPCHK(s->alloc(&W.nbw, (size_t)capacity * ((p->max_neighbors + 7) / 8)));
     HIPCHK(s->alloc(&W.nbf, capacity));
     HIPCHK(s->alloc(&W.nbe, (size_t)capacity * NB_ESC));
     HIPCHK(s->alloc(&W.cells, ncells));
@@ -1934,7 +2012,8 @@ int upf_set_tuning(upf_sim* s, const upf_tuning* t) {
     if (!s || !t) return UPF_ERR_ARG;
     if (t->loop_block != 64 && t->loop_block != 128 && t->loop_block != 256) return UPF_ERR_ARG;
     if (t->grid_cap < 1 || (t->nbr_rounds != 2 && t->nbr_rounds != 4 && t->nbr_rounds != 8) ||
-        (t->nbr_stage != 32 && t->nbr_stage != 64) || t->speculate < 1 || t->speculate > 64)
+        (t->nbr_stage != 0 && t->nbr_stage != 32 && t->nbr_stage != 64) || t->speculate < 1 || t->speculate > 64 ||
+        (t->loop_list != 0 && t->loop_list != 1) || (t->loop_width != 4 && t->loop_width != 8))
         return UPF_ERR_ARG;
     s->tune = *t;
     return UPF_OK;
