@@ -342,16 +342,20 @@ int upf_adjoint_timing(upf_adjoint* a, float* forward_ms, float* backward_ms, si
 /* ------------------------------------------------------ launch tuning */
 /* Launch shapes that change speed only, never results (no reference
  * counterpart; the reference fixes them at compile time).  Defaults
- * {256, 8192, 4, 0, 4, 1, 4}.  A batch recaptures its step graph when its
+ * {256, 8192, 4, 0, 4, 1, 8, 0, 4}.  A batch recaptures its step graph when its
  * tuning (set through its upf_batch_sim handle) changed.                   */
 typedef struct upf_tuning {
     int loop_block;   /* threads per block of the grid-stride passes: 64, 128, 256     */
     int grid_cap;     /* blocks of the grid-stride passes at most (>= 1)               */
     int nbr_rounds;   /* list build: candidates per lane per chunk, 2, 4 or 8 (x 64)    */
     int nbr_stage;    /* list build: entries per list staged in LDS: 0, 32 or 64       */
-    int speculate;    /* host loop: PCISPH iterations enqueued past the known count    */
+    int speculate;    /* host loop: iterations enqueued per read-back of the decision (1 .. 64) */
     int loop_list;    /* correction loop reads: 0 packed 32-bit lists, 1 delta-coded   */
     int loop_width;   /* correction loop: neighbours whose loads are issued together, 4 or 8 */
+    int nbr_tile;     /* list build: 0 cell by cell (nbr_rounds, nbr_stage), or the tile
+                         form with 256 / 512 candidates staged in LDS at a time        */
+    int spec_first;   /* host loop: first enqueue the previous step's iteration count,
+                         at most this many past the minimum (0 .. 64)                  */
 } upf_tuning;
 int upf_set_tuning(upf_sim* s, const upf_tuning* t);
 int upf_get_tuning(const upf_sim* s, upf_tuning* t);

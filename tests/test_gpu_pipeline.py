@@ -138,11 +138,15 @@ def test_pcisph_steps_with_box_boundary_force_bit_exact(gpu):
     assert np.array_equal(sim.stats()[0], o.stats()[0])
 
 
-@pytest.mark.parametrize("steps", [0, 3])
-def test_neighbour_lists_equal_oracle(gpu, steps):
-    """GetNeighbors of every liquid particle: same slots, same (ascending) order."""
+@pytest.mark.parametrize("steps,tile", [(0, 0), (3, 0), (0, 256), (3, 512)])
+def test_neighbour_lists_equal_oracle(gpu, steps, tile):
+    """GetNeighbors of every liquid particle: same slots, same (ascending)
+    order, with the cell-by-cell and the tile form of the build."""
     p, pos, typ, nf = scenes.dam_break(16)
     sim = upf.Simulation(p, len(typ))
+    t = sim.tuning()
+    t.nbr_tile = tile
+    sim.set_tuning(t)
     sim.upload(pos, typ)
     o = ol.OracleSim(p, pos, None, typ)
     if steps:

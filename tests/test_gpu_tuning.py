@@ -33,10 +33,14 @@ def _run(p, pos, typ, tuning, steps=5):
 
 
 @pytest.mark.parametrize("tuning", [
-    dict(loop_block=64, grid_cap=16, nbr_rounds=2, nbr_stage=32, speculate=1, loop_list=0, loop_width=4),
-    dict(loop_block=128, grid_cap=2048, nbr_rounds=8, nbr_stage=32, speculate=6, loop_list=1, loop_width=8),
-    dict(loop_block=256, grid_cap=3, nbr_rounds=4, nbr_stage=64, speculate=2, loop_list=0, loop_width=8),
-    dict(loop_block=256, grid_cap=8192, nbr_rounds=8, nbr_stage=0, speculate=4, loop_list=1, loop_width=4),
+    dict(loop_block=64, grid_cap=16, nbr_rounds=2, nbr_stage=32, speculate=1, loop_list=0, loop_width=4, nbr_tile=0,
+         spec_first=0),
+    dict(loop_block=128, grid_cap=2048, nbr_rounds=8, nbr_stage=32, speculate=6, loop_list=1, loop_width=8,
+         nbr_tile=256, spec_first=64),
+    dict(loop_block=256, grid_cap=3, nbr_rounds=4, nbr_stage=64, speculate=2, loop_list=0, loop_width=8, nbr_tile=512,
+         spec_first=2),
+    dict(loop_block=256, grid_cap=8192, nbr_rounds=8, nbr_stage=0, speculate=4, loop_list=1, loop_width=4, nbr_tile=0,
+         spec_first=4),
 ])
 def test_tunings_give_identical_steps(gpu, tuning):
     p, pos, typ = _compressed_scene()
@@ -53,7 +57,8 @@ def test_invalid_tuning_rejected(gpu):
     sim = upf.Simulation(p, len(typ))
     for d in (dict(autotune.DEFAULT, loop_block=96), dict(autotune.DEFAULT, nbr_rounds=3),
               dict(autotune.DEFAULT, nbr_stage=16), dict(autotune.DEFAULT, grid_cap=0),
-              dict(autotune.DEFAULT, loop_list=2), dict(autotune.DEFAULT, loop_width=6)):
+              dict(autotune.DEFAULT, loop_list=2), dict(autotune.DEFAULT, loop_width=6),
+              dict(autotune.DEFAULT, nbr_tile=128)):
         with pytest.raises(upf.UPFError):
             sim.set_tuning(d)
     assert sim.tuning().as_dict() == autotune.DEFAULT

@@ -512,6 +512,197 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
         if (k < mine) nbr[(size_t)k * stride + i] = stage[k * 65 + lane];
 }
 
+// GetNeighbors again, tile form: one wave per 64 consecutive slots of one
+// scene.  The occupied cells of the box around the slots' cells (grown by one
+// cell) are ranked into Morton order -- the slot order -- in LDS; their
+// particles are staged in LDS CH at a time, and every lane tests all of them
+// against its own particle (query box of the reference's BoxQuery, distance
+// below h), appending to its list in ascending slot order.  No per-particle
+// serialisation; a slot range whose box exceeds TB_CELLS cells is halved
+// until it fits.
+constexpr int TB_CELLS = 128;
+
+__device__ inline uint32_t wave_min_u(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+__device__ inline uint32_t wave_max_u(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+template <int CH>
+__global__ __launch_bounds__(64) void k_nbr_build_tile(StepArgs A, const float4* __restrict__ pos,
+                                                       const uint32_t* __restrict__ keys,
+                                                       const uint2* __restrict__ cells, uint32_t* __restrict__ nbr,
+                                                       int* __restrict__ ncount, Ctl* ctl) {
+    __shared__ float4 cpos[CH];
+    __shared__ uint32_t cpk[CH], cslot[CH];
+    __shared__ uint32_t o_start[TB_CELLS], o_len[TB_CELLS], o_cell[TB_CELLS];
+    __shared__ uint32_t s_start[TB_CELLS], s_cell[TB_CELLS], s_pref[TB_CELLS + 1];
+    const upf_params& P = A.P;
+    const int lane = threadIdx.x;
+    const int base = blockIdx.x * 64;
+    if (base >= A.n) return;
+    const int wend = min(base + 64, A.n);
+    const int p = base + lane;
+    const bool valid = p < wend;
+    const float4 xp = valid ? pos[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const bool liquid = valid && ptype(xp) == UPF_LIQUID;
+    const int my_scene = valid ? scene_at(A, p) : -1;
+    const float T = P.support_radius_sq_limit;
+    const float h = P.support_radius;
+    const int cap = P.max_neighbors;
+    const size_t stride = (size_t)A.stride;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int cnt = 0;
+    int s = base;
+    while (s < wend) {
+        const int sc = scene_at(A, s);
+        const Dom d = dom_at(A, s);
+        const uint32_t shift3 = 3u * d.block_shift, G = d.grid_dim;
+        const uint2* T3 = cells + d.cell_off;
+        const uint64_t other = __ballot(valid && p >= s && my_scene != sc);
+        int se = other ? base + __ffsll((long long)other) - 1 : wend;
+        // this lane's query box in cell coordinates (its own scene's grid)
+        uint32_t blo[3] = {0u, 0u, 0u}, bhi[3] = {0u, 0u, 0u};
+        if (liquid && my_scene == sc) {
+            const float xv[3] = {xp.x, xp.y, xp.z};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float lo_f = d.scales[a] * (xv[a] - h);
+                float hi_f = d.scales[a] * (xv[a] + h);
+                lo_f = lo_f >= 0.0f ? lo_f : 0.0f;
+                hi_f = hi_f <= 1024.0f ? hi_f : 1024.0f;
+                blo[a] = ((uint32_t)(unsigned short)lo_f) >> d.block_shift;
+                bhi[a] = ((uint32_t)(unsigned short)hi_f) >> d.block_shift;
+            }
+        }
+        for (;;) {  // shrink [s, se) until its cell box fits
+            const bool in_seg = valid && p >= s && p < se;
+            const uint32_t c = in_seg ? keys[p] >> shift3 : 0u;
+            const uint32_t cx = compact10(c), cy = compact10(c >> 1), cz = compact10(c >> 2);
+            uint32_t lo[3], ext[3];
+            const uint32_t cv[3] = {cx, cy, cz};
+            uint32_t nbox = 1;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const uint32_t mn = wave_min_u(in_seg ? cv[a] : 0xFFFFFFFFu);
+                const uint32_t mx = wave_max_u(in_seg ? cv[a] : 0u);
+                lo[a] = mn > 0 ? mn - 1 : 0;
+                const uint32_t hi = mx + 1 < G ? mx + 1 : G - 1;
+                ext[a] = hi - lo[a] + 1;
+                nbox *= ext[a];
+            }
+            if (nbox > (uint32_t)TB_CELLS) {
+                if (se - s > 1) {
+                    se = s + (se - s) / 2;
+                    continue;
+                }
+                set_err(ctl, UPF_ERR_NBR_OVERFLOW);  // one cell's box holds <= 27 cells: cannot happen
+                break;
+            }
+            // occupied cells of the box (unsorted), two per lane
+            int n_occ = 0;
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const uint32_t q = (uint32_t)(lane + 64 * r);
+                bool occ = false;
+                uint2 rg = make_uint2(0u, 0u);
+                uint32_t pk = 0u;
+                if (q < nbox) {
+                    const uint32_t qx = lo[0] + q % ext[0], qy = lo[1] + (q / ext[0]) % ext[1],
+                                   qz = lo[2] + q / (ext[0] * ext[1]);
+                    rg = T3[dmorton3(qx, qy, qz)];
+                    occ = rg.y > rg.x;
+                    pk = qx | (qy << 10) | (qz << 20);
+                }
+                const uint64_t m = __ballot(occ);
+                if (occ) {
+                    const int k = n_occ + __popcll(m & lt);
+                    o_start[k] = rg.x;
+                    o_len[k] = rg.y - rg.x;
+                    o_cell[k] = pk;
+                }
+                n_occ += __popcll(m);
+            }
+            __syncthreads();
+            // rank by first slot = Morton order of the cells
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int e = lane + 64 * r;
+                if (e < n_occ) {
+                    const uint32_t st0 = o_start[e];
+                    int rank = 0;
+                    for (int f = 0; f < n_occ; ++f) rank += o_start[f] < st0;
+                    s_start[rank] = st0;
+                    s_cell[rank] = o_cell[e];
+                    s_pref[rank + 1] = o_len[e];
+                }
+            }
+            __syncthreads();
+            // prefix of the cell lengths (two per lane, wave scan)
+            {
+                const uint32_t a0 = 2 * lane < n_occ ? s_pref[2 * lane + 1] : 0u;
+                const uint32_t a1 = 2 * lane + 1 < n_occ ? s_pref[2 * lane + 2] : 0u;
+                uint32_t incl = a0 + a1;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+                    if (lane >= o) incl += y;
+                }
+                const uint32_t excl = incl - (a0 + a1);
+                __syncthreads();
+                if (2 * lane < n_occ) s_pref[2 * lane + 1] = excl + a0;
+                if (2 * lane + 1 < n_occ) s_pref[2 * lane + 2] = excl + a0 + a1;
+                if (lane == 0) s_pref[0] = 0u;
+            }
+            __syncthreads();
+            const int M = (int)s_pref[n_occ];
+            for (int c0 = 0; c0 < M; c0 += CH) {
+                const int chunk = min(CH, M - c0);
+                for (int idx = lane; idx < chunk; idx += 64) {
+                    const uint32_t g = (uint32_t)(c0 + idx);
+                    int k0 = 0, k1 = n_occ;  // last cell with s_pref[k] <= g
+                    while (k1 - k0 > 1) {
+                        const int mid = (k0 + k1) >> 1;
+                        if (s_pref[mid] <= g) k0 = mid; else k1 = mid;
+                    }
+                    const uint32_t slot = s_start[k0] + (g - s_pref[k0]);
+                    cslot[idx] = slot;
+                    cpk[idx] = s_cell[k0];
+                    cpos[idx] = pos[slot];
+                }
+                __syncthreads();
+                if (liquid && p >= s && p < se) {
+                    for (int idx = 0; idx < chunk; ++idx) {
+                        const uint32_t qc = cpk[idx];
+                        const uint32_t qx = qc & 1023u, qy = (qc >> 10) & 1023u, qz = qc >> 20;
+                        if (qx < blo[0] || qx > bhi[0] || qy < blo[1] || qy > bhi[1] || qz < blo[2] || qz > bhi[2])
+                            continue;
+                        const uint32_t j = cslot[idx];
+                        if ((int)j == p) continue;
+                        const float4 xj = cpos[idx];
+                        const float dx = xp.x - xj.x, dy = xp.y - xj.y, dz = xp.z - xj.z;
+                        const float d2 = dx * dx + dy * dy + dz * dz;
+                        if (d2 < T) {
+                            if (cnt < cap)
+                                nbr[(size_t)cnt * stride + p] = (j << NB_SLOT_SHIFT) | lut_bin(sqrtf(d2), A.lut_factor);
+                            ++cnt;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            break;
+        }
+        s = se;
+    }
+    if (valid) {
+        ncount[p] = cnt < cap ? cnt : cap;
+        if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
+    }
+}
+
 // CalculateExternalForcesStaticBoundariesPCISPH (:5198-5250) over the lists +
 // the first PredictPositionAndVelocity of the loop; every slot resets its
 // loop pressure.
@@ -1477,7 +1668,7 @@ struct upf_sim {
     float* d_rstat = nullptr;
     int stat_cap = 0;
     int steps_done = 0;
-    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 4};
+    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4};
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
     uint32_t ncells = 0;
@@ -1636,11 +1827,13 @@ template <int MODE>
 static int pcisph_loop(upf_sim* s, const StepArgs& A) {
     const int maxl = std::max(s->P.max_loops, 1);
     const int lo = std::min(std::max(s->P.min_loops, 1), maxl);
-    // a skipped iteration costs three near-empty launches, a read-back round
-    // trip about as much as three of them: speculate a few iterations ahead
+    // an iteration past the decision costs two launches that return at once,
+    // a read-back round trip several of them: enqueue as many as the previous
+    // step needed (at most spec_first past the minimum), then `speculate`
+    // more per round trip
     const int prev = s->k_hist.empty() ? lo : s->k_hist.back();
     const int spec = std::max(1, s->tune.speculate);
-    const int ahead = std::min(std::min(std::max(lo, prev), lo + spec), maxl);
+    const int ahead = std::min(std::min(std::max(lo, prev), lo + std::max(0, s->tune.spec_first)), maxl);
     hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(256), 0, s->st, A, s->ctl);
     int t = 0;
     for (; t < ahead; ++t) enqueue_iteration<MODE>(s, A, t);
@@ -1676,7 +1869,11 @@ static void build_lists(upf_sim* s, const StepArgs& A) {
                            s->ctl);
     };
     const int r = s->tune.nbr_rounds;
-    if (s->tune.nbr_stage == 0) {
+    if (s->tune.nbr_tile == 256) {
+        launch(k_nbr_build_tile<256>);
+    } else if (s->tune.nbr_tile == 512) {
+        launch(k_nbr_build_tile<512>);
+    } else if (s->tune.nbr_stage == 0) {
         if (r == 2) launch(k_nbr_build<2, 0>);
         else if (r == 8) launch(k_nbr_build<8, 0>);
         else launch(k_nbr_build<4, 0>);
@@ -2013,7 +2210,8 @@ int upf_set_tuning(upf_sim* s, const upf_tuning* t) {
     if (t->loop_block != 64 && t->loop_block != 128 && t->loop_block != 256) return UPF_ERR_ARG;
     if (t->grid_cap < 1 || (t->nbr_rounds != 2 && t->nbr_rounds != 4 && t->nbr_rounds != 8) ||
         (t->nbr_stage != 0 && t->nbr_stage != 32 && t->nbr_stage != 64) || t->speculate < 1 || t->speculate > 64 ||
-        (t->loop_list != 0 && t->loop_list != 1) || (t->loop_width != 4 && t->loop_width != 8))
+        (t->loop_list != 0 && t->loop_list != 1) || (t->loop_width != 4 && t->loop_width != 8) ||
+        (t->nbr_tile != 0 && t->nbr_tile != 256 && t->nbr_tile != 512) || t->spec_first < 0 || t->spec_first > 64)
         return UPF_ERR_ARG;
     s->tune = *t;
     return UPF_OK;
