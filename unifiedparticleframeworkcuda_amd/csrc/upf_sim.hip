@@ -364,19 +364,19 @@ __device__ __forceinline__ void for_each_listed(const Work& W, int i, size_t st,
 // candidates come in ascending slot order), then tests the candidates 64 at
 // a time against each particle of the cell: the candidate's cell must lie in
 // the particle's query box (the reference's BoxQuery) and its distance must be
-// below h.  Ballots keep each list in ascending order.  ROUNDS x 64
+// below h.  Ballots keep each list in ascending order (BIN: entries carry
+// the pair's table bin, else only the slot).  ROUNDS x 64
 // candidates are held in registers per chunk.  The lists of the wave's 64
 // slots are staged in LDS (row k, one column per slot, padded against bank
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
 // which keeps the LDS free for more waves).
-template <int ROUNDS, int STAGE>
+template <int ROUNDS, int STAGE, int BIN>
 __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                                   uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
     constexpr int NB_CHUNK = 64 * ROUNDS;
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
-    __shared__ int pcnt[64];
     __shared__ uint32_t stage[(STAGE > 0 ? STAGE : 1) * 65];
     const upf_params& P = A.P;
     const float lut_factor = A.lut_factor;
@@ -390,16 +390,26 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
     const float h = P.support_radius;
     const int cap = P.max_neighbors;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    pcnt[lane] = 0;
+    // the wave's 64 slots, one per lane: position, key, scene; the serial
+    // loops below take them from the owning lane instead of from memory
+    const int me = base + lane;
+    const bool mine_valid = me < wend;
+    const float4 my_x = mine_valid ? pos[me] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t my_key = mine_valid ? keys[me] : 0xFFFFFFFFu;
+    const int my_scene = mine_valid ? scene_at(A, me) : -1;
+    int my_cnt = 0;  // this lane's slot: list length so far
     int s = base;
     while (s < wend) {
-        const Dom d = dom_at(A, s);  // the cell's scene
+        const int sl = s - base;
+        const int sc = __shfl(my_scene, sl, 64);
+        const Dom d = A.scene_of ? A.doms[sc] : A.dom0;  // the cell's scene
         const uint32_t shift3 = 3u * d.block_shift;
         const uint32_t G = d.grid_dim;
         const uint2* T3 = cells + d.cell_off;
-        const uint32_t c = keys[s] >> shift3;
-        const uint2 cr = T3[c];
-        const int pe = min((int)cr.y, wend);
+        const uint32_t c = (uint32_t)__shfl((int)my_key, sl, 64) >> shift3;
+        // end of the cell within the wave: first later slot of another cell or scene
+        const uint64_t other = __ballot(mine_valid && me > s && ((my_key >> shift3) != c || my_scene != sc));
+        const int pe = other ? base + __ffsll((long long)other) - 1 : wend;
         const uint32_t cc[3] = {compact10(c), compact10(c >> 1), compact10(c >> 2)};
         // stencil cells in Morton order: lane e names enumeration index e
         BoxCells bc;
@@ -430,11 +440,14 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
         const uint64_t vm = __ballot(valid);
         const int nr = __popcll(vm);
         const int rank = __popcll(vm & lt);
-        if (valid) { r_start[rank] = rs; r_cell[rank] = rcell; r_pref[rank + 1] = rl; }
+        // prefix of the occupied cells' lengths in rank (= lane) order
+        uint32_t incl = valid ? rl : 0u;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        if (valid) { r_start[rank] = rs; r_cell[rank] = rcell; r_pref[rank + 1] = incl; }
         if (lane == 0) r_pref[0] = 0;
-        __syncthreads();
-        if (lane == 0)
-            for (int k = 1; k <= nr; ++k) r_pref[k] += r_pref[k - 1];
         __syncthreads();
         const int M = (int)r_pref[nr];
         for (int c0 = 0; c0 < M; c0 += NB_CHUNK) {
@@ -455,7 +468,9 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                 }
             }
             for (int p = s; p < pe; ++p) {
-                const float4 xp = pos[p];
+                const int pl = p - base;
+                const float4 xp = make_float4(__shfl(my_x.x, pl, 64), __shfl(my_x.y, pl, 64),
+                                              __shfl(my_x.z, pl, 64), __shfl(my_x.w, pl, 64));
                 if (ptype(xp) != UPF_LIQUID) continue;
                 // the particle's query box in cell coordinates
                 uint32_t blo[3], bhi[3];
@@ -469,7 +484,7 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     blo[a] = ((uint32_t)(unsigned short)lo_f) >> d.block_shift;
                     bhi[a] = ((uint32_t)(unsigned short)hi_f) >> d.block_shift;
                 }
-                int cnt = pcnt[p - base];
+                int cnt = __shfl(my_cnt, pl, 64);
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
                     bool pass = false;
@@ -485,31 +500,32 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     const uint64_t m = __ballot(pass);
                     if (pass) {
                         const int slot = cnt + __popcll(m & lt);
-                        const uint32_t e = ((uint32_t)cj[r] << NB_SLOT_SHIFT) | lut_bin(sqrtf(d2), lut_factor);
-                        if (slot < STAGE) stage[slot * 65 + (p - base)] = e;
+                        // BIN == 0: the table bin is left to the readers (the
+                        // external-force pass computes it from the same distance)
+                        const uint32_t e = ((uint32_t)cj[r] << NB_SLOT_SHIFT) | (BIN ? lut_bin(sqrtf(d2), lut_factor) : 0u);
+                        if (slot < STAGE) stage[slot * 65 + pl] = e;
                         else if (slot < cap) nbr[(size_t)slot * stride + p] = e;
                     }
                     cnt += __popcll(m);
                 }
-                if (lane == 0) pcnt[p - base] = cnt;
-                __syncthreads();
+                if (lane == pl) my_cnt = cnt;
             }
         }
-        __syncthreads();
+        __syncthreads();  // r_* are rewritten for the next cell
         s = pe;
     }
-    __syncthreads();
-    const int i = base + lane;
-    const int cnt = i < wend ? pcnt[lane] : 0;
-    if (i < wend) {
-        ncount[i] = cnt < cap ? cnt : cap;
-        if (cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
+    if (mine_valid) {
+        ncount[me] = my_cnt < cap ? my_cnt : cap;
+        if (my_cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
     }
-    const int mine = min(min(cnt, cap), STAGE);
-    int rows = mine;
-    for (int o = 32; o > 0; o >>= 1) rows = max(rows, __shfl_xor(rows, o, 64));
-    for (int k = 0; k < rows; ++k)
-        if (k < mine) nbr[(size_t)k * stride + i] = stage[k * 65 + lane];
+    if (STAGE > 0) {
+        __syncthreads();
+        const int mine = min(min(my_cnt, cap), STAGE);
+        int rows = mine;
+        for (int o = 32; o > 0; o >>= 1) rows = max(rows, __shfl_xor(rows, o, 64));
+        for (int k = 0; k < rows; ++k)
+            if (k < mine) nbr[(size_t)k * stride + me] = stage[k * 65 + lane];
+    }
 }
 
 // GetNeighbors again, tile form: one wave per 64 consecutive slots of one
@@ -738,11 +754,17 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
     const uint32_t* L = W.nbr + i;
+    const bool bin_free = A.code_lists != 0;  // the lists of this step carry no bins
     const size_t st = (size_t)A.stride;
     auto pair = [&](uint32_t e, const float4& xj, const float4& vj) __attribute__((always_inline)) {
         const int tj = ptype(xj);
         if (tj == UPF_LIQUID) {
-            const float kv = lvl[e & NB_BIN_MASK];
+            uint32_t bin = e & NB_BIN_MASK;
+            if (bin_free) {  // the list build's expression on the same positions
+                const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                bin = lut_bin(sqrtf(dx * dx + dy * dy + dz * dz), A.lut_factor);
+            }
+            const float kv = lvl[bin];
             const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
             fx = fx - vx * rv * rv * visc * kv;
             fy = fy - vy * rv * rv * visc * kv;
@@ -1669,6 +1691,7 @@ struct upf_sim {
     int stat_cap = 0;
     int steps_done = 0;
     upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4};
+    bool static_lists = false;  // the lists being built serve the static PCISPH step only
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
     uint32_t ncells = 0;
@@ -1869,22 +1892,30 @@ static void build_lists(upf_sim* s, const StepArgs& A) {
                            s->ctl);
     };
     const int r = s->tune.nbr_rounds;
+    // the correction loop of the delta-coded lists never reads the bins
+    const bool bins = !(A.code_lists && s->static_lists);
     if (s->tune.nbr_tile == 256) {
         launch(k_nbr_build_tile<256>);
     } else if (s->tune.nbr_tile == 512) {
         launch(k_nbr_build_tile<512>);
     } else if (s->tune.nbr_stage == 0) {
-        if (r == 2) launch(k_nbr_build<2, 0>);
-        else if (r == 8) launch(k_nbr_build<8, 0>);
-        else launch(k_nbr_build<4, 0>);
+        if (!bins) {
+            if (r == 2) launch(k_nbr_build<2, 0, 0>);
+            else if (r == 8) launch(k_nbr_build<8, 0, 0>);
+            else launch(k_nbr_build<4, 0, 0>);
+        } else {
+            if (r == 2) launch(k_nbr_build<2, 0, 1>);
+            else if (r == 8) launch(k_nbr_build<8, 0, 1>);
+            else launch(k_nbr_build<4, 0, 1>);
+        }
     } else if (s->tune.nbr_stage == 32) {
-        if (r == 2) launch(k_nbr_build<2, 32>);
-        else if (r == 8) launch(k_nbr_build<8, 32>);
-        else launch(k_nbr_build<4, 32>);
+        if (r == 2) launch(k_nbr_build<2, 32, 1>);
+        else if (r == 8) launch(k_nbr_build<8, 32, 1>);
+        else launch(k_nbr_build<4, 32, 1>);
     } else {
-        if (r == 2) launch(k_nbr_build<2, 64>);
-        else if (r == 8) launch(k_nbr_build<8, 64>);
-        else launch(k_nbr_build<4, 64>);
+        if (r == 2) launch(k_nbr_build<2, 64, 1>);
+        else if (r == 8) launch(k_nbr_build<8, 64, 1>);
+        else launch(k_nbr_build<4, 64, 1>);
     }
     s->prof.end(s->st);
 }
@@ -1907,7 +1938,9 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
         s->prof.end(s->st);
         s->psi_ready = true;
     }
+    s->static_lists = true;
     build_lists(s, A);
+    s->static_lists = false;
     s->prof.begin(PC_NBR_EXT, s->st);
     if (akinci)
         hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
