@@ -91,19 +91,14 @@ def run_gpu(args):
         if args.show_trials:
             for t, sec in trials:
                 print("tuning trial %s: %.3f ms" % (t, sec * 1e3), file=sys.stderr)
-    sim.upload(pos, typ)
-    sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
     L = upf.lib()
-    L.upf_profile_enable.argtypes = [C.c_void_p, C.c_int]
-    L.upf_profile_read.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
-    L.upf_profile_name.restype = C.c_char_p
-    L.upf_profile_reset.argtypes = [C.c_void_p]
-    L.upf_profile_enable(sim.h, 1)
-    L.upf_profile_reset(sim.h)
     # The library links the system HIP runtime; torch bundles its own, and the
     # two do not share one process, so the device is synchronised through the
     # library's runtime (upf_step also returns only after its stream drained).
     hip = C.CDLL("libamdhip64.so")
+    # timed run: no event recording
+    sim.upload(pos, typ)
+    sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
     sim.step(args.steps, upf.UPF_PCISPH_STATIC)
@@ -114,12 +109,18 @@ def run_gpu(args):
     rt = rmax[-args.steps:]
     keys, ids, cells = sim.sorted_state()
     ncells = len(cells)
-    prof = {}
-    for c in range(L.upf_profile_classes()):
-        ms, cnt = C.c_double(), C.c_longlong()
-        L.upf_profile_read(sim.h, c, C.byref(ms), C.byref(cnt))
-        if cnt.value:
-            prof[L.upf_profile_name(c).decode()] = (ms.value, cnt.value)
+    # the same steps again (same inputs, same iteration counts) with HIP events
+    # recorded around every launch: per-kernel-class device time
+    sim.upload(pos, typ)
+    sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
+    L.upf_profile_enable(sim.h, 1)
+    L.upf_profile_reset(sim.h)
+    sim.step(args.steps, upf.UPF_PCISPH_STATIC)
+    hip.hipDeviceSynchronize()
+    Kp, _ = sim.stats()
+    prof = upf.profile(sim.h)
+    L.upf_profile_enable(sim.h, 0)
+    assert np.array_equal(Kp[-args.steps:], Kt), "the profiled replay took other iteration counts"
     sim.close()
     out = dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof, tuning=tuning)
     if not args.no_graph:

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -472,8 +473,10 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                 const float4 xp = make_float4(__shfl(my_x.x, pl, 64), __shfl(my_x.y, pl, 64),
                                               __shfl(my_x.z, pl, 64), __shfl(my_x.w, pl, 64));
                 if (ptype(xp) != UPF_LIQUID) continue;
-                // the particle's query box in cell coordinates
-                uint32_t blo[3], bhi[3];
+                // the particle's query box in cell coordinates, packed like
+                // the candidates' cells: 10-bit fields whose top bit (coordinates
+                // are below 512) serves as the borrow guard of a packed compare
+                uint32_t blo = 0u, bhi = 0u;
                 const float xv[3] = {xp.x, xp.y, xp.z};
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
@@ -481,18 +484,23 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     float hi_f = d.scales[a] * (xv[a] + h);
                     lo_f = lo_f >= 0.0f ? lo_f : 0.0f;
                     hi_f = hi_f <= 1024.0f ? hi_f : 1024.0f;
-                    blo[a] = ((uint32_t)(unsigned short)lo_f) >> d.block_shift;
-                    bhi[a] = ((uint32_t)(unsigned short)hi_f) >> d.block_shift;
+                    const uint32_t l = ((uint32_t)(unsigned short)lo_f) >> d.block_shift;
+                    uint32_t u = ((uint32_t)(unsigned short)hi_f) >> d.block_shift;
+                    u = u < G ? u : G - 1;  // cells lie below G <= 512
+                    blo |= l << (10 * a);
+                    bhi |= u << (10 * a);
                 }
+                constexpr uint32_t GUARD = (1u << 9) | (1u << 19) | (1u << 29);
+                const uint32_t bhi_g = bhi | GUARD;
                 int cnt = __shfl(my_cnt, pl, 64);
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
                     bool pass = false;
                     float d2 = 0.0f;
                     if (cj[r] >= 0 && cj[r] != p) {
-                        const uint32_t qx = ccell[r] & 1023u, qy = (ccell[r] >> 10) & 1023u, qz = ccell[r] >> 20;
-                        const bool inbox = qx >= blo[0] && qx <= bhi[0] && qy >= blo[1] && qy <= bhi[1] &&
-                                           qz >= blo[2] && qz <= bhi[2];
+                        // blo <= cell <= bhi in every field: no field borrows
+                        const uint32_t q = ccell[r];
+                        const bool inbox = (((q | GUARD) - blo) & (bhi_g - q) & GUARD) == GUARD;
                         const float dx = xp.x - cx[r].x, dy = xp.y - cx[r].y, dz = xp.z - cx[r].z;
                         d2 = dx * dx + dy * dy + dz * dz;
                         pass = inbox && d2 < T;
@@ -976,6 +984,23 @@ __global__ void k_record(StepArgs A, Ctl* ctl, int* __restrict__ k_out, float* _
     }
     __syncthreads();
     if (threadIdx.x == 0) ctl->step = step + 1u;
+}
+
+// Loop state for the host: the control words and scene 0's loop state are
+// written to pinned host memory, then a sequence number the host waits for.
+struct CtlMirror {
+    Ctl ctl;
+    SceneCtl scene0;
+    uint32_t seq;
+    uint32_t pad[3];
+};
+
+__global__ void k_publish(const Ctl* __restrict__ ctl, const SceneCtl* __restrict__ sctl, CtlMirror* m, uint32_t seq) {
+    if (threadIdx.x != 0) return;
+    m->ctl = *ctl;
+    m->scene0 = *sctl;
+    __threadfence_system();
+    *(volatile uint32_t*)&m->seq = seq;
 }
 
 // ComputeCorrectivePressureForce (:5493-5522), then the prediction of the
@@ -1677,6 +1702,9 @@ struct upf_sim {
     Ctl* h_ctl = nullptr;   // pinned mirror
     SceneCtl* sctl = nullptr;
     SceneCtl* h_sctl = nullptr;  // pinned mirror of scene 0
+    CtlMirror* mirror = nullptr; // pinned, written by k_publish
+    CtlMirror* mirror_dev = nullptr;  // its device address
+    uint32_t mirror_seq = 0;
     // scenes (a plain simulation is one scene with no per-slot arrays)
     int n_scenes = 1;
     int sps = MAX_SLOTS;          // max-reduction slots per scene
@@ -1808,10 +1836,28 @@ static int sort_and_bin(upf_sim* s) {
     return UPF_OK;
 }
 
+// The loop state after all work enqueued so far: a one-thread kernel writes
+// it to pinned host memory with a sequence number the host spins on (a
+// blocking stream synchronisation wakes the host up later than the GPU
+// finishes, and the GPU idles meanwhile).
 static int read_ctl(upf_sim* s) {
-    HIPCHK(hipMemcpyAsync(s->h_ctl, s->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipMemcpyAsync(s->h_sctl, s->sctl, sizeof(SceneCtl), hipMemcpyDeviceToHost, s->st));
-    HIPCHK(hipStreamSynchronize(s->st));
+    const uint32_t seq = ++s->mirror_seq;
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s->st, s->ctl, s->sctl, s->mirror_dev, seq);
+    HIPCHK(hipGetLastError());
+    const volatile uint32_t* flag = &s->mirror->seq;
+    for (uint64_t spins = 1; *flag != seq; ++spins) {
+        if ((spins & 4095u) == 0) {
+            const hipError_t e = hipStreamQuery(s->st);
+            if (e == hipSuccess) {
+                if (*flag == seq) break;
+                return UPF_ERR_HIP;  // the stream drained without the mirror update
+            }
+            if (e != hipErrorNotReady) return UPF_ERR_HIP;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    *s->h_ctl = s->mirror->ctl;
+    *s->h_sctl = s->mirror->scene0;
     s->prof.collect();
     return UPF_OK;
 }
@@ -2078,6 +2124,9 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     s->allocs.push_back(tmp);
     HIPCHK(hipHostMalloc((void**)&s->h_ctl, sizeof(Ctl), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&s->h_sctl, sizeof(SceneCtl), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&s->mirror, sizeof(CtlMirror), hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(s->mirror, 0, sizeof(CtlMirror));
+    HIPCHK(hipHostGetDevicePointer((void**)&s->mirror_dev, s->mirror, 0));
     std::vector<float> L(5 * LUT_N);
     build_luts(p->support_radius, L.data());
     HIPCHK(hipMemcpy(s->luts, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -2129,6 +2178,7 @@ int upf_destroy(upf_sim* s) {
     if (s->d_body_start) (void)hipFree(s->d_body_start);
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     if (s->h_sctl) (void)hipHostFree(s->h_sctl);
+    if (s->mirror) (void)hipHostFree(s->mirror);
     if (s->st) (void)hipStreamDestroy(s->st);
     delete s;
     return UPF_OK;
