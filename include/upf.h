@@ -342,7 +342,7 @@ int upf_adjoint_timing(upf_adjoint* a, float* forward_ms, float* backward_ms, si
 /* ------------------------------------------------------ launch tuning */
 /* Launch shapes that change speed only, never results (no reference
  * counterpart; the reference fixes them at compile time).  Defaults
- * {256, 8192, 4, 0, 4, 1, 8, 0, 4}.  A batch recaptures its step graph when its
+ * {256, 8192, 4, 0, 4, 1, 8, 0, 4, 1}.  A batch recaptures its step graph when its
  * tuning (set through its upf_batch_sim handle) changed.                   */
 typedef struct upf_tuning {
     int loop_block;   /* threads per block of the grid-stride passes: 64, 128, 256     */
@@ -356,6 +356,8 @@ typedef struct upf_tuning {
                          form with 256 / 512 candidates staged in LDS at a time        */
     int spec_first;   /* host loop: first enqueue the previous step's iteration count,
                          at most this many past the minimum (0 .. 64)                  */
+    int xcd_remap;    /* 1: each XCD (own L2) takes one contiguous eighth of the blocks
+                         of the list build and the grid-stride passes; 0: round robin   */
 } upf_tuning;
 int upf_set_tuning(upf_sim* s, const upf_tuning* t);
 int upf_get_tuning(const upf_sim* s, upf_tuning* t);

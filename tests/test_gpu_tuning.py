@@ -34,13 +34,13 @@ def _run(p, pos, typ, tuning, steps=5):
 
 @pytest.mark.parametrize("tuning", [
     dict(loop_block=64, grid_cap=16, nbr_rounds=2, nbr_stage=32, speculate=1, loop_list=0, loop_width=4, nbr_tile=0,
-         spec_first=0),
+         spec_first=0, xcd_remap=0),
     dict(loop_block=128, grid_cap=2048, nbr_rounds=8, nbr_stage=32, speculate=6, loop_list=1, loop_width=8,
-         nbr_tile=256, spec_first=64),
+         nbr_tile=256, spec_first=64, xcd_remap=1),
     dict(loop_block=256, grid_cap=3, nbr_rounds=4, nbr_stage=64, speculate=2, loop_list=0, loop_width=8, nbr_tile=512,
-         spec_first=2),
+         spec_first=2, xcd_remap=1),
     dict(loop_block=256, grid_cap=8192, nbr_rounds=8, nbr_stage=0, speculate=4, loop_list=1, loop_width=4, nbr_tile=0,
-         spec_first=4),
+         spec_first=4, xcd_remap=0),
 ])
 def test_tunings_give_identical_steps(gpu, tuning):
     p, pos, typ = _compressed_scene()

@@ -57,11 +57,22 @@ struct StepArgs {
     int n_scenes;
     int slots_per_scene;
     int code_lists;      // the correction loop reads delta-coded lists (the external-force pass writes them)
+    int xcd;             // remap blocks to XCDs by contiguous ranges (grids are multiples of 8)
 };
 
 __device__ inline int scene_at(const StepArgs& A, int i) { return A.scene_of ? A.scene_of[i] : 0; }
 __device__ inline Dom dom_at(const StepArgs& A, int i) { return A.scene_of ? A.doms[A.scene_of[i]] : A.dom0; }
 __device__ inline int gstride() { return gridDim.x * blockDim.x; }
+
+// Workgroups go to the 8 XCDs round-robin, each XCD with its own L2.  With the
+// grid a multiple of 8, remap gives every XCD one contiguous eighth of the
+// block range, so that blocks of neighbouring slots -- which gather the same
+// neighbour data -- share an L2.
+__device__ inline int xblk(int remap) {
+    if (!remap) return blockIdx.x;
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
 
 // ------------------------------------------------------------------ keys
 __global__ void k_keys(StepArgs A, const float4* __restrict__ pos, uint32_t* __restrict__ keys, Ctl* ctl) {
@@ -384,7 +395,7 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
     const int n = A.n;
     const int stride = A.stride;
     const int lane = threadIdx.x;
-    const int base = blockIdx.x * 64;
+    const int base = xblk(A.xcd) * 64;
     if (base >= n) return;
     const int wend = min(base + 64, n);
     const float T = P.support_radius_sq_limit;
@@ -566,7 +577,7 @@ __global__ __launch_bounds__(64) void k_nbr_build_tile(StepArgs A, const float4*
     __shared__ uint32_t s_start[TB_CELLS], s_cell[TB_CELLS], s_pref[TB_CELLS + 1];
     const upf_params& P = A.P;
     const int lane = threadIdx.x;
-    const int base = blockIdx.x * 64;
+    const int base = xblk(A.xcd) * 64;
     if (base >= A.n) return;
     const int wend = min(base + 64, A.n);
     const int p = base + lane;
@@ -740,7 +751,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     }
     __syncthreads();
     const upf_params& P = A.P;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
+    for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
     const float4 xi = S.pos[i];
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
@@ -860,7 +871,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     float run = 0.0f;  // single scene: running max over the thread's slots
     const int n_round = (A.n + gstride() - 1) / gstride();
     for (int r = 0; r < n_round; ++r) {
-        const int i = r * gstride() + blockIdx.x * blockDim.x + threadIdx.x;
+        const int i = r * gstride() + xblk(A.xcd) * blockDim.x + threadIdx.x;
         float mine = 0.0f;
         int sc = -1;
         if (i < A.n) {
@@ -1027,7 +1038,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lpg[k] = luts[LUT_N + k];
     __syncthreads();
     const upf_params& P = A.P;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
+    for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
     const float4 xi = S.pos[i];
     if (ptype(xi) != UPF_LIQUID) continue;
     const int sc = scene_at(A, i);
@@ -1071,7 +1082,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
 // box clamp with damping -0.5, new z-index key of every particle.
 __global__ void k_integrate(StepArgs A, Slots S, Work W, int pcisph, int pure_fluid_clamp, int has_bodies, Ctl* ctl) {
     const upf_params& P = A.P;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
+    for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
     float4 x = S.pos[i];
     const int ti = ptype(x);
     const Dom d = dom_at(A, i);
@@ -1718,7 +1729,7 @@ struct upf_sim {
     float* d_rstat = nullptr;
     int stat_cap = 0;
     int steps_done = 0;
-    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4};
+    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4, 1};
     bool static_lists = false;  // the lists being built serve the static PCISPH step only
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
@@ -1763,6 +1774,7 @@ struct upf_sim {
         A.n_scenes = n_scenes;
         A.slots_per_scene = sps;
         A.code_lists = tune.loop_list;
+        A.xcd = tune.xcd_remap;
         return A;
     }
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -1777,7 +1789,8 @@ static inline dim3 grid_for(int n) { return dim3((unsigned)((n + BLOCK - 1) / BL
 static inline dim3 block_loop(const upf_sim* s) { return dim3((unsigned)s->tune.loop_block); }
 static inline dim3 grid_persist(const upf_sim* s, int n) {
     const int b = s->tune.loop_block;
-    return dim3((unsigned)std::max(1, std::min((n + b - 1) / b, s->tune.grid_cap)));
+    const int g = std::max(1, std::min((n + b - 1) / b, s->tune.grid_cap));
+    return dim3((unsigned)((g + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
 }
 
 static int check_params(const upf_params* p) {
@@ -1932,7 +1945,7 @@ static void pcisph_loop_device(upf_sim* s, const StepArgs& A) {
 static void build_lists(upf_sim* s, const StepArgs& A) {
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_NBR_BUILD, s->st);
-    const dim3 g((unsigned)((s->n + 63) / 64));
+    const dim3 g((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
     auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, g, dim3(64), 0, s->st, A, S.pos, s->W.keys_sorted, s->W.cells, s->W.nbr, s->W.ncount,
                            s->ctl);
@@ -2294,7 +2307,8 @@ int upf_set_tuning(upf_sim* s, const upf_tuning* t) {
     if (t->grid_cap < 1 || (t->nbr_rounds != 2 && t->nbr_rounds != 4 && t->nbr_rounds != 8) ||
         (t->nbr_stage != 0 && t->nbr_stage != 32 && t->nbr_stage != 64) || t->speculate < 1 || t->speculate > 64 ||
         (t->loop_list != 0 && t->loop_list != 1) || (t->loop_width != 4 && t->loop_width != 8) ||
-        (t->nbr_tile != 0 && t->nbr_tile != 256 && t->nbr_tile != 512) || t->spec_first < 0 || t->spec_first > 64)
+        (t->nbr_tile != 0 && t->nbr_tile != 256 && t->nbr_tile != 512) || t->spec_first < 0 || t->spec_first > 64 ||
+        (t->xcd_remap != 0 && t->xcd_remap != 1))
         return UPF_ERR_ARG;
     s->tune = *t;
     return UPF_OK;
