@@ -388,6 +388,9 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                                   uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
     constexpr int NB_CHUNK = 64 * ROUNDS;
+    // packed cell coordinates: 10-bit fields whose top bit (coordinates are
+    // below 512) serves as the borrow guard of a packed compare
+    constexpr uint32_t GUARD = (1u << 9) | (1u << 19) | (1u << 29);
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
     __shared__ uint32_t stage[(STAGE > 0 ? STAGE : 1) * 65];
     const upf_params& P = A.P;
@@ -401,14 +404,34 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
     const float T = P.support_radius_sq_limit;
     const float h = P.support_radius;
     const int cap = P.max_neighbors;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    // the wave's 64 slots, one per lane: position, key, scene; the serial
-    // loops below take them from the owning lane instead of from memory
+    // the wave's 64 slots, one per lane: position, key, scene and the query
+    // box of the slot's particle; the serial loop below reads them from the
+    // owning lane into scalars
     const int me = base + lane;
     const bool mine_valid = me < wend;
     const float4 my_x = mine_valid ? pos[me] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t my_key = mine_valid ? keys[me] : 0xFFFFFFFFu;
     const int my_scene = mine_valid ? scene_at(A, me) : -1;
+    const bool my_liquid = mine_valid && ptype(my_x) == UPF_LIQUID;
+    uint32_t my_blo = 0u, my_bhi = GUARD;
+    if (my_liquid) {
+        // BoxQuery of the particle in cell coordinates, clipped to the grid
+        const Dom dm = A.scene_of ? A.doms[my_scene] : A.dom0;
+        const float xv[3] = {my_x.x, my_x.y, my_x.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float lo_f = dm.scales[a] * (xv[a] - h);
+            float hi_f = dm.scales[a] * (xv[a] + h);
+            lo_f = lo_f >= 0.0f ? lo_f : 0.0f;
+            hi_f = hi_f <= 1024.0f ? hi_f : 1024.0f;
+            const uint32_t l = ((uint32_t)(unsigned short)lo_f) >> dm.block_shift;
+            uint32_t u = ((uint32_t)(unsigned short)hi_f) >> dm.block_shift;
+            u = u < dm.grid_dim ? u : dm.grid_dim - 1;  // cells lie below G <= 512
+            my_blo |= l << (10 * a);
+            my_bhi |= u << (10 * a);
+        }
+    }
+    const uint64_t liquid = __ballot(my_liquid);
     int my_cnt = 0;  // this lane's slot: list length so far
     int s = base;
     while (s < wend) {
@@ -422,6 +445,9 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
         // end of the cell within the wave: first later slot of another cell or scene
         const uint64_t other = __ballot(mine_valid && me > s && ((my_key >> shift3) != c || my_scene != sc));
         const int pe = other ? base + __ffsll((long long)other) - 1 : wend;
+        // the cell's liquid slots (bits sl .. pe - base - 1)
+        const uint64_t upto = (pe - base) >= 64 ? ~0ull : ((1ull << (pe - base)) - 1ull);
+        const uint64_t cell_liquid = liquid & upto & ~((1ull << sl) - 1ull);
         const uint32_t cc[3] = {compact10(c), compact10(c >> 1), compact10(c >> 2)};
         // stencil cells in Morton order: lane e names enumeration index e
         BoxCells bc;
@@ -449,6 +475,7 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                 rcell = q[0] | (q[1] << 10) | (q[2] << 20);
             }
         }
+        const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
         const uint64_t vm = __ballot(valid);
         const int nr = __popcll(vm);
         const int rank = __popcll(vm & lt);
@@ -462,8 +489,9 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
         if (lane == 0) r_pref[0] = 0;
         __syncthreads();
         const int M = (int)r_pref[nr];
-        for (int c0 = 0; c0 < M; c0 += NB_CHUNK) {
-            // this lane's candidates of the chunk
+        for (int c0 = 0; c0 < M && cell_liquid; c0 += NB_CHUNK) {
+            // this lane's candidates of the chunk; a lane past the end holds
+            // an infinitely distant candidate, which fails the distance test
             float4 cx[ROUNDS];
             int cj[ROUNDS];
             uint32_t ccell[ROUNDS];
@@ -472,6 +500,8 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
             for (int r = 0; r < ROUNDS; ++r) {
                 const int ci = c0 + r * 64 + lane;
                 cj[r] = -1;
+                ccell[r] = 0u;
+                cx[r] = make_float4(__int_as_float(0x7f800000), 0.f, 0.f, 0.f);
                 if (ci < M) {
                     while (k + 1 < nr && (int)r_pref[k + 1] <= ci) ++k;
                     cj[r] = (int)r_start[k] + (ci - (int)r_pref[k]);
@@ -479,51 +509,44 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     cx[r] = pos[cj[r]];
                 }
             }
-            for (int p = s; p < pe; ++p) {
-                const int pl = p - base;
-                const float4 xp = make_float4(__shfl(my_x.x, pl, 64), __shfl(my_x.y, pl, 64),
-                                              __shfl(my_x.z, pl, 64), __shfl(my_x.w, pl, 64));
-                if (ptype(xp) != UPF_LIQUID) continue;
-                // the particle's query box in cell coordinates, packed like
-                // the candidates' cells: 10-bit fields whose top bit (coordinates
-                // are below 512) serves as the borrow guard of a packed compare
-                uint32_t blo = 0u, bhi = 0u;
-                const float xv[3] = {xp.x, xp.y, xp.z};
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    float lo_f = d.scales[a] * (xv[a] - h);
-                    float hi_f = d.scales[a] * (xv[a] + h);
-                    lo_f = lo_f >= 0.0f ? lo_f : 0.0f;
-                    hi_f = hi_f <= 1024.0f ? hi_f : 1024.0f;
-                    const uint32_t l = ((uint32_t)(unsigned short)lo_f) >> d.block_shift;
-                    uint32_t u = ((uint32_t)(unsigned short)hi_f) >> d.block_shift;
-                    u = u < G ? u : G - 1;  // cells lie below G <= 512
-                    blo |= l << (10 * a);
-                    bhi |= u << (10 * a);
-                }
-                constexpr uint32_t GUARD = (1u << 9) | (1u << 19) | (1u << 29);
-                const uint32_t bhi_g = bhi | GUARD;
-                int cnt = __shfl(my_cnt, pl, 64);
+            const int rounds = min(ROUNDS, (M - c0 + 63) >> 6);
+            uint64_t pm = cell_liquid;
+            while (pm) {
+                const int pl = __builtin_ctzll(pm);
+                pm &= pm - 1ull;
+                const int p = base + pl;
+                const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_x.x), pl));
+                const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_x.y), pl));
+                const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_x.z), pl));
+                const uint32_t blo = (uint32_t)__builtin_amdgcn_readlane((int)my_blo, pl);
+                const uint32_t bhi_g = (uint32_t)__builtin_amdgcn_readlane((int)my_bhi, pl);
+                int cnt = __builtin_amdgcn_readlane(my_cnt, pl);
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
-                    bool pass = false;
-                    float d2 = 0.0f;
-                    if (cj[r] >= 0 && cj[r] != p) {
-                        // blo <= cell <= bhi in every field: no field borrows
-                        const uint32_t q = ccell[r];
-                        const bool inbox = (((q | GUARD) - blo) & (bhi_g - q) & GUARD) == GUARD;
-                        const float dx = xp.x - cx[r].x, dy = xp.y - cx[r].y, dz = xp.z - cx[r].z;
-                        d2 = dx * dx + dy * dy + dz * dz;
-                        pass = inbox && d2 < T;
-                    }
+                    if (r >= rounds) break;
+                    // blo <= cell <= bhi in every field: no field borrows
+                    const uint32_t q = ccell[r];
+                    const bool inbox = (((q | GUARD) - blo) & (bhi_g - q) & GUARD) == GUARD;
+                    const float dx = px - cx[r].x, dy = py - cx[r].y, dz = pz - cx[r].z;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    const bool pass = inbox & (d2 < T) & (cj[r] != p);
                     const uint64_t m = __ballot(pass);
+                    if (m == 0ull) continue;
                     if (pass) {
-                        const int slot = cnt + __popcll(m & lt);
+                        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        const int slot = cnt + (int)below;
                         // BIN == 0: the table bin is left to the readers (the
                         // external-force pass computes it from the same distance)
                         const uint32_t e = ((uint32_t)cj[r] << NB_SLOT_SHIFT) | (BIN ? lut_bin(sqrtf(d2), lut_factor) : 0u);
-                        if (slot < STAGE) stage[slot * 65 + pl] = e;
-                        else if (slot < cap) nbr[(size_t)slot * stride + p] = e;
+                        if constexpr (STAGE > 0) {
+                            if (slot < STAGE) stage[slot * 65 + pl] = e;
+                            else if (slot < cap) nbr[(size_t)slot * stride + p] = e;
+                        } else {
+                            // row cnt of the list column, then `below` rows on
+                            // (slots < 2^23: the offset fits 24-bit multiplies)
+                            if (slot < cap) (nbr + (size_t)cnt * stride + p)[__umul24(below, (uint32_t)stride)] = e;
+                        }
                     }
                     cnt += __popcll(m);
                 }
