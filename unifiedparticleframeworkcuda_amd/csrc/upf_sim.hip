@@ -18,8 +18,10 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "upf.h"
@@ -239,6 +241,38 @@ __device__ inline uint32_t compact10(uint32_t v) {  // inverse of dspread10
 __device__ inline uint32_t lut_bin(float dist, float factor) {
     const int idx = (int)(dist * factor * 300.0f);
     return idx >= 300 ? 300u : (uint32_t)idx;
+}
+
+// lut_bin(sqrt(d2)) with the correctly rounded square root, without
+// computing that root.  The bin is a non-decreasing step function of d2, so
+// it is fixed by its thresholds thr[k] = min{ d2 : bin(d2) >= k } (k = 1 ..
+// 300; thr[0] = 0, thr[301] = +inf), which the host finds by bisection over
+// the float bit patterns.  The hardware square root (within 1 ulp) gives a
+// bin b0 at most one away from the true one, and the two thresholds around
+// b0 (th[b0] = (thr[b0], thr[b0 + 1])) settle it.  upf_create checks this
+// equality on the device for every float below 2 thr[300].
+constexpr int LUT_TH_OFF = 1508;  // float offset of th[] after the 5 tables (8-byte aligned)
+constexpr int LUT_TOTAL = LUT_TH_OFF + 2 * LUT_N;
+__device__ __forceinline__ uint32_t bin_of_d2(float d2, float factor, const float2* th) {
+    const uint32_t b0 = lut_bin(__builtin_amdgcn_sqrtf(d2), factor);
+    const float2 t = th[b0];
+    return b0 - (d2 < t.x ? 1u : 0u) + (d2 >= t.y ? 1u : 0u);
+}
+
+// Counts the floats d2 in [0, limit) whose bin_of_d2 differs from
+// lut_bin(sqrt_rn(d2)).
+__global__ void k_check_bins(float factor, const float* __restrict__ luts, uint32_t limit_bits,
+                             unsigned long long* bad) {
+    __shared__ float2 th[LUT_N];
+    for (int k = threadIdx.x; k < LUT_N; k += blockDim.x)
+        th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
+    __syncthreads();
+    unsigned long long n_bad = 0;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < limit_bits; u += gridDim.x * blockDim.x) {
+        const float d2 = __uint_as_float(u);
+        n_bad += bin_of_d2(d2, factor, th) != lut_bin(sqrtf(d2), factor) ? 1ull : 0ull;
+    }
+    if (n_bad) atomicAdd(bad, n_bad);
 }
 
 // Delta coding of a list for the passes that re-read it every iteration:
@@ -768,9 +802,11 @@ template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                       Ctl* ctl) {
     __shared__ float lvl[LUT_N], lpg[LUT_N];
+    __shared__ float2 th[LUT_N];
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
         lvl[t] = luts[2 * LUT_N + t];
         lpg[t] = luts[LUT_N + t];
+        th[t] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[t];
     }
     __syncthreads();
     const upf_params& P = A.P;
@@ -779,6 +815,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
     W.pdiv[i] = 0.0f;
+    W.xpd[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
     W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     W.rho[i] = S.rho_last[i];
     if (ti != UPF_LIQUID) {
@@ -804,7 +841,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
             uint32_t bin = e & NB_BIN_MASK;
             if (bin_free) {  // the list build's expression on the same positions
                 const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                bin = lut_bin(sqrtf(dx * dx + dy * dy + dz * dz), A.lut_factor);
+                bin = bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th);
             }
             const float kv = lvl[bin];
             const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
@@ -885,8 +922,12 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                                                    Ctl* ctl) {
     if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
     __shared__ float lw[LUT_N];
+    __shared__ float2 th[LUT_N];
     __shared__ float wmax[BLOCK / 64];  // blockDim.x <= BLOCK
-    for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lw[k] = luts[k];
+    for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
+        lw[k] = k < 300 ? luts[k] : 0.0f;  // bin 300 (dist >= h) weighs 0, as in lut_at
+        th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
+    }
     __syncthreads();
     const upf_params& P = A.P;
     const int lane = threadIdx.x & 63;
@@ -914,7 +955,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                         const float d2 = dx * dx + dy * dy + dz * dz;
                         if (d2 < T) {
-                            const float w = lut_at(lw, sqrtf(d2), A.lut_factor);
+                            const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
                             float mj = m;
                             if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
                             rho = rho + w * mj;
@@ -924,7 +965,15 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 const float delta = A.scene_of ? A.doms[sc].delta : P.density_error_factor;
                 const float p = W.p[i] + fmax_ref(err * delta, 0.0f);
                 W.p[i] = p;
-                W.pdiv[i] = p / (P.rest_density * P.rest_density);
+                const float pd = p / (P.rest_density * P.rest_density);
+                if (MODE == UPF_BOUNDARY_AKINCI) {
+                    W.pdiv[i] = pd;
+                } else {
+                    // the force pass gathers current position and p / rho0^2
+                    // of a neighbour in one 16-byte load
+                    const float4 x0 = S.pos[i];
+                    W.xpd[i] = make_float4(x0.x, x0.y, x0.z, pd);
+                }
                 W.rho[i] = rho;
                 mine = rho;
             } else {
@@ -1058,7 +1107,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
             loop_test(A, sc, t, threadIdx.x & 63, ctl);
     }
     __shared__ float lpg[LUT_N];
-    for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) lpg[k] = luts[LUT_N + k];
+    __shared__ float2 th[LUT_N];
+    for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
+        lpg[k] = luts[LUT_N + k];
+        th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
+    }
     __syncthreads();
     const upf_params& P = A.P;
     for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
@@ -1077,12 +1130,18 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     // reference's `dist < h` test holds; the table bin is the one the list
     // build computed from the same positions (carried by the packed entry,
     // recomputed by the same expression for the coded lists)
+    // (Akinci boundaries also need the neighbour's type: position and p /
+    // rho0^2 come from their own arrays; else from one record)
     for_each_listed<CODED, G>(
-        W, i, st, cnt, [&](int j) { return PosPd{S.pos[j], W.pdiv[j]}; },
+        W, i, st, cnt,
+        [&](int j) {
+            if constexpr (MODE == UPF_BOUNDARY_AKINCI) return PosPd{S.pos[j], W.pdiv[j]};
+            else { const float4 r = W.xpd[j]; return PosPd{r, r.w}; }
+        },
         [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
             const float4& xj = q.x;
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const uint32_t bin = CODED ? lut_bin(sqrtf(dx * dx + dy * dy + dz * dz), A.lut_factor) : (e & NB_BIN_MASK);
+            const uint32_t bin = CODED ? bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th) : (e & NB_BIN_MASK);
             const float g = lpg[bin];
             // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
             // precomputed per particle by the density pass (same division)
@@ -1320,6 +1379,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
     W.pdiv[i] = 0.0f;
+    W.xpd[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
     W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     W.rho[i] = S.rho_last[i];
     if (ti != UPF_LIQUID && ti != UPF_RIGID) {
@@ -2124,6 +2184,38 @@ const char* upf_version(void) { return "upf-mi355x 0.1 (gfx950)"; }
 
 // Device buffers of a simulation of `capacity` slots in `n_scenes` scenes
 // whose cell tables hold `ncells` entries together.
+// Thresholds of the table bin as a function of the squared distance:
+// thr[k] = the smallest float d2 >= 0 with lut_bin(sqrt_rn(d2)) >= k, found by
+// bisection over the (ordered) bit patterns of the non-negative floats; the
+// host's sqrtf is the correctly rounded IEEE root.  out[2b] = thr[b],
+// out[2b + 1] = thr[b + 1] for b = 0 .. 300, with thr[0] = 0, thr[301] = +inf.
+static void build_bin_thresholds(float factor, float* out) {
+    auto bin = [&](uint32_t u) {
+        float d2;
+        std::memcpy(&d2, &u, 4);
+        const float v = std::sqrt(d2) * factor * 300.0f;
+        if (!(v < 2147483648.0f)) return 300;  // the device conversion saturates
+        const int idx = (int)v;
+        return idx >= 300 ? 300 : idx;
+    };
+    std::vector<float> thr(302);
+    thr[0] = 0.0f;
+    const uint32_t inf_bits = 0x7f800000u;
+    for (int k = 1; k <= 300; ++k) {
+        uint32_t lo = 0, hi = inf_bits;  // bin(lo) < k <= bin(hi)
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (bin(mid) >= k) hi = mid; else lo = mid;
+        }
+        std::memcpy(&thr[k], &hi, 4);
+    }
+    thr[301] = std::numeric_limits<float>::infinity();
+    for (int b = 0; b <= 300; ++b) {
+        out[2 * b] = thr[b];
+        out[2 * b + 1] = thr[b + 1];
+    }
+}
+
 static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     const upf_params* p = &s->P;
     s->cap = capacity;
@@ -2144,6 +2236,7 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.rho, capacity));
     HIPCHK(s->alloc(&W.pres, capacity));
     HIPCHK(s->alloc(&W.pdiv, capacity));
+    HIPCHK(s->alloc(&W.xpd, capacity));
     HIPCHK(s->alloc(&W.max_slots, nslots));
     HIPCHK(s->alloc(&W.ncount, capacity));
     HIPCHK(s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors));
@@ -2151,7 +2244,7 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.nbf, capacity));
     HIPCHK(s->alloc(&W.nbe, (size_t)capacity * NB_ESC));
     HIPCHK(s->alloc(&W.cells, ncells));
-    HIPCHK(s->alloc(&s->luts, 5 * LUT_N));
+    HIPCHK(s->alloc(&s->luts, LUT_TOTAL));
     HIPCHK(s->alloc(&s->ctl, 1));
     HIPCHK(s->alloc(&s->sctl, n_scenes));
     void* tmp = nullptr;
@@ -2163,9 +2256,29 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(hipHostMalloc((void**)&s->mirror, sizeof(CtlMirror), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(s->mirror, 0, sizeof(CtlMirror));
     HIPCHK(hipHostGetDevicePointer((void**)&s->mirror_dev, s->mirror, 0));
-    std::vector<float> L(5 * LUT_N);
+    std::vector<float> L(LUT_TOTAL, 0.0f);
     build_luts(p->support_radius, L.data());
+    const float factor = (float)(1.0 / (double)p->support_radius);  // StepArgs::lut_factor
+    build_bin_thresholds(factor, L.data() + LUT_TH_OFF);
     HIPCHK(hipMemcpy(s->luts, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice));
+    {
+        // the sqrt-free bins equal the correctly rounded ones for every float
+        // below 2 thr[300] (above it both are 300)
+        unsigned long long* bad = nullptr;
+        HIPCHK(s->alloc(&bad, 1));
+        HIPCHK(hipMemset(bad, 0, sizeof(unsigned long long)));
+        const float top = 2.0f * L[LUT_TH_OFF + 2 * 300];
+        uint32_t top_bits;
+        std::memcpy(&top_bits, &top, 4);
+        hipLaunchKernelGGL(k_check_bins, dim3(4096), dim3(256), 0, s->st, factor, s->luts, top_bits, bad);
+        unsigned long long h_bad = 1;
+        HIPCHK(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, s->st));
+        HIPCHK(hipStreamSynchronize(s->st));
+        if (h_bad != 0) {
+            std::fprintf(stderr, "upf: hardware square root bins disagree with the correctly rounded ones (%llu floats)\n", h_bad);
+            return UPF_ERR_HIP;
+        }
+    }
     HIPCHK(hipMemset(s->ctl, 0, sizeof(Ctl)));
     HIPCHK(hipMemset(s->sctl, 0, (size_t)n_scenes * sizeof(SceneCtl)));
     HIPCHK(hipMemset(W.max_slots, 0, nslots * sizeof(uint32_t)));
