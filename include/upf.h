@@ -106,7 +106,9 @@ typedef struct upf_params {
     float vbox_min[3];             /* virtual bounding box (origin must be 0)      */
     float vbox_max[3];
     float scales[3];               /* GRID_RESOLUTION / extent                     */
-    uint32_t block_size;           /* ZIndex::block_size_ (power of two)          */
+    uint32_t block_size;           /* ZIndex::block_size_ (power of two; must exceed
+                                      scales[a] * support_radius by 0.01 % so that a
+                                      query box spans at most 3 cells per axis) */
     uint32_t block_shift;          /* log2(block_size)                             */
     uint32_t grid_dim;             /* cells per axis = 1024 / block_size           */
     float collision_min[3];        /* collisionBox                                 */

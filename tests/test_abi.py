@@ -72,6 +72,26 @@ def test_set_domain_rejects_cells_smaller_than_h():
     assert rc == upf._abi.UPF_ERR_ARG
 
 
+def test_cells_exactly_as_wide_as_h_are_rejected():
+    """A cell exactly as wide as the scaled support radius lets a query box
+    reach a fourth cell by rounding; set_domain and upf_create refuse it
+    (upf_create checks parameters before it looks for a device)."""
+    p = upf.make_params(0.01)
+    E = 1024 * p.support_radius / 8.0  # scales * h = 8 = block size
+    vmax = (C.c_float * 3)(E, E, E)
+    cmin = (C.c_float * 3)(0.0, 0.0, 0.0)
+    cmax = (C.c_float * 3)(E * 0.5, E * 0.5, E * 0.5)
+    L = upf.lib()
+    assert L.upf_params_set_domain(C.byref(p), vmax, cmin, cmax) == upf._abi.UPF_ERR_ARG
+    q = upf.make_params(0.01)
+    E = 1024 * q.support_radius / 7.8  # valid: scales * h = 7.8 < 8
+    vmax = (C.c_float * 3)(E, E, E)
+    assert L.upf_params_set_domain(C.byref(q), vmax, cmin, cmax) == upf._abi.UPF_OK
+    q.scales[0] = q.block_size / q.support_radius
+    h = C.c_void_p()
+    assert L.upf_create(C.byref(q), 64, C.byref(h)) == upf._abi.UPF_ERR_ARG
+
+
 def test_density_error_factor_rejects_reference_ub_spacing():
     p = upf.make_params(0.0125)
     rc = upf.lib().upf_compute_density_error_factor(C.byref(p), None)

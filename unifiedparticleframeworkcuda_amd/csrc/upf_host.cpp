@@ -172,7 +172,9 @@ int upf_params_set_domain(upf_params* p, const float vmax[3], const float cmin[3
     while ((1u << sh) < b) ++sh;
     p->block_shift = sh;
     p->grid_dim = 1024u / b;
-    if (b < 2 || (float)b < bs) return UPF_ERR_ARG;  // cell must hold the support radius
+    // a cell must hold the support radius with room for rounding (a query box
+    // then spans at most 3 cells per axis)
+    if (b < 2 || !((float)b > bs * 1.0001f)) return UPF_ERR_ARG;
     for (int a = 0; a < 3; ++a)
         if (!(cmin[a] >= 0.0f) || !(cmax[a] * p->scales[a] < 1024.0f) || !(cmin[a] < cmax[a])) return UPF_ERR_ARG;
     return UPF_OK;

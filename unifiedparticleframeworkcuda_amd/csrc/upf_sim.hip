@@ -353,6 +353,13 @@ struct ListDecoder {
 // together, then use(j, entry, value) runs for them in list order.  CODED:
 // the delta-coded lists (entry = 0; eight entries per list load), else the
 // packed 32-bit lists (entry = (slot << 9) | bin).
+// base[j] addressed with a 32-bit byte offset (slots < 2^23, records <= 16
+// bytes), which lets the load take its base from scalar registers
+template <class T>
+__device__ __forceinline__ T at_u32(const T* __restrict__ base, int j) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (uint32_t)j * (uint32_t)sizeof(T));
+}
+
 template <int CODED, int G, class Load, class Use>
 __device__ __forceinline__ void for_each_listed(const Work& W, int i, size_t st, int cnt, Load load, Use use) {
     using V = decltype(load(0));
@@ -950,15 +957,23 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 // G neighbours per round: their loads are issued together,
                 // the sum still runs in list order
                 for_each_listed<CODED, G>(
-                    W, i, st, cnt, [&](int j) { return W.xs[j]; },
+                    W, i, st, cnt, [&](int j) { return at_u32(W.xs, j); },
                     [&](int j, uint32_t, const float4& xj) __attribute__((always_inline)) {
                         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                         const float d2 = dx * dx + dy * dy + dz * dz;
-                        if (d2 < T) {
-                            const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
-                            float mj = m;
-                            if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
-                            rho = rho + w * mj;
+                        if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
+                            if (d2 < T) {
+                                const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
+                                float mj = m;
+                                if (ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
+                                rho = rho + w * mj;
+                            }
+                        } else {
+                            // without a branch: a pair at d2 >= T adds w = +0, and
+                            // rho + (+0) * m is rho exactly (rho > 0)
+                            const float wt = lw[bin_of_d2(d2, A.lut_factor, th)];
+                            const float w = d2 < T ? wt : 0.0f;
+                            rho = rho + w * m;
                         }
                     });
                 const float err = fmax_ref(rho - P.rest_density, 0.0f);
@@ -1136,7 +1151,7 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
         W, i, st, cnt,
         [&](int j) {
             if constexpr (MODE == UPF_BOUNDARY_AKINCI) return PosPd{S.pos[j], W.pdiv[j]};
-            else { const float4 r = W.xpd[j]; return PosPd{r, r.w}; }
+            else { const float4 r = at_u32(W.xpd, j); return PosPd{r, r.w}; }
         },
         [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
             const float4& xj = q.x;
@@ -1880,6 +1895,11 @@ static int check_params(const upf_params* p) {
     if (!p) return UPF_ERR_ARG;
     if (p->max_neighbors > 4096) return UPF_ERR_ARG;
     if (p->block_size < 2 || p->grid_dim == 0 || p->grid_dim > 512) return UPF_ERR_ARG;
+    // the list builds take candidates from the 3x3x3 cells around a particle's
+    // cell: the query box [x - h, x + h] must not reach a fourth cell, so a
+    // cell is wider than the scaled support radius by more than rounding
+    for (int a = 0; a < 3; ++a)
+        if (!((float)p->block_size > p->scales[a] * p->support_radius * 1.0001f)) return UPF_ERR_ARG;
     if (p->vbox_min[0] != 0.0f || p->vbox_min[1] != 0.0f || p->vbox_min[2] != 0.0f) return UPF_ERR_ARG;
     if (p->max_neighbors <= 0 || p->max_neighbors > 4096) return UPF_ERR_ARG;
     return UPF_OK;
