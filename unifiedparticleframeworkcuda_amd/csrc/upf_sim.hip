@@ -1547,11 +1547,11 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
 
 // key of a slot for grouping rigid particles by body (non-rigid last)
 __global__ void k_body_keys(int n, const int* __restrict__ body, const float4* __restrict__ pos,
-                            uint32_t* __restrict__ keys) {
+                            uint32_t* __restrict__ keys, uint32_t none) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int b = body[i];
-    keys[i] = (b >= 0 && ptype(pos[i]) == UPF_RIGID) ? (uint32_t)b : 0xFFFFu;
+    keys[i] = (b >= 0 && ptype(pos[i]) == UPF_RIGID) ? (uint32_t)b : none;
 }
 
 __global__ void k_body_bounds(int n, const uint32_t* __restrict__ keys, int nb, int* __restrict__ start) {
@@ -1575,52 +1575,92 @@ __device__ inline void mat3mul(const float* A, const float* B, float* C) {  // M
 }
 
 // CalculateRigidBodyForceTorqueMomenta (:3620-3703) with RigidBody::UpdateMomenta,
-// PerformStep, UpdateRotationMatrix (src/cpu/RigidBody.cpp:55-160): one thread
-// per body sums its particles' forces, torques and weights in slot order.
-__global__ void k_rigid_bodies(int nb, upf_body* __restrict__ bodies, const int* __restrict__ start,
-                               const uint32_t* __restrict__ members, const float4* __restrict__ pos,
-                               const float4* __restrict__ force, const float4* __restrict__ rel, upf_params P) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+// PerformStep, UpdateRotationMatrix (src/cpu/RigidBody.cpp:55-160): one block
+// per body.  The block loads RB_CHUNK members (in slot order) at a time and
+// computes each member's force, torque and weight terms in parallel; six
+// lanes then run the six sums -- F.x, F.y, F.z, T.x, T.y, T.z -- each in
+// member order with the reference's additions (F.x + f.x + 0, F.y + f.y +
+// gy, F.z + f.z + 0, T + t), so every sum has the serial loop's rounding.
+constexpr int RB_CHUNK = 256;
+__global__ __launch_bounds__(RB_CHUNK) void k_rigid_bodies(int nb, upf_body* __restrict__ bodies,
+                                                           const int* __restrict__ start,
+                                                           const uint32_t* __restrict__ members,
+                                                           const float4* __restrict__ pos,
+                                                           const float4* __restrict__ force,
+                                                           const float4* __restrict__ rel, upf_params P) {
+    const int b = blockIdx.x;
     if (b >= nb) return;
-    upf_body B = bodies[b];
-    float F[3] = {0.0f, 0.0f, 0.0f}, Tq[3] = {0.0f, 0.0f, 0.0f};
-    const int s0 = start[b];
-    int s1 = nb;
-    for (int k = b + 1; k <= nb; ++k) {
-        if (start[k] >= 0) { s1 = start[k]; break; }
+    // f.x f.y f.z t.x t.y t.z gy of a chunk's members, two chunks in flight
+    __shared__ float terms[2][7][RB_CHUNK];
+    __shared__ float sums[6];
+    __shared__ int range[2];
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        const int s0 = start[b];
+        int s1 = nb;
+        for (int k = b + 1; k <= nb; ++k) {
+            if (start[k] >= 0) { s1 = start[k]; break; }
+        }
+        range[0] = s0;
+        range[1] = s0 >= 0 ? s1 : s0;
     }
-    // eight members per round: their loads are issued together, the sums
-    // still run in slot order
-    for (int k0 = s0; s0 >= 0 && k0 < s1; k0 += 8) {
-        const int mk = s1 - k0;
-        int ii[8];
-        float4 ff[8], xx[8];
-        float ww[8];
+    __syncthreads();
+    const int s0 = range[0], s1 = range[1];
+    const float bx = bodies[b].pos[0], by = bodies[b].pos[1], bz = bodies[b].pos[2];
+    // the terms of member k0 + tid; past the body's last member every term
+    // is +0, which leaves a sum unchanged (a sum from +0 is never -0)
+    auto load_chunk = [&](int k0, int buf) {
+        const int k = k0 + tid;
+        float t[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (k < s1) {
+            const int i = (int)members[k];
+            const float4 f = force[i];
+            const float4 x = pos[i];
+            const float r0 = x.x - bx, r1 = x.y - by, r2 = x.z - bz;
+            t[0] = f.x;
+            t[1] = f.y;
+            t[2] = f.z;
+            t[3] = r1 * f.z - r2 * f.y;
+            t[4] = r2 * f.x - r0 * f.z;
+            t[5] = r0 * f.y - r1 * f.x;
+            t[6] = -rel[i].w * P.gravity;
+        }
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (u < mk) ii[u] = (int)members[k0 + u];
+        for (int c = 0; c < 7; ++c) terms[buf][c][tid] = t[c];
+    };
+    float acc = 0.0f;  // lane a < 6: sum a
+    if (s0 < s1) load_chunk(s0, 0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = s0; k0 < s1; k0 += RB_CHUNK, buf ^= 1) {
+        // the next chunk's loads go out before this chunk's sums
+        if (k0 + RB_CHUNK < s1) load_chunk(k0 + RB_CHUNK, buf ^ 1);
+        if (tid < 6) {
+            // sixteen members' terms at a time into registers (their LDS
+            // reads overlap), then the additions in member order
+            const int m = min(RB_CHUNK, s1 - k0);
+            const bool second = tid < 3;  // F sums add a second term per member
+            for (int u0 = 0; u0 < m; u0 += 16) {
+                float v[16], g[16];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (u < mk) {
-                ff[u] = force[ii[u]];
-                xx[u] = pos[ii[u]];
-                ww[u] = rel[ii[u]].w;
+                for (int q = 0; q < 16; ++q) {
+                    v[q] = terms[buf][tid][u0 + q];
+                    g[q] = tid == 1 ? terms[buf][6][u0 + q] : 0.0f;
+                }
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    acc = acc + v[q];
+                    if (second) acc = acc + g[q];
+                }
             }
         }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            if (u >= mk) break;
-            const float4 f = ff[u];
-            const float4 x = xx[u];
-            F[0] = F[0] + f.x; F[1] = F[1] + f.y; F[2] = F[2] + f.z;
-            const float r0 = x.x - B.pos[0], r1 = x.y - B.pos[1], r2 = x.z - B.pos[2];
-            Tq[0] = Tq[0] + (r1 * f.z - r2 * f.y);
-            Tq[1] = Tq[1] + (r2 * f.x - r0 * f.z);
-            Tq[2] = Tq[2] + (r0 * f.y - r1 * f.x);
-            const float gy = -ww[u] * P.gravity;
-            F[0] = F[0] + 0.0f; F[1] = F[1] + gy; F[2] = F[2] + 0.0f;
-        }
+        __syncthreads();
     }
+    if (tid < 6) sums[tid] = acc;
+    __syncthreads();
+    if (tid != 0) return;
+    upf_body B = bodies[b];
+    const float F[3] = {sums[0], sums[1], sums[2]}, Tq[3] = {sums[3], sums[4], sums[5]};
     const float dt = P.dt;
     const float terminal = B.mass * P.terminal_speed;
 #pragma unroll
@@ -2161,11 +2201,14 @@ static int step_pcisph_rigid(upf_sim* s) {
     s->prof.end(s->st);
     // rigid bodies: members grouped by body in slot order, one thread per body
     s->prof.begin(PC_RIGID, s->st);
-    hipLaunchKernelGGL(k_body_keys, g, dim3(BLOCK), 0, s->st, s->n, S.body, S.pos, s->d_body_keys);
-    HIPCHK(radix_sort(s->d_body_keys, nullptr, s->d_body_keys2, s->d_body_members, s->n, 16, s->sort_tmp, s->st));
+    // body keys below 255: one 8-bit pass of the stable sort (non-rigid: 255)
+    const int body_bits = s->n_bodies < 255 ? 8 : 16;
+    hipLaunchKernelGGL(k_body_keys, g, dim3(BLOCK), 0, s->st, s->n, S.body, S.pos, s->d_body_keys,
+                       body_bits == 8 ? 0xFFu : 0xFFFFu);
+    HIPCHK(radix_sort(s->d_body_keys, nullptr, s->d_body_keys2, s->d_body_members, s->n, body_bits, s->sort_tmp, s->st));
     HIPCHK(hipMemsetAsync(s->d_body_start, 0xFF, (size_t)(s->n_bodies + 1) * sizeof(int), s->st));
     hipLaunchKernelGGL(k_body_bounds, g, dim3(BLOCK), 0, s->st, s->n, s->d_body_keys2, s->n_bodies, s->d_body_start);
-    hipLaunchKernelGGL(k_rigid_bodies, dim3((s->n_bodies + 63) / 64), dim3(64), 0, s->st, s->n_bodies, s->d_bodies,
+    hipLaunchKernelGGL(k_rigid_bodies, dim3(s->n_bodies), dim3(RB_CHUNK), 0, s->st, s->n_bodies, s->d_bodies,
                        s->d_body_start, s->d_body_members, S.pos, s->W.force, S.rel, s->P);
     hipLaunchKernelGGL(k_rigid_sync, g, dim3(BLOCK), 0, s->st, s->n, S, s->d_bodies);
     s->prof.end(s->st);
