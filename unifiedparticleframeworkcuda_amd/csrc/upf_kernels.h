@@ -64,8 +64,8 @@ struct Work {           // per-step arrays, slot order
     float4* fp;         // correction pressure force
     float4* xs;         // predicted position
     float* p;           // correction pressure of the running loop
-    float* pdiv;        // p / (rho0 * rho0), the neighbour term of the pressure force
-    float4* xpd;        // (current position, p / (rho0 * rho0)): one gather per pressure-force pair
+    float4* xpd;        // (current position, p / (rho0 * rho0) -- or -(rho0 psi) for an Akinci
+                        // boundary particle): one gather per pressure-force pair
     uint32_t* max_slots; // MAX_SLOTS x 32 uints: per-slot max of the predicted density (bits)
     float* rho;         // predicted density (PCISPH) / density (WCSPH)
     float* pres;        // WCSPH pressure

@@ -805,6 +805,16 @@ __global__ __launch_bounds__(64) void k_nbr_build_tile(StepArgs A, const float4*
 // CalculateExternalForcesStaticBoundariesPCISPH (:5198-5250) over the lists +
 // the first PredictPositionAndVelocity of the loop; every slot resets its
 // loop pressure.
+// w of a particle's pressure-force record at the start of a step.  Liquid:
+// p / rho0^2 = +0 until the density pass writes it (p >= 0, so never -0).
+// Other particles have p = 0; with Akinci boundaries they carry their mass
+// rho0 psi as -(rho0 psi), whose sign bit marks them (psi >= 0).
+template <int MODE>
+__device__ __forceinline__ float pressure_record_w(int type, float psi, float rho0) {
+    if (MODE == UPF_BOUNDARY_AKINCI && type != UPF_LIQUID) return -(rho0 * psi);
+    return 0.0f;
+}
+
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                       Ctl* ctl) {
@@ -821,8 +831,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     const float4 xi = S.pos[i];
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
-    W.pdiv[i] = 0.0f;
-    W.xpd[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
+    W.xpd[i] = make_float4(xi.x, xi.y, xi.z, pressure_record_w<MODE>(ti, S.psi[i], P.rest_density));
     W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     W.rho[i] = S.rho_last[i];
     if (ti != UPF_LIQUID) {
@@ -980,15 +989,11 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 const float delta = A.scene_of ? A.doms[sc].delta : P.density_error_factor;
                 const float p = W.p[i] + fmax_ref(err * delta, 0.0f);
                 W.p[i] = p;
+                // the force pass gathers current position and p / rho0^2 of a
+                // neighbour in one 16-byte load
                 const float pd = p / (P.rest_density * P.rest_density);
-                if (MODE == UPF_BOUNDARY_AKINCI) {
-                    W.pdiv[i] = pd;
-                } else {
-                    // the force pass gathers current position and p / rho0^2
-                    // of a neighbour in one 16-byte load
-                    const float4 x0 = S.pos[i];
-                    W.xpd[i] = make_float4(x0.x, x0.y, x0.z, pd);
-                }
+                const float4 x0 = S.pos[i];
+                W.xpd[i] = make_float4(x0.x, x0.y, x0.z, pd);
                 W.rho[i] = rho;
                 mine = rho;
             } else {
@@ -1145,14 +1150,11 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     // reference's `dist < h` test holds; the table bin is the one the list
     // build computed from the same positions (carried by the packed entry,
     // recomputed by the same expression for the coded lists)
-    // (Akinci boundaries also need the neighbour's type: position and p /
-    // rho0^2 come from their own arrays; else from one record)
+    // (one 16-byte record per neighbour: current position and p / rho0^2,
+    // or for an Akinci boundary neighbour its mass rho0 psi, negated)
     for_each_listed<CODED, G>(
         W, i, st, cnt,
-        [&](int j) {
-            if constexpr (MODE == UPF_BOUNDARY_AKINCI) return PosPd{S.pos[j], W.pdiv[j]};
-            else { const float4 r = at_u32(W.xpd, j); return PosPd{r, r.w}; }
-        },
+        [&](int j) { const float4 r = at_u32(W.xpd, j); return PosPd{r, r.w}; },
         [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
             const float4& xj = q.x;
             const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
@@ -1160,9 +1162,14 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
             const float g = lpg[bin];
             // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
             // precomputed per particle by the density pass (same division)
-            const float grad = pi_d + q.pd;
-            float mj = m;
-            if (MODE == UPF_BOUNDARY_AKINCI && ptype(xj) != UPF_LIQUID) mj = rho0 * S.psi[j];
+            float pd = q.pd, mj = m;
+            if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
+                if (__builtin_signbit(q.pd)) {  // boundary: p_j = 0, mass rho0 psi_j
+                    mj = -q.pd;
+                    pd = 0.0f;
+                }
+            }
+            const float grad = pi_d + pd;
             fx = fx - (((dx * g) * grad) * m) * mj;
             fy = fy - (((dy * g) * grad) * m) * mj;
             fz = fz - (((dz * g) * grad) * m) * mj;
@@ -1393,8 +1400,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
     const float4 xi = S.pos[i];
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
-    W.pdiv[i] = 0.0f;
-    W.xpd[i] = make_float4(xi.x, xi.y, xi.z, 0.0f);
+    W.xpd[i] = make_float4(xi.x, xi.y, xi.z, pressure_record_w<MODE>(ti, S.psi[i], P.rest_density));
     W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     W.rho[i] = S.rho_last[i];
     if (ti != UPF_LIQUID && ti != UPF_RIGID) {
@@ -2298,7 +2304,6 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.p, capacity));
     HIPCHK(s->alloc(&W.rho, capacity));
     HIPCHK(s->alloc(&W.pres, capacity));
-    HIPCHK(s->alloc(&W.pdiv, capacity));
     HIPCHK(s->alloc(&W.xpd, capacity));
     HIPCHK(s->alloc(&W.max_slots, nslots));
     HIPCHK(s->alloc(&W.ncount, capacity));
