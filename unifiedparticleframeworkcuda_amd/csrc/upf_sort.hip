@@ -4,11 +4,13 @@
 // RadixAddOffsetsAndShuffle in src/gpu/radixsort_kernel.cu:64,216,339) with a
 // design of its own:
 //   * 8-bit digits, one pass per byte of the key range that is in use;
-//   * tile = 256 threads x 8 items; each wave ranks 64 consecutive items per
+//   * tile = 256 threads x 4 items; each wave ranks 64 consecutive items per
 //     round with 8 ballots (lanes with equal digits form a peer mask), so the
 //     rank of an item is the number of equal digits before it -> stable;
 //   * per-tile digit counts (digit-major) are scanned by one workgroup per
-//     digit; the scatter adds digit base + tile offset + wave offset + rank.
+//     digit; the scatter orders its tile by digit in LDS first, so that the
+//     items of one digit leave as one contiguous (coalesced) run at digit
+//     base + tile offset.
 #include <hip/hip_runtime.h>
 
 #include "upf_sort.h"
@@ -16,7 +18,7 @@
 namespace upf {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_IPT = 8;
+constexpr int RS_IPT = 4;
 constexpr int RS_TILE = RS_THREADS * RS_IPT;
 constexpr int RS_WAVES = RS_THREADS / 64;
 
@@ -69,8 +71,10 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint32_t* __restr
                                                          uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                          int n, int shift, const uint32_t* __restrict__ hist,
                                                          int ntiles, const uint32_t* __restrict__ totals) {
-    __shared__ uint32_t digit_base[256];
+    __shared__ uint32_t digit_base[256];  // global start of each digit, then of this tile's run of it
+    __shared__ uint32_t local_start[256]; // start of each digit within the tile
     __shared__ uint32_t wcnt[RS_WAVES][256];
+    __shared__ uint32_t skey[RS_TILE], sval[RS_TILE];
     const int t = threadIdx.x;
     const int lane = t & 63;
     const int w = t >> 6;
@@ -91,7 +95,8 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint32_t* __restr
     __syncthreads();
 
     uint32_t key[RS_IPT], val[RS_IPT], rank[RS_IPT];
-    const int base = blockIdx.x * RS_TILE + w * 64 * RS_IPT;
+    const int tile0 = blockIdx.x * RS_TILE;
+    const int base = tile0 + w * 64 * RS_IPT;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int r = 0; r < RS_IPT; ++r) {
@@ -117,26 +122,47 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint32_t* __restr
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // exclusive prefix over the waves for every digit
-    {
-        uint32_t s = 0;
+    // per digit: exclusive prefix over the waves, and the tile's count
+    uint32_t tile_cnt = 0;
 #pragma unroll
-        for (int k = 0; k < RS_WAVES; ++k) {
-            const uint32_t c = wcnt[k][t];
-            wcnt[k][t] = s;
-            s += c;
-        }
+    for (int k = 0; k < RS_WAVES; ++k) {
+        const uint32_t c = wcnt[k][t];
+        wcnt[k][t] = tile_cnt;
+        tile_cnt += c;
     }
+    // exclusive scan of the tile's digit counts: where each digit starts in the tile
+    local_start[t] = tile_cnt;
     __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = t >= off ? local_start[t - off] : 0u;
+        __syncthreads();
+        local_start[t] += v;
+        __syncthreads();
+    }
+    const uint32_t ls = local_start[t] - tile_cnt;
+    __syncthreads();
+    local_start[t] = ls;
+    // destination of the tile's run of digit t, minus the run's local start
+    digit_base[t] = digit_base[t] + hist[t * ntiles + blockIdx.x] - ls;
+    __syncthreads();
+    // the tile in digit order (stable: wave, round, lane order within a digit)
 #pragma unroll
     for (int r = 0; r < RS_IPT; ++r) {
         const int idx = base + r * 64 + lane;
         if (idx < n) {
             const uint32_t d = (key[r] >> shift) & 255u;
-            const uint32_t dst = digit_base[d] + hist[d * ntiles + blockIdx.x] + wcnt[w][d] + rank[r];
-            kout[dst] = key[r];
-            vout[dst] = val[r];
+            const uint32_t lp = local_start[d] + wcnt[w][d] + rank[r];
+            skey[lp] = key[r];
+            sval[lp] = val[r];
         }
+    }
+    __syncthreads();
+    const int cnt = min(RS_TILE, n - tile0);
+    for (int q = t; q < cnt; q += RS_THREADS) {
+        const uint32_t k = skey[q];
+        const uint32_t dst = digit_base[(k >> shift) & 255u] + (uint32_t)q;
+        kout[dst] = k;
+        vout[dst] = sval[q];
     }
 }
 
