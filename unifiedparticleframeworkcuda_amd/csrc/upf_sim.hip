@@ -990,10 +990,10 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 const float p = W.p[i] + fmax_ref(err * delta, 0.0f);
                 W.p[i] = p;
                 // the force pass gathers current position and p / rho0^2 of a
-                // neighbour in one 16-byte load
+                // neighbour in one 16-byte load; the position part was written
+                // at the start of the step
                 const float pd = p / (P.rest_density * P.rest_density);
-                const float4 x0 = S.pos[i];
-                W.xpd[i] = make_float4(x0.x, x0.y, x0.z, pd);
+                reinterpret_cast<float*>(W.xpd)[4 * (size_t)i + 3] = pd;
                 W.rho[i] = rho;
                 mine = rho;
             } else {
