@@ -67,12 +67,14 @@ def measured_traffic(cls):
         return None
     in_table = False
     for line in open(PROFILE):
-        if line.startswith("| kernel | dispatches"):
-            in_table = True
+        if line.startswith("| kernel |"):
+            # the traffic table: | kernel | dispatches | fetch MB | write MB | total MB |
+            in_table = "total MB" in line
             continue
         if in_table and line.startswith("| " + pre):
             cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            return float(cells[4]) * 1e6
+            if len(cells) == 5:
+                return float(cells[4]) * 1e6
     return None
 
 

@@ -65,7 +65,9 @@ struct Work {           // per-step arrays, slot order
     float4* xs;         // predicted position
     float* p;           // correction pressure of the running loop
     float4* xpd;        // (current position, p / (rho0 * rho0) -- or -(rho0 psi) for an Akinci
-                        // boundary particle): one gather per pressure-force pair
+                        // boundary particle): one gather per pressure-force pair; WCSPH:
+                        // (position, m / rho), w < 0 for a non-fluid particle
+    float4* vpg;        // WCSPH: (velocity, p / rho^2) of a particle, the force pass's second record
     uint32_t* max_slots; // MAX_SLOTS x 32 uints: per-slot max of the predicted density (bits)
     float* rho;         // predicted density (PCISPH) / density (WCSPH)
     float* pres;        // WCSPH pressure

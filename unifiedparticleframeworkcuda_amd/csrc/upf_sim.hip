@@ -1114,6 +1114,10 @@ struct PosPd {
     float pd;
 };
 
+struct PosVel {
+    float4 x, v;
+};
+
 template <int MODE, int CODED, int G>
 __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                  Ctl* ctl) {
@@ -1249,6 +1253,7 @@ __global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Wo
         W.rho[i] = 0.0f;
         W.pres[i] = 0.0f;
         W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        W.xpd[i] = make_float4(xi.x, xi.y, xi.z, -1.0f);  // w < 0: not a fluid neighbour
         return;
     }
     const float m = P.initial_mass;
@@ -1259,8 +1264,15 @@ __global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Wo
     const float b = P.rest_density * P.gas_const_wcsph / gamma;
     const float frac = rho / P.rest_density;
     const float pg = powf(frac, gamma);
+    const float pres = fmax_ref(0.0f, b * (pg - 1.0f));
     W.rho[i] = rho;
-    W.pres[i] = fmax_ref(0.0f, b * (pg - 1.0f));
+    W.pres[i] = pres;
+    // what the force pass needs of this particle as a neighbour, in two
+    // 16-byte records: (position, m / rho) and (velocity, p / rho^2) -- the
+    // same expressions it would evaluate per pair
+    const float4 vi = S.vel[i];
+    W.xpd[i] = make_float4(xi.x, xi.y, xi.z, m / rho);
+    W.vpg[i] = make_float4(vi.x, vi.y, vi.z, pres / (rho * rho));
 }
 
 // CalculateLiquidParticleForces (:3009-3083) + gravity (:2822-2824)
@@ -1284,26 +1296,25 @@ __global__ __launch_bounds__(BLOCK) void k_force_wcsph(StepArgs A, Slots S, Work
     const float4 vi = S.vel[i];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
-    for (int k = 0; k < cnt; ++k) {
-        const uint32_t e = W.nbr[(size_t)k * A.stride + i];
-        const int j = (int)(e >> NB_SLOT_SHIFT);
-        const float4 xj = S.pos[j];
-        if (ptype(xj) != UPF_LIQUID) continue;
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const float kv = lvl[e & NB_BIN_MASK];
-        const float rho_j = W.rho[j];
-        const float nVol = m / rho_j;
-        const float g = lpg[e & NB_BIN_MASK];
-        const float grad = p_i * invD * invD + W.pres[j] / (rho_j * rho_j);
-        fx = fx - (((dx * g) * grad) * m) * m;
-        fy = fy - (((dy * g) * grad) * m) * m;
-        fz = fz - (((dz * g) * grad) * m) * m;
-        const float4 vj = S.vel[j];
-        const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-        fx = fx - vx * pVol * nVol * P.visc_const * kv;
-        fy = fy - vy * pVol * nVol * P.visc_const * kv;
-        fz = fz - vz * pVol * nVol * P.visc_const * kv;
-    }
+    // two 16-byte records per neighbour, four neighbours' loads in flight
+    for_each_listed<0, 4>(
+        W, i, (size_t)A.stride, cnt, [&](int j) { return PosVel{at_u32(W.xpd, j), at_u32(W.vpg, j)}; },
+        [&](int, uint32_t e, const PosVel& q) __attribute__((always_inline)) {
+            const float4& xj = q.x;
+            if (xj.w < 0.0f) return;  // not a fluid neighbour
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float kv = lvl[e & NB_BIN_MASK];
+            const float nVol = xj.w;  // m / rho_j
+            const float g = lpg[e & NB_BIN_MASK];
+            const float grad = p_i * invD * invD + q.v.w;  // + p_j / rho_j^2
+            fx = fx - (((dx * g) * grad) * m) * m;
+            fy = fy - (((dy * g) * grad) * m) * m;
+            fz = fz - (((dz * g) * grad) * m) * m;
+            const float vx = vi.x - q.v.x, vy = vi.y - q.v.y, vz = vi.z - q.v.z;
+            fx = fx - vx * pVol * nVol * P.visc_const * kv;
+            fy = fy - vy * pVol * nVol * P.visc_const * kv;
+            fz = fz - vz * pVol * nVol * P.visc_const * kv;
+        });
     if (P.add_boundary_force) box_force_static(P, dom_at(A, i), xi, fx, fy, fz);
     fy = fy - P.initial_mass * P.gravity;
     W.force[i] = make_float4(fx, fy, fz, 0.0f);
@@ -2305,6 +2316,7 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.rho, capacity));
     HIPCHK(s->alloc(&W.pres, capacity));
     HIPCHK(s->alloc(&W.xpd, capacity));
+    HIPCHK(s->alloc(&W.vpg, capacity));
     HIPCHK(s->alloc(&W.max_slots, nslots));
     HIPCHK(s->alloc(&W.ncount, capacity));
     HIPCHK(s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors));
