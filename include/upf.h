@@ -41,6 +41,7 @@ extern "C" {
 #define UPF_ERR_NO_DEVICE       7  /* no HIP device */
 #define UPF_ERR_ISOLATED        8  /* boundary particle without boundary neighbours (abort at src/cpu/UnifiedPhysics.cpp:2739-2740) */
 #define UPF_ERR_STATE           9  /* call order violated (e.g. step before upload) */
+#define UPF_ERR_IO             10  /* file missing or unreadable (ObjectLoader::FileNotFound, src/cpu/ObjectLoader.h:36-46) */
 
 /* ------------------------------------------------------ particle types */
 #define UPF_RIGID   0
@@ -340,6 +341,33 @@ int upf_adjoint_backward(upf_adjoint* a, const float* dl_dpos4, const float* dl_
 /* Device time of the last forward / backward call (HIP events on the
  * rollout's stream) and the device memory the rollout holds.              */
 int upf_adjoint_timing(upf_adjoint* a, float* forward_ms, float* backward_ms, size_t* device_bytes);
+
+/* -------------------------------------------- OBJ model -> particles */
+/* The scene-input path of the reference's rigid objects
+ * (Scene::CreateFullParticleObject, src/sceneConfigure/Scene.cpp:797-824).
+ *
+ * upf_obj_read replaces ObjLoader::Load + LoadFaces (src/cpu/ObjLoader.cpp:18-258):
+ * the triangles of a Wavefront OBJ file in file order, 9 floats each (three
+ * corners), read as the reference reads them -- lines through a 256-byte
+ * buffer, the last line of the file read twice (its end-of-file quirk), faces
+ * as v/vt/vn triples.  faces9 == NULL: count only.  A missing file is
+ * UPF_ERR_IO; a file without faces or vertices has no triangles (the
+ * reference's FileError); a face or vertex line the reference would read
+ * into undefined values is UPF_ERR_ARG.
+ *
+ * upf_mesh_fill replaces ObjectLoader::LoadParticles (src/cpu/ObjectLoader.cpp:8-97)
+ * with its Polygon tests (src/cpu/Polygon.cpp): the lattice points of the
+ * model's box (spacing / scale apart, odd x planes shifted half a step in y
+ * and z) that the segment from outside the box reaches through an odd number
+ * of faces, or that lie on a face, as (point - box minimum) * scale, in the
+ * reference's loop order (x, then y, then z).  The box of every face starts
+ * at the origin, as the reference's does.  Runs on the GPU (one thread per
+ * lattice point).  pos4 == NULL: count only (n_out); cap < count:
+ * UPF_ERR_RANGE with n_out set.  The caller adds the object's offset
+ * (CreateFullParticleObject: position += offset).                          */
+int upf_obj_read(const char* path, float* faces9, int cap, int* n_faces);
+int upf_mesh_fill(const float* faces9, int n_faces, const float scale[3], float spacing, float* pos4, int cap,
+                  int* n_out);
 
 /* ------------------------------------------------------ launch tuning */
 /* Launch shapes that change speed only, never results (no reference

@@ -1,6 +1,7 @@
 // ref_driver.cpp -- test-only C entry points over the reference's own
 // translation units (WeightingKernel.cpp, zIndex.cpp, UnifiedConstants.cpp,
-// RigidBody.cpp, Matrix3x3.cpp, UnifiedParticle.cpp), compiled from their
+// RigidBody.cpp, Matrix3x3.cpp, UnifiedParticle.cpp, and the OBJ model loader
+// ObjLoader.cpp, ObjectLoader.cpp, Polygon.cpp, Face.cpp, Mesh.cpp), compiled from their
 // place under the reference tree by oracle/Makefile into oracle/_ref/.
 // It pins the oracle's restatement of those files; nothing here is shipped.
 
@@ -14,6 +15,7 @@
 #define private public
 #include "RigidBody.h"
 #undef private
+#include "ObjLoader.h"
 #include "UnifiedConstants.h"
 #include "UnifiedParticle.h"
 #include "WeightingKernel.h"
@@ -159,6 +161,20 @@ int ref_rigid_step(float* st, int n, const float* ppos, const float* pforce, con
             st[37 + 3 * r + c] = b.inverted_inertia_world_.elements[r][c];
         }
     return 0;
+}
+
+// ObjLoader::GetInstance().LoadParticles(path, sx, sy, sz, spacing, template)
+// (src/cpu/ObjectLoader.cpp:8-97): the particle positions, 3 floats each, up
+// to cap of them; returns the number of particles.
+int ref_obj_particles(const char* path, float sx, float sy, float sz, float spacing, float* out3, int cap) {
+    Quiet q;
+    UnifiedParticle templ(0.0f, 0.0f, 0.0f, 1000.0, 1.0, 0);
+    std::vector<UnifiedParticle>* v = ObjLoader::GetInstance().LoadParticles(path, sx, sy, sz, spacing, templ);
+    const int n = (int)v->size();
+    for (int i = 0; i < n && i < cap; ++i)
+        for (int a = 0; a < 3; ++a) out3[3 * i + a] = (*v)[i].position_[a];
+    delete v;
+    return n;
 }
 
 }  // extern "C"

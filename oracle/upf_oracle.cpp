@@ -954,6 +954,210 @@ static int set_domain(upf_params* p, const float vmax[3], const float cmin[3], c
     return (float)b >= bsz ? UPF_OK : UPF_ERR_ARG;
 }
 
+// ----------------------------------------------------- OBJ model -> particles
+// ObjLoader::Load (src/cpu/ObjLoader.cpp:61-258): two passes over the file
+// with a 256-byte fgets buffer under while(!feof(fp)); the fgets that meets
+// the end of the file leaves the buffer as it was, so the last line is seen
+// twice.  Faces are "f a/b/c d/e/f g/h/i" (1-based vertex index first); the
+// triangles' vertex coordinates come out in file order.  A file without
+// faces or vertices is the reference's FileError: no faces.
+struct OV3 {
+    float x, y, z;
+};
+
+static int obj_read(const char* path, std::vector<OV3>& tri) {
+    tri.clear();
+    FILE* fp = std::fopen(path, "r");
+    if (!fp) return UPF_ERR_IO;
+    char buf[256] = {0};
+    long nv = 0, nt = 0;
+    while (!std::feof(fp)) {
+        if (!std::fgets(buf, sizeof(buf), fp)) {}  // buffer unchanged at the end
+        if (std::strncmp("vt ", buf, 3) == 0) {
+        } else if (std::strncmp("vn ", buf, 3) == 0) {
+        } else if (std::strncmp("v ", buf, 2) == 0) {
+            ++nv;
+        } else if (std::strncmp("f ", buf, 2) == 0) {
+            ++nt;
+        }
+    }
+    std::fclose(fp);
+    if (nt == 0 || nv == 0) return UPF_OK;
+    std::vector<OV3> verts;
+    std::vector<int> idx;
+    fp = std::fopen(path, "r");
+    if (!fp) return UPF_ERR_IO;
+    std::memset(buf, 0, sizeof(buf));
+    int rc = UPF_OK;
+    while (!std::feof(fp)) {
+        if (!std::fgets(buf, sizeof(buf), fp)) {}
+        if (std::strncmp("vn ", buf, 3) == 0 || std::strncmp("vt ", buf, 3) == 0) continue;
+        if (std::strncmp("v ", buf, 2) == 0) {
+            OV3 v{0.f, 0.f, 0.f};
+            if (std::sscanf(buf + 2, "%f%f%f", &v.x, &v.y, &v.z) != 3) rc = UPF_ERR_ARG;
+            verts.push_back(v);
+        } else if (std::strncmp("f ", buf, 2) == 0) {
+            int f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            const int got = std::sscanf(buf + 2, "%d/%d/%d %d/%d/%d %d/%d/%d", &f[0], &f[1], &f[2], &f[3], &f[4],
+                                        &f[5], &f[6], &f[7], &f[8]);
+            if (got < 7) rc = UPF_ERR_ARG;  // the reference reads garbage indices here
+            idx.push_back(f[0] - 1);
+            idx.push_back(f[3] - 1);
+            idx.push_back(f[6] - 1);
+        }
+    }
+    std::fclose(fp);
+    if (rc) return rc;
+    for (int k : idx) {
+        if (k < 0 || k >= (int)verts.size()) return UPF_ERR_ARG;
+        tri.push_back(verts[k]);
+    }
+    return UPF_OK;
+}
+
+// Polygon (src/cpu/Polygon.cpp) of three vertices: bounding box from (0,0,0)
+// (the constructor's initial box), normal set when the third vertex arrives
+// and normalised with vmml's normalize (1/l in double).
+struct Poly {
+    OV3 v[3];
+    OV3 bmin, bmax, n;
+};
+
+static float dot3(const OV3& a, const OV3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static float comp(const OV3& a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+static Poly make_poly(const OV3* v) {
+    Poly p;
+    p.bmin = {0.f, 0.f, 0.f};
+    p.bmax = {0.f, 0.f, 0.f};
+    for (int k = 0; k < 3; ++k) {
+        float* mn = &p.bmin.x;
+        float* mx = &p.bmax.x;
+        const float* c = &v[k].x;
+        for (int i = 0; i < 3; ++i) {
+            mn[i] = std::min(c[i], mn[i]);
+            mx[i] = std::max(c[i], mx[i]);
+        }
+        if (k == 2) {
+            const OV3 a{v[0].x - v[1].x, v[0].y - v[1].y, v[0].z - v[1].z};
+            const OV3 b{v[2].x - v[1].x, v[2].y - v[1].y, v[2].z - v[1].z};
+            OV3 n{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+            const float l = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+            if (l != 0.0f) {
+                const float ll = (float)(1.0 / (double)l);
+                n.x *= ll;
+                n.y *= ll;
+                n.z *= ll;
+            }
+            p.n = n;
+        }
+        p.v[k] = v[k];
+    }
+    return p;
+}
+
+// Polygon::ArePointsOnBothSidesOfTheLine (Polygon.cpp:73-103)
+static bool both_sides(const Poly& p, const OV3& ls, const OV3& le, const OV3& p1, const OV3& p2) {
+    int fi, si;
+    if (p.n.x == 0) {
+        fi = 0;
+        si = p.n.y == 0 ? 1 : 2;
+    } else {
+        fi = 1;
+        si = 2;
+    }
+    const float dxES = comp(le, fi) - comp(ls, fi);
+    const float dx1E = comp(p1, fi) - comp(le, fi);
+    const float dySE = comp(ls, si) - comp(le, si);
+    const float dyE1 = comp(le, si) - comp(p1, si);
+    const bool ccw1 = (dySE * dx1E) > (dyE1 * dxES);
+    const float dx2E = comp(p2, fi) - comp(le, fi);
+    const float dyE2 = comp(le, si) - comp(p2, si);
+    const bool ccw2 = (dySE * dx2E) > (dyE2 * dxES);
+    return ccw1 != ccw2;
+}
+
+// Polygon::IsPointInPlaneInside (Polygon.cpp:118-137)
+static bool inside_in_plane(const Poly& p, const OV3& pt) {
+    bool inside = false;
+    const OV3 out{p.bmin.x - 1.0f, p.bmin.y - 1.0f, p.bmin.z - 1.0f};
+    for (int k = 0; k < 3; ++k) {
+        const OV3& a = p.v[k];
+        const OV3& b = p.v[k == 2 ? 0 : k + 1];
+        if (both_sides(p, out, pt, a, b) && both_sides(p, a, b, out, pt)) inside = !inside;
+    }
+    return inside;
+}
+
+// Polygon::DoesLineIntersect (Polygon.cpp:47-70)
+static bool line_intersects(const Poly& p, const OV3& s, const OV3& e) {
+    const OV3 dv{e.x - s.x, e.y - s.y, e.z - s.z};
+    const float den = dot3(dv, p.n);
+    if (std::fabs(den) < 0.000001f) return false;
+    const float d = -dot3(p.v[0], p.n);
+    const float dof = -(d + dot3(s, p.n)) / den;
+    if (dof < 0 || dof > 1) return false;
+    const OV3 x{s.x + dv.x * dof, s.y + dv.y * dof, s.z + dv.z * dof};
+    return inside_in_plane(p, x);
+}
+
+// Polygon::DoesPointTouch (Polygon.cpp:106-115)
+static bool point_touches(const Poly& p, const OV3& pt) {
+    const float d = -dot3(p.v[0], p.n);
+    const float eq = dot3(pt, p.n) + d;
+    if (std::fabs(eq) > 0.000001f) return false;
+    return inside_in_plane(p, pt);
+}
+
+// ObjectLoader::LoadParticles (src/cpu/ObjectLoader.cpp:8-97): lattice
+// points (staggered by half a spacing in y and z on odd x) whose segment from
+// outside the box crosses the faces an odd number of times, or that touch a
+// face; positions relative to the box minimum, scaled.
+static void mesh_fill(const std::vector<OV3>& tri, const float* sc, float spacing, std::vector<OV3>& out) {
+    out.clear();
+    const size_t nf = tri.size() / 3;
+    if (nf == 0) return;
+    std::vector<Poly> faces(nf);
+    for (size_t f = 0; f < nf; ++f) faces[f] = make_poly(&tri[3 * f]);
+    OV3 mn = faces[0].bmin, mx = faces[0].bmax;
+    for (size_t f = 0; f < nf; ++f) {
+        float* a = &mn.x;
+        float* b = &mx.x;
+        const float* c = &faces[f].bmin.x;
+        const float* d = &faces[f].bmax.x;
+        for (int i = 0; i < 3; ++i) {
+            a[i] = std::min(a[i], c[i]);
+            b[i] = std::max(b[i], d[i]);
+        }
+    }
+    const float xs = (mx.x - mn.x) * sc[0] / spacing;
+    const float ys = (mx.y - mn.y) * sc[1] / spacing;
+    const float zs = (mx.z - mn.z) * sc[2] / spacing;
+    const OV3 outside{mn.x - spacing, mn.y - spacing, mn.z - spacing};
+    OV3 cur = mn;
+    for (int x = 0; x <= xs; ++x) {
+        cur.y = mn.y + (x % 2) * spacing / (2 * sc[1]);
+        for (int y = 0; y <= ys; ++y) {
+            cur.z = mn.z + (x % 2) * spacing / (2 * sc[2]);
+            for (int z = 0; z <= zs; ++z) {
+                bool add = false;
+                for (size_t f = 0; f < nf; ++f) {
+                    if (line_intersects(faces[f], outside, cur)) {
+                        add = !add;
+                    } else if (point_touches(faces[f], cur)) {
+                        add = true;
+                        break;
+                    }
+                }
+                if (add) out.push_back({(cur.x - mn.x) * sc[0], (cur.y - mn.y) * sc[1], (cur.z - mn.z) * sc[2]});
+                cur.z += spacing / sc[2];
+            }
+            cur.y += spacing / sc[1];
+        }
+        cur.x += spacing / sc[0];
+    }
+}
+
 }  // namespace orc
 
 // =================================================================== C ABI
@@ -1629,4 +1833,33 @@ extern "C" int orc_diff_kernels(float h, int n, const double* r, double* W, doub
         L[i] = k.L(r[i]);
     }
     return 0;
+}
+
+// OBJ model -> particles (test infrastructure: the oracle of upf_obj_read /
+// upf_mesh_fill)
+extern "C" int orc_obj_read(const char* path, float* faces9, int cap, int* n_faces) {
+    std::vector<orc::OV3> tri;
+    const int rc = orc::obj_read(path, tri);
+    if (rc) return rc;
+    const int n = (int)(tri.size() / 3);
+    if (n_faces) *n_faces = n;
+    if (faces9) {
+        if (cap < n) return UPF_ERR_RANGE;
+        std::memcpy(faces9, tri.data(), tri.size() * sizeof(orc::OV3));
+    }
+    return UPF_OK;
+}
+
+extern "C" int orc_mesh_fill(const float* faces9, int n_faces, const float* scale, float spacing, float* pos3, int cap,
+                             int* n_out) {
+    std::vector<orc::OV3> tri((size_t)n_faces * 3);
+    if (n_faces > 0) std::memcpy(tri.data(), faces9, tri.size() * sizeof(orc::OV3));
+    std::vector<orc::OV3> out;
+    orc::mesh_fill(tri, scale, spacing, out);
+    *n_out = (int)out.size();
+    if (pos3) {
+        if (cap < (int)out.size()) return UPF_ERR_RANGE;
+        std::memcpy(pos3, out.data(), out.size() * sizeof(orc::OV3));
+    }
+    return UPF_OK;
 }

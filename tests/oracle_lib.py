@@ -44,6 +44,8 @@ def _bind(lib):
     pd = C.POINTER(C.c_double)
     lib.orc_diff_run.argtypes = [pp, C.c_int, C.c_int, pd, pd, pi, C.c_int, C.c_int, pd, pd, pd, pd, pd, pd]
     lib.orc_diff_kernels.argtypes = [f, C.c_int, pd, pd, pd, pd, pd, pd]
+    lib.orc_obj_read.argtypes = [C.c_char_p, pf, C.c_int, pi]
+    lib.orc_mesh_fill.argtypes = [pf, C.c_int, pf, f, pf, C.c_int, pi]
     return lib
 
 
@@ -73,8 +75,50 @@ def ref_lib():
         lib.ref_ztable.argtypes = [pu]
         lib.ref_box_query.argtypes = [C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), pu, C.c_int, pu]
         lib.ref_rigid_step.argtypes = [pf, C.c_int, pf, pf, pf, C.c_float, C.c_float, C.c_float]
+        lib.ref_obj_particles.argtypes = [C.c_char_p, C.c_float, C.c_float, C.c_float, C.c_float, pf, C.c_int]
         _libs["ref"] = lib
     return _libs["ref"]
+
+
+def obj_read(path):
+    """The oracle's OBJ reader: (n, 3, 3) float32 triangles (raises on error codes)."""
+    L = oracle()
+    n = C.c_int()
+    rc = L.orc_obj_read(os.fsencode(path), None, 0, C.byref(n))
+    if rc:
+        raise ValueError("orc_obj_read: %d" % rc)
+    out = np.zeros((n.value, 3, 3), np.float32)
+    if n.value:
+        L.orc_obj_read(os.fsencode(path), ptr(out), n.value, C.byref(n))
+    return out
+
+
+def mesh_fill(faces, scale, spacing):
+    """The oracle's ObjectLoader::LoadParticles: (m, 3) float32 positions."""
+    L = oracle()
+    f = np.ascontiguousarray(faces, np.float32).reshape(-1, 9)
+    sc = (C.c_float * 3)(*[float(v) for v in np.broadcast_to(np.asarray(scale, np.float32), (3,))])
+    n = C.c_int()
+    fp = ptr(f) if len(f) else None
+    L.orc_mesh_fill(fp, len(f), sc, float(spacing), None, 0, C.byref(n))
+    out = np.zeros((n.value, 3), np.float32)
+    if n.value:
+        L.orc_mesh_fill(fp, len(f), sc, float(spacing), ptr(out), n.value, C.byref(n))
+    return out
+
+
+def ref_obj_particles(path, scale, spacing):
+    """The reference's ObjLoader + LoadParticles compiled into oracle/_ref
+    (None without the reference tree): (m, 3) float32."""
+    L = ref_lib()
+    if L is None:
+        return None
+    sc = [float(v) for v in np.broadcast_to(np.asarray(scale, np.float32), (3,))]
+    n = L.ref_obj_particles(os.fsencode(path), sc[0], sc[1], sc[2], float(spacing), None, 0)
+    out = np.zeros((n, 3), np.float32)
+    if n:
+        L.ref_obj_particles(os.fsencode(path), sc[0], sc[1], sc[2], float(spacing), ptr(out), n)
+    return out
 
 
 def params(spacing=0.01, vmax=None, cmin=None, cmax=None):

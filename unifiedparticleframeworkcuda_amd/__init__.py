@@ -73,6 +73,8 @@ def lib():
         L.upf_profile_reset.argtypes = [vp]
         L.upf_profile_read.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
         L.upf_profile_name.restype = C.c_char_p
+        L.upf_obj_read.argtypes = [C.c_char_p, pf, C.c_int, pi]
+        L.upf_mesh_fill.argtypes = [pf, C.c_int, pf, C.c_float, pf, C.c_int, pi]
         for name, args in (("upf_upload_bodies", [vp, C.c_int, C.POINTER(Body), pi, pf]),
                            ("upf_download_bodies", [vp, C.POINTER(Body)])):
             if hasattr(L, name):
@@ -95,6 +97,42 @@ def make_params(spacing=0.01, dt=None):
     if dt is not None:
         p.dt = dt
     return p
+
+
+def load_obj(path):
+    """Triangles of a Wavefront OBJ file as the reference's ObjLoader reads
+    them (src/cpu/ObjLoader.cpp:18-258): (n, 3, 3) float32, file order."""
+    L = lib()
+    n = C.c_int()
+    check("upf_obj_read", L.upf_obj_read(os.fsencode(path), None, 0, C.byref(n)))
+    out = np.zeros((n.value, 3, 3), np.float32)
+    if n.value:
+        check("upf_obj_read", L.upf_obj_read(os.fsencode(path), ptr(out), n.value, C.byref(n)))
+    return out
+
+
+def mesh_fill(faces, scale, spacing):
+    """ObjectLoader::LoadParticles (src/cpu/ObjectLoader.cpp:8-97) on the GPU:
+    the lattice points inside the triangles `faces` ((n, 3, 3)) or on them,
+    (point - box minimum) * scale, in the reference's order; (m, 3) float32."""
+    L = lib()
+    f = np.ascontiguousarray(faces, np.float32).reshape(-1, 9)
+    sc = (C.c_float * 3)(*[float(v) for v in np.broadcast_to(np.asarray(scale, np.float32), (3,))])
+    n = C.c_int()
+    fp = ptr(f) if len(f) else None
+    check("upf_mesh_fill", L.upf_mesh_fill(fp, len(f), sc, float(spacing), None, 0, C.byref(n)))
+    out = np.zeros((n.value, 4), np.float32)
+    if n.value:
+        check("upf_mesh_fill", L.upf_mesh_fill(fp, len(f), sc, float(spacing), ptr(out), n.value, C.byref(n)))
+    return out[:, :3].copy()
+
+
+def model_particles(path, scale, spacing, offset=(0.0, 0.0, 0.0)):
+    """Particle positions of an OBJ model placed as Scene::CreateFullParticleObject
+    places them (src/sceneConfigure/Scene.cpp:797-824): the filled lattice plus
+    the object's offset (float32 additions)."""
+    pos = mesh_fill(load_obj(path), scale, spacing)
+    return pos + np.asarray(offset, np.float32)
 
 
 def luts(p):

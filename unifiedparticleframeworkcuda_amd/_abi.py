@@ -15,11 +15,12 @@ UPF_ERR_DELTA = 6
 UPF_ERR_NO_DEVICE = 7
 UPF_ERR_ISOLATED = 8
 UPF_ERR_STATE = 9
+UPF_ERR_IO = 10
 
 ERROR_NAMES = {
     0: "UPF_OK", 1: "UPF_ERR_HIP", 2: "UPF_ERR_ARG", 3: "UPF_ERR_RANGE",
     4: "UPF_ERR_NAN", 5: "UPF_ERR_NBR_OVERFLOW", 6: "UPF_ERR_DELTA",
-    7: "UPF_ERR_NO_DEVICE", 8: "UPF_ERR_ISOLATED", 9: "UPF_ERR_STATE",
+    7: "UPF_ERR_NO_DEVICE", 8: "UPF_ERR_ISOLATED", 9: "UPF_ERR_STATE", 10: "UPF_ERR_IO",
 }
 
 UPF_RIGID = 0
