@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 from ._abi import (Params, DamBreakDesc, RigidSceneDesc, Body, Tuning, ptr, ERROR_NAMES,  # noqa: F401
-                   UPF_OK, UPF_LIQUID, UPF_FROZEN, UPF_RIGID, UPF_PCISPH_STATIC,
+                   UPF_OK, UPF_ERR_RANGE, UPF_LIQUID, UPF_FROZEN, UPF_RIGID, UPF_PCISPH_STATIC,
                    UPF_WCSPH_STATIC, UPF_PCISPH_RIGID, UPF_BOUNDARY_CPU_REF, UPF_BOUNDARY_AKINCI)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -117,14 +117,25 @@ def mesh_fill(faces, scale, spacing):
     (point - box minimum) * scale, in the reference's order; (m, 3) float32."""
     L = lib()
     f = np.ascontiguousarray(faces, np.float32).reshape(-1, 9)
-    sc = (C.c_float * 3)(*[float(v) for v in np.broadcast_to(np.asarray(scale, np.float32), (3,))])
+    s3 = np.broadcast_to(np.asarray(scale, np.float32), (3,)).copy()
+    sc = (C.c_float * 3)(*[float(v) for v in s3])
     n = C.c_int()
-    fp = ptr(f) if len(f) else None
-    check("upf_mesh_fill", L.upf_mesh_fill(fp, len(f), sc, float(spacing), None, 0, C.byref(n)))
-    out = np.zeros((n.value, 4), np.float32)
-    if n.value:
-        check("upf_mesh_fill", L.upf_mesh_fill(fp, len(f), sc, float(spacing), ptr(out), n.value, C.byref(n)))
-    return out[:, :3].copy()
+    if len(f) == 0:
+        return np.zeros((0, 3), np.float32)
+    # the lattice holds at most as many particles as it has points: the box
+    # of the faces (from the origin) in steps of spacing / scale, counted as
+    # the reference's loops count them
+    v = f.reshape(-1, 3)
+    steps = (np.maximum(v.max(0), 0).astype(np.float32) - np.minimum(v.min(0), 0).astype(np.float32)) * s3 \
+        / np.float32(spacing)
+    cap = int(np.prod(np.floor(np.maximum(steps, -1)) + 1)) if np.all(steps >= 0) else 0
+    out = np.zeros((max(cap, 1), 4), np.float32)
+    rc = L.upf_mesh_fill(ptr(f), len(f), sc, float(spacing), ptr(out), cap, C.byref(n))
+    if rc == UPF_ERR_RANGE:  # not expected: count, then fill
+        out = np.zeros((n.value, 4), np.float32)
+        rc = L.upf_mesh_fill(ptr(f), len(f), sc, float(spacing), ptr(out), n.value, C.byref(n))
+    check("upf_mesh_fill", rc)
+    return out[:n.value, :3].copy()
 
 
 def model_particles(path, scale, spacing, offset=(0.0, 0.0, 0.0)):
