@@ -347,6 +347,58 @@ def run_model_fill(args):
     return out
 
 
+def run_state_io(args):
+    """Particle state files of the 1M dam break after 5 steps (the reference's
+    UnifiedIO, src/cpu/UnifiedIO.cpp): the restart file written from the
+    device (upf_save_text: download + formatting on up to 16 host threads),
+    read back (upf_read_restart) and uploaded (upf_upload_restart), and the
+    liquid dump SaveLiquidParticleInfoFull writes.  CPU leg: the oracle's
+    iostream writer and reader (one thread, the reference's formatting) on
+    the same particles."""
+    import filecmp
+    import tempfile
+    import unifiedparticleframeworkcuda_amd as upf
+    nx, ny, nz = args.lattice
+    p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz)
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    sim.step(5, upf.UPF_PCISPH_STATIC)
+    n = len(typ)
+    with tempfile.TemporaryDirectory() as d:
+        rpath, lpath = os.path.join(d, "restart.txt"), os.path.join(d, "liquid.txt")
+        t0 = time.perf_counter()
+        sim.save_text(rpath, upf.UPF_TEXT_RESTART)
+        t_save = time.perf_counter() - t0
+        size = os.path.getsize(rpath)
+        t0 = time.perf_counter()
+        st = upf.read_restart(rpath)
+        t_read = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sim.upload_restart(st)
+        t_up = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sim.save_text(lpath, upf.UPF_TEXT_LIQUID)
+        t_liq = time.perf_counter() - t0
+        out = {"particles": n, "restart_bytes": size, "save_restart_ms": t_save * 1e3, "read_restart_ms": t_read * 1e3,
+               "upload_restart_ms": t_up * 1e3, "save_liquid_ms": t_liq * 1e3,
+               "liquid_bytes": os.path.getsize(lpath), "save_restart_MBps": size / t_save / 1e6,
+               "read_restart_MBps": size / t_read / 1e6}
+        if not args.no_cpu:
+            import oracle_lib as ol
+            opath = os.path.join(d, "oracle.txt")
+            t0 = time.perf_counter()
+            ol.write_text(opath, upf.UPF_TEXT_RESTART, st["pos"], st["type"], st["vel"], st["corr_pressure"],
+                          st["predicted_density"])
+            t_ow = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            ol.read_restart(opath, cap=n)
+            t_or = time.perf_counter() - t0
+            out["cpu_oracle"] = {"write_ms": t_ow * 1e3, "read_ms": t_or * 1e3, "threads": 1,
+                                 "same_bytes": filecmp.cmp(opath, rpath, shallow=False)}
+    sim.close()
+    return out
+
+
 def cpu_baseline(args, budget_s):
     """The oracle (OpenMP C++ port of the reference CPU path) on the same
     workload: steps until the time budget is used (at least one)."""
@@ -386,6 +438,7 @@ def main():
     ap.add_argument("--no-wcsph", action="store_true")
     ap.add_argument("--no-rigid", action="store_true")
     ap.add_argument("--no-model-fill", action="store_true")
+    ap.add_argument("--no-state-io", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--show-trials", action="store_true", help="print every tuning trial to stderr")
@@ -459,6 +512,8 @@ def main():
         out["rigid"] = run_rigid(args)
     if not args.no_model_fill:
         out["model_fill"] = run_model_fill(args)
+    if not args.no_state_io:
+        out["state_io"] = run_state_io(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, args.cpu_budget)
     print(json.dumps(out))

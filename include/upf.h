@@ -369,6 +369,57 @@ int upf_obj_read(const char* path, float* faces9, int cap, int* n_faces);
 int upf_mesh_fill(const float* faces9, int n_faces, const float scale[3], float spacing, float* pos4, int cap,
                   int* n_out);
 
+/* ----------------------------------------------------- particle state files */
+/* upf_save_text writes the simulation's particles in slot order (the order
+ * of the last z-index sort, as CopyParticlesCUDA hands them to the writers,
+ * src/cpu/UnifiedPhysics.cpp:6176-6178; upload order before the first
+ * step), byte for byte in the reference's
+ * formats (src/cpu/UnifiedIO.cpp):
+ *   UPF_TEXT_LIQUID     SaveLiquidParticleInfoFull (:62-113): count line, then
+ *                       per liquid particle its position and type, the other
+ *                       fields written as zeros (as the reference does);
+ *   UPF_TEXT_RIGID      SaveRigidParticleInfoFull (:115-166): the same for
+ *                       rigid and frozen particles;
+ *   UPF_TEXT_POSITIONS  SaveParticlePositions (:13-60): every particle,
+ *                       20 significant digits;
+ *   UPF_TEXT_RESTART    the lines ReadInFluidParticlesPCISPH reads (:196-220):
+ *                       position, type, velocity, correction pressure and
+ *                       predicted density of the last step, 9 significant
+ *                       digits (each float exactly).  Loaded back by
+ *                       upf_read_restart + upf_upload_restart, the next steps
+ *                       equal the steps of the simulation that wrote it.
+ * The count line is the number of particle lines that follow.  The file name
+ * is the caller's (the reference numbers files under a fixed directory).  */
+#define UPF_TEXT_LIQUID     1
+#define UPF_TEXT_RIGID      2
+#define UPF_TEXT_POSITIONS  3
+#define UPF_TEXT_RESTART    4
+int upf_save_text(upf_sim* s, const char* path, int which);
+/* The same files from host arrays (n particles, e.g. from upf_download):
+ * pos4 float4, type per particle; vel4 / corr_pressure / predicted_density
+ * are read for UPF_TEXT_RESTART only.  Host code, no device needed.      */
+int upf_write_text(const char* path, int which, int n, const float* pos4, const int* type, const float* vel4,
+                   const float* corr_pressure, const float* predicted_density);
+
+/* ReadInFluidParticleNum + ReadInFluidParticlesPCISPH (src/cpu/UnifiedIO.cpp:168-220):
+ * the count, then 9 whitespace-separated fields per particle.  All output
+ * pointers NULL: count only (n_out).  cap < count: UPF_ERR_RANGE with n_out
+ * set.  A missing file is UPF_ERR_IO (the reference exits); fewer particles
+ * than the count, or a field that is not a decimal number (int for the
+ * type), is UPF_ERR_ARG (the reference stops reading and keeps undefined
+ * values).  pos4/vel4 are float4 with w = 0.                             */
+int upf_read_restart(const char* path, int cap, int* n_out, float* pos4, float* vel4, int* type,
+                     float* corr_pressure, float* predicted_density);
+
+/* Scene::readFluidObjectFromFile (src/sceneConfigure/Scene.cpp:648-697)
+ * followed by the upload: as upf_upload_particles, with each particle's
+ * correction pressure and predicted density of the step before (the
+ * UnifiedParticle constructor that takes them, src/cpu/UnifiedParticle.cpp:29-45;
+ * the pressure of two steps back, which that constructor leaves unset, is 0).
+ * Liquid and frozen particles (rigid bodies are not part of the format).  */
+int upf_upload_restart(upf_sim* s, int n, const float* pos4, const float* vel4, const int* type,
+                       const float* corr_pressure, const float* predicted_density);
+
 /* ------------------------------------------------------ launch tuning */
 /* Launch shapes that change speed only, never results (no reference
  * counterpart; the reference fixes them at compile time).  Defaults

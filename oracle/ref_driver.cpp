@@ -1,10 +1,14 @@
 // ref_driver.cpp -- test-only C entry points over the reference's own
 // translation units (WeightingKernel.cpp, zIndex.cpp, UnifiedConstants.cpp,
-// RigidBody.cpp, Matrix3x3.cpp, UnifiedParticle.cpp, and the OBJ model loader
-// ObjLoader.cpp, ObjectLoader.cpp, Polygon.cpp, Face.cpp, Mesh.cpp), compiled from their
+// RigidBody.cpp, Matrix3x3.cpp, UnifiedParticle.cpp, the OBJ model loader
+// ObjLoader.cpp, ObjectLoader.cpp, Polygon.cpp, Face.cpp, Mesh.cpp, and the
+// particle file writers and reader UnifiedIO.cpp), compiled from their
 // place under the reference tree by oracle/Makefile into oracle/_ref/.
 // It pins the oracle's restatement of those files; nothing here is shipped.
 
+#include <unistd.h>
+
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <iostream>
@@ -17,6 +21,7 @@
 #undef private
 #include "ObjLoader.h"
 #include "UnifiedConstants.h"
+#include "UnifiedIO.h"
 #include "UnifiedParticle.h"
 #include "WeightingKernel.h"
 #include "zIndex.h"
@@ -175,6 +180,49 @@ int ref_obj_particles(const char* path, float sx, float sy, float sz, float spac
         for (int a = 0; a < 3; ++a) out3[3 * i + a] = (*v)[i].position_[a];
     delete v;
     return n;
+}
+
+// One call of a UnifiedIO writer (src/cpu/UnifiedIO.cpp:13-166) with `dir` as
+// the working directory: the writers open "K:/SimulationData/output/..."
+// relative to it, so dir must hold K:/SimulationData/output/FluidParticles and
+// .../RigidParticles.  which: 1 SaveLiquidParticleInfoFull, 2
+// SaveRigidParticleInfoFull, 3 SaveParticlePositions (particles built from
+// pos4 / type).  count is the count argument of writers 1 and 2.  Returns the
+// writer's file counter of this call (its file is particles%04d.txt,
+// rigid1Particles%04d.txt or particles%04d.txt under the writer's folder),
+// -1 on a bad argument.
+int ref_save_text(const char* dir, int which, int n, const float* pos4, const int* type, int count) {
+    static UnifiedIO io;
+    static int calls[4] = {0, 0, 0, 0};
+    if (which < 1 || which > 3) return -1;
+    char cwd[PATH_MAX];
+    if (!getcwd(cwd, sizeof cwd) || chdir(dir) != 0) return -1;
+    {
+        Quiet q;
+        std::vector<float> p4(pos4, pos4 + 4 * (size_t)n);
+        std::vector<int> t(type, type + n);
+        if (which == 1) io.SaveLiquidParticleInfoFull(p4.data(), t.data(), n, count);
+        else if (which == 2) io.SaveRigidParticleInfoFull(p4.data(), t.data(), n, count);
+        else {
+            std::vector<UnifiedParticle> ps;
+            for (int i = 0; i < n; ++i)
+                ps.push_back(UnifiedParticle(pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2], 1000.0, 1.0, type[i]));
+            io.SaveParticlePositions(ps);
+        }
+    }
+    if (chdir(cwd) != 0) return -1;
+    return calls[which]++;
+}
+
+// ReadInFluidParticleNum + ReadInFluidParticlesPCISPH (src/cpu/UnifiedIO.cpp:168-220)
+// into arrays of cap particles (3 floats per position / velocity); returns
+// the count the file states.  The file must exist (the reference exits).
+int ref_read_restart(const char* path, int cap, float* pos3, float* vel3, int* type, float* cp, float* rho) {
+    static UnifiedIO io;
+    const int cnt = io.ReadInFluidParticleNum(path);
+    if (cnt < 0 || cnt > cap) return cnt;
+    io.ReadInFluidParticlesPCISPH(path, pos3, vel3, type, cp, rho);
+    return cnt;
 }
 
 }  // extern "C"

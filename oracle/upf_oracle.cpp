@@ -24,13 +24,19 @@
 //   into oracle/_ref (when /root/reference is present) and closed forms;
 //   neighbour search : brute force; sort : std::stable_sort and invariants;
 //   density error factor : closed-form lattice sum; PCISPH / WCSPH step :
-//   conservation laws, hydrostatic rest, FP64 twin of the same restatement.
+//   conservation laws, hydrostatic rest, FP64 twin of the same restatement;
+//   OBJ reader / lattice fill and the particle text files (UnifiedIO
+//   writers and restart reader): the reference's own translation units in
+//   oracle/_ref and golden results written by them (tests/golden/).
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
 #include <utility>
 #include <vector>
 
@@ -1862,4 +1868,99 @@ extern "C" int orc_mesh_fill(const float* faces9, int n_faces, const float* scal
         std::memcpy(pos3, out.data(), out.size() * sizeof(orc::OV3));
     }
     return UPF_OK;
+}
+
+// ------------------------------------------------------------ state files
+// UnifiedIO's writers and reader (src/cpu/UnifiedIO.cpp) with the reference's
+// own iostream formatting: precision() on the stream, std::ios::left set once
+// and kept, setw() for one field at a time.
+namespace orc {
+
+// SaveLiquidParticleInfoFull (:62-113) / SaveRigidParticleInfoFull (:115-166)
+// (keep_liquid selects which), SaveParticlePositions (:13-60) and the nine
+// fields ReadInFluidParticlesPCISPH reads (restart, precision 9).
+static void io_write(std::ostream& o, int which, int n, const float* pos4, const int* type, const float* vel4,
+                     const float* cp, const float* rho) {
+    auto keep = [&](int i) {
+        if (which == UPF_TEXT_LIQUID) return type[i] == 2;
+        if (which == UPF_TEXT_RIGID) return type[i] == 0 || type[i] == 6;
+        return true;
+    };
+    int count = 0;
+    for (int i = 0; i < n; ++i) count += keep(i);
+    o << count << std::endl;
+    const float zero = 0;
+    for (int i = 0; i < n; ++i) {
+        if (!keep(i)) continue;
+        const float x = pos4[4 * i], y = pos4[4 * i + 1], z = pos4[4 * i + 2];
+        if (which == UPF_TEXT_LIQUID || which == UPF_TEXT_RIGID) {
+            o.precision(8);
+            o << std::setiosflags(std::ios::left);
+            o << std::setw(15) << x << " " << std::setw(15) << y << " " << std::setw(15) << z << " ";
+            o << std::setw(3) << type[i] << " ";
+            o << std::setw(15) << zero << " " << zero << " " << zero << " ";  // velocity
+            for (int k = 0; k < 4; ++k) o << std::setw(2) << zero << " ";     // rest density, mass, density, gas constant
+            o << std::endl;
+        } else if (which == UPF_TEXT_POSITIONS) {
+            o.precision(20);
+            o << x << " " << y << " " << z << " ";
+            for (int k = 0; k < 8; ++k) o << zero << " ";
+            o << std::endl;
+        } else {
+            o.precision(9);
+            o << x << " " << y << " " << z << " " << type[i] << " " << vel4[4 * i] << " " << vel4[4 * i + 1] << " "
+              << vel4[4 * i + 2] << " " << cp[i] << " " << rho[i] << "\n";
+        }
+    }
+}
+
+// ReadInFluidParticleNum + ReadInFluidParticlesPCISPH (:168-220): the count,
+// then fields read with operator>> while the stream is good.  Returns the
+// number of particles read completely, -1 if the file does not open.
+static int io_read(const char* path, int cap, int* count, float* pos3, float* vel3, int* type, float* cp,
+                   float* rho) {
+    std::ifstream in(path);
+    if (!in) return -1;
+    int cnt = 0;
+    in >> cnt;
+    *count = cnt;
+    int done = 0;
+    while (in && done < cnt && done < cap) {
+        in >> pos3[3 * done] >> pos3[3 * done + 1] >> pos3[3 * done + 2] >> type[done] >> vel3[3 * done] >>
+            vel3[3 * done + 1] >> vel3[3 * done + 2] >> cp[done] >> rho[done];
+        if (in) ++done;
+    }
+    return done;
+}
+
+}  // namespace orc
+
+extern "C" int orc_write_text(const char* path, int which, int n, const float* pos4, const int* type,
+                              const float* vel4, const float* cp, const float* rho) {
+    std::ofstream o(path);
+    if (!o) return UPF_ERR_IO;
+    orc::io_write(o, which, n, pos4, type, vel4, cp, rho);
+    return o ? 0 : UPF_ERR_IO;
+}
+
+extern "C" int orc_read_restart(const char* path, int cap, int* count, float* pos3, float* vel3, int* type, float* cp,
+                                float* rho) {
+    return orc::io_read(path, cap, count, pos3, vel3, type, cp, rho);
+}
+
+// The state Scene::readFluidObjectFromFile gives each particle
+// (src/sceneConfigure/Scene.cpp:648-697, UnifiedParticle.cpp:29-45): the
+// correction pressure and predicted density of the file, by upload index.
+template <class T> static void set_restart(orc::Sim<T>& s, const float* cp, const float* rho) {
+    for (auto& p : s.ps) {
+        p.corr_p = (T)cp[p.id];
+        p.rho_pred = (T)rho[p.id];
+        p.prev_corr_p = 0;
+    }
+}
+
+extern "C" int orc_sim_set_restart(orc_sim* s, const float* cp, const float* rho) {
+    if (s->prec == 64) set_restart(s->d, cp, rho);
+    else set_restart(s->f, cp, rho);
+    return 0;
 }

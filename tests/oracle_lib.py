@@ -46,6 +46,9 @@ def _bind(lib):
     lib.orc_diff_kernels.argtypes = [f, C.c_int, pd, pd, pd, pd, pd, pd]
     lib.orc_obj_read.argtypes = [C.c_char_p, pf, C.c_int, pi]
     lib.orc_mesh_fill.argtypes = [pf, C.c_int, pf, f, pf, C.c_int, pi]
+    lib.orc_write_text.argtypes = [C.c_char_p, C.c_int, C.c_int, pf, pi, pf, pf, pf]
+    lib.orc_read_restart.argtypes = [C.c_char_p, C.c_int, pi, pf, pf, pi, pf, pf]
+    lib.orc_sim_set_restart.argtypes = [C.c_void_p, pf, pf]
     return lib
 
 
@@ -76,6 +79,8 @@ def ref_lib():
         lib.ref_box_query.argtypes = [C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), pu, C.c_int, pu]
         lib.ref_rigid_step.argtypes = [pf, C.c_int, pf, pf, pf, C.c_float, C.c_float, C.c_float]
         lib.ref_obj_particles.argtypes = [C.c_char_p, C.c_float, C.c_float, C.c_float, C.c_float, pf, C.c_int]
+        lib.ref_save_text.argtypes = [C.c_char_p, C.c_int, C.c_int, pf, C.POINTER(C.c_int), C.c_int]
+        lib.ref_read_restart.argtypes = [C.c_char_p, C.c_int, pf, pf, C.POINTER(C.c_int), pf, pf]
         _libs["ref"] = lib
     return _libs["ref"]
 
@@ -119,6 +124,71 @@ def ref_obj_particles(path, scale, spacing):
     if n:
         L.ref_obj_particles(os.fsencode(path), sc[0], sc[1], sc[2], float(spacing), ptr(out), n)
     return out
+
+
+def write_text(path, which, pos4, types, vel4=None, cp=None, rho=None):
+    """The oracle's UnifiedIO writers (iostream formatting, as the reference)."""
+    arr = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (pos4, vel4, cp, rho)]
+    t = np.ascontiguousarray(types, np.int32)
+    rc = oracle().orc_write_text(os.fsencode(path), int(which), len(t), ptr(arr[0]), ptr(t, C.c_int),
+                                 ptr(arr[1]), ptr(arr[2]), ptr(arr[3]))
+    if rc:
+        raise OSError("orc_write_text: %d" % rc)
+
+
+def read_restart(path, cap=1 << 20):
+    """The oracle's ReadInFluidParticlesPCISPH (operator>> while the stream is
+    good): (count stated by the file, dict of the particles read completely)."""
+    n = C.c_int()
+    out = dict(pos=np.zeros((cap, 3), np.float32), vel=np.zeros((cap, 3), np.float32),
+               type=np.zeros(cap, np.int32), corr_pressure=np.zeros(cap, np.float32),
+               predicted_density=np.zeros(cap, np.float32))
+    done = oracle().orc_read_restart(os.fsencode(path), cap, C.byref(n), ptr(out["pos"]), ptr(out["vel"]),
+                                     ptr(out["type"], C.c_int), ptr(out["corr_pressure"]),
+                                     ptr(out["predicted_density"]))
+    if done < 0:
+        raise OSError("orc_read_restart: cannot open %s" % path)
+    return n.value, {k: v[:done] for k, v in out.items()}
+
+
+REF_TEXT_FILES = {1: "FluidParticles/particles%04d.txt", 2: "RigidParticles/rigid1Particles%04d.txt",
+                  3: "particles%04d.txt"}
+
+
+def ref_save_text(workdir, which, pos4, types, count=None):
+    """Runs the reference's UnifiedIO writer `which` (1 liquid, 2 rigid, 3
+    positions) with `workdir` as working directory; returns the bytes it wrote,
+    or None without the reference tree."""
+    L = ref_lib()
+    if L is None:
+        return None
+    out = os.path.join(workdir, "K:", "SimulationData", "output")
+    for d in ("FluidParticles", "RigidParticles"):
+        os.makedirs(os.path.join(out, d), exist_ok=True)
+    p4 = np.ascontiguousarray(pos4, np.float32)
+    t = np.ascontiguousarray(types, np.int32)
+    if count is None:
+        count = int(np.sum(t == 2)) if which == 1 else int(np.sum((t == 0) | (t == 6)))
+    k = L.ref_save_text(os.fsencode(workdir), which, len(t), ptr(p4), ptr(t, C.c_int), count)
+    if k < 0:
+        raise OSError("ref_save_text failed")
+    with open(os.path.join(out, REF_TEXT_FILES[which] % k), "rb") as f:
+        return f.read()
+
+
+def ref_read_restart(path, cap=1 << 20):
+    """The reference's ReadInFluidParticleNum + ReadInFluidParticlesPCISPH (the
+    file must exist): (count, dict of arrays of that count), None without the
+    reference tree."""
+    L = ref_lib()
+    if L is None:
+        return None
+    out = dict(pos=np.zeros((cap, 3), np.float32), vel=np.zeros((cap, 3), np.float32),
+               type=np.zeros(cap, np.int32), corr_pressure=np.zeros(cap, np.float32),
+               predicted_density=np.zeros(cap, np.float32))
+    n = L.ref_read_restart(os.fsencode(path), cap, ptr(out["pos"]), ptr(out["vel"]), ptr(out["type"], C.c_int),
+                           ptr(out["corr_pressure"]), ptr(out["predicted_density"]))
+    return n, {k: v[:max(n, 0)] for k, v in out.items()}
 
 
 def params(spacing=0.01, vmax=None, cmin=None, cmax=None):
@@ -167,6 +237,20 @@ class OracleSim:
                                   ptr(out["force"]), ptr(out["psi"]), ptr(out["type"], C.c_int),
                                   ptr(out["keys"], C.c_uint32), ptr(out["ids"], C.c_int), 1 if pcisph else 0)
         return out
+
+    def set_restart(self, corr_pressure, predicted_density):
+        """Correction pressure and predicted density of a restart file, by upload index."""
+        cp = np.ascontiguousarray(corr_pressure, np.float32)
+        rho = np.ascontiguousarray(predicted_density, np.float32)
+        self.lib.orc_sim_set_restart(self.h, ptr(cp), ptr(rho))
+
+    def slot_state(self, pcisph=True, ids=None):
+        """state() in the order of the oracle's particles (the CPU path's
+        particles_, sorted at the end of every step), or in the order of the
+        upload indices `ids`."""
+        st = self.state(pcisph)
+        ids = st["ids"] if ids is None else ids
+        return {k: (v[ids] if k not in ("keys", "ids") else v) for k, v in st.items()}
 
     def bodies(self, nb):
         arr = (Body * nb)()

@@ -14,7 +14,8 @@ import numpy as np
 
 from ._abi import (Params, DamBreakDesc, RigidSceneDesc, Body, Tuning, ptr, ERROR_NAMES,  # noqa: F401
                    UPF_OK, UPF_ERR_RANGE, UPF_LIQUID, UPF_FROZEN, UPF_RIGID, UPF_PCISPH_STATIC,
-                   UPF_WCSPH_STATIC, UPF_PCISPH_RIGID, UPF_BOUNDARY_CPU_REF, UPF_BOUNDARY_AKINCI)
+                   UPF_WCSPH_STATIC, UPF_PCISPH_RIGID, UPF_BOUNDARY_CPU_REF, UPF_BOUNDARY_AKINCI,
+                   UPF_TEXT_LIQUID, UPF_TEXT_RIGID, UPF_TEXT_POSITIONS, UPF_TEXT_RESTART)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libupf.so")
@@ -75,6 +76,10 @@ def lib():
         L.upf_profile_name.restype = C.c_char_p
         L.upf_obj_read.argtypes = [C.c_char_p, pf, C.c_int, pi]
         L.upf_mesh_fill.argtypes = [pf, C.c_int, pf, C.c_float, pf, C.c_int, pi]
+        L.upf_save_text.argtypes = [vp, C.c_char_p, C.c_int]
+        L.upf_write_text.argtypes = [C.c_char_p, C.c_int, C.c_int, pf, pi, pf, pf, pf]
+        L.upf_read_restart.argtypes = [C.c_char_p, C.c_int, pi, pf, pf, pi, pf, pf]
+        L.upf_upload_restart.argtypes = [vp, C.c_int, pf, pf, pi, pf, pf]
         for name, args in (("upf_upload_bodies", [vp, C.c_int, C.POINTER(Body), pi, pf]),
                            ("upf_download_bodies", [vp, C.POINTER(Body)])):
             if hasattr(L, name):
@@ -144,6 +149,35 @@ def model_particles(path, scale, spacing, offset=(0.0, 0.0, 0.0)):
     the object's offset (float32 additions)."""
     pos = mesh_fill(load_obj(path), scale, spacing)
     return pos + np.asarray(offset, np.float32)
+
+
+def write_text(path, which, pos4, types, vel4=None, corr_pressure=None, predicted_density=None):
+    """The reference's particle files (src/cpu/UnifiedIO.cpp) from host arrays:
+    which = UPF_TEXT_LIQUID / _RIGID / _POSITIONS, or UPF_TEXT_RESTART (then
+    velocity, correction pressure and predicted density are written too)."""
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    types = np.ascontiguousarray(types, np.int32)
+    n = len(types)
+    arr = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (vel4, corr_pressure,
+                                                                              predicted_density)]
+    check("upf_write_text", lib().upf_write_text(os.fsencode(path), int(which), n, ptr(pos4), ptr(types, C.c_int),
+                                                 *[ptr(a) for a in arr]))
+
+
+def read_restart(path):
+    """ReadInFluidParticleNum + ReadInFluidParticlesPCISPH (src/cpu/UnifiedIO.cpp:168-220):
+    dict of pos (n, 4), vel (n, 4), type, corr_pressure, predicted_density."""
+    L = lib()
+    n = C.c_int()
+    check("upf_read_restart", L.upf_read_restart(os.fsencode(path), 0, C.byref(n), None, None, None, None, None))
+    m = n.value
+    out = dict(pos=np.zeros((m, 4), np.float32), vel=np.zeros((m, 4), np.float32), type=np.zeros(m, np.int32),
+               corr_pressure=np.zeros(m, np.float32), predicted_density=np.zeros(m, np.float32))
+    if m:
+        check("upf_read_restart", L.upf_read_restart(os.fsencode(path), m, C.byref(n), ptr(out["pos"]),
+                                                     ptr(out["vel"]), ptr(out["type"], C.c_int),
+                                                     ptr(out["corr_pressure"]), ptr(out["predicted_density"])))
+    return out
 
 
 def luts(p):
@@ -250,6 +284,26 @@ class Simulation:
         self.n = len(types)
         check("upf_upload_particles", lib().upf_upload_particles(self.h, self.n, ptr(pos4), ptr(vel),
                                                                  ptr(types, C.c_int)))
+
+    def upload_restart(self, state):
+        """Scene::readFluidObjectFromFile + upload (src/sceneConfigure/Scene.cpp:648-697):
+        `state` is what read_restart returns (or a path to read)."""
+        if not isinstance(state, dict):
+            state = read_restart(state)
+        pos4 = np.ascontiguousarray(state["pos"], np.float32)
+        types = np.ascontiguousarray(state["type"], np.int32)
+        vel = np.ascontiguousarray(state["vel"], np.float32)
+        cp = np.ascontiguousarray(state["corr_pressure"], np.float32)
+        rho = np.ascontiguousarray(state["predicted_density"], np.float32)
+        self.n = len(types)
+        check("upf_upload_restart", lib().upf_upload_restart(self.h, self.n, ptr(pos4), ptr(vel),
+                                                             ptr(types, C.c_int), ptr(cp), ptr(rho)))
+
+    def save_text(self, path, which=UPF_TEXT_RESTART):
+        """The particles in slot order as one of the reference's files
+        (UnifiedIO, src/cpu/UnifiedIO.cpp); UPF_TEXT_RESTART reloads with
+        upload_restart and continues exactly."""
+        check("upf_save_text", lib().upf_save_text(self.h, os.fsencode(path), int(which)))
 
     def upload_bodies(self, bodies, body_id, rel4):
         arr = (Body * len(bodies))(*bodies)

@@ -27,6 +27,11 @@ UPF_RIGID = 0
 UPF_LIQUID = 2
 UPF_FROZEN = 6
 
+UPF_TEXT_LIQUID = 1
+UPF_TEXT_RIGID = 2
+UPF_TEXT_POSITIONS = 3
+UPF_TEXT_RESTART = 4
+
 UPF_PCISPH_STATIC = 0
 UPF_WCSPH_STATIC = 1
 UPF_PCISPH_RIGID = 2

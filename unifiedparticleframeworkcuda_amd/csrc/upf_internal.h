@@ -17,6 +17,10 @@ float sqrt_threshold(float h);
 uint64_t splitmix64(uint64_t& s);
 double uniform01(uint64_t& s);
 int choose_domain(upf_params* p, const float box[3], float pad);
+// Text file of `which` (UPF_TEXT_*) from host arrays; vel4, p and rho are
+// read for UPF_TEXT_RESTART only.
+int write_text(const char* path, int which, int n, const float* pos4, const int* type, const float* vel4,
+               const float* p, const float* rho);
 
 }  // namespace upf
 

@@ -2429,9 +2429,12 @@ int upf_set_params(upf_sim* s, const upf_params* p) {
 }  // extern "C"
 
 // Uploads n particles; scene_of / scene_params give every particle's scene and
-// the scenes' domains (null: one scene with the simulation's params).
+// the scenes' domains (null: one scene with the simulation's params).  p0 /
+// rho0, if given, are the correction pressure and predicted density of the
+// step before (a restart); else 0 and the rest density.
 static int upload_common(upf_sim* s, int n, const float* pos4, const float* vel4, const int* type,
-                         const int* scene_of, const upf_params* scene_params) {
+                         const int* scene_of, const upf_params* scene_params, const float* p0 = nullptr,
+                         const float* rho0 = nullptr) {
     if (!s || n <= 0 || n > s->cap || !pos4 || !type) return UPF_ERR_ARG;
     std::vector<float> pos((size_t)4 * n), vel((size_t)4 * n, 0.0f), x0((size_t)4 * n);
     std::vector<int> id(n), body(n, -1);
@@ -2474,8 +2477,8 @@ static int upload_common(upf_sim* s, int n, const float* pos4, const float* vel4
     HIPCHK(hipMemcpyAsync(S.rho_last, rho.data(), b1, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipMemcpyAsync(S.psi, psi.data(), b1, hipMemcpyHostToDevice, s->st));
     // loop arrays of the "previous step" (gathered into p_last / rho_last by the first sort)
-    HIPCHK(hipMemcpyAsync(s->W.p, zero.data(), b1, hipMemcpyHostToDevice, s->st));
-    HIPCHK(hipMemcpyAsync(s->W.rho, rho.data(), b1, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->W.p, p0 ? p0 : zero.data(), b1, hipMemcpyHostToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->W.rho, rho0 ? rho0 : rho.data(), b1, hipMemcpyHostToDevice, s->st));
     HIPCHK(hipMemsetAsync(s->ctl, 0, sizeof(Ctl), s->st));
     if (scene_of) HIPCHK(hipMemcpyAsync(s->d_scene_of, scene_of, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s->st));
     hipLaunchKernelGGL(k_keys, grid_for(n), dim3(BLOCK), 0, s->st, s->args(), S.pos, s->W.keys, s->ctl);
@@ -2704,6 +2707,36 @@ int upf_upload_bodies(upf_sim* s, int n_bodies, const upf_body* bodies, const in
     s->n_bodies = n_bodies;
     s->rigid = true;
     return UPF_OK;
+}
+
+int upf_upload_restart(upf_sim* s, int n, const float* pos4, const float* vel4, const int* type,
+                       const float* corr_pressure, const float* predicted_density) {
+    if (s && s->n_scenes != 1) return UPF_ERR_STATE;
+    if (!corr_pressure || !predicted_density) return UPF_ERR_ARG;
+    return upload_common(s, n, pos4, vel4, type, nullptr, nullptr, corr_pressure, predicted_density);
+}
+
+int upf_save_text(upf_sim* s, const char* path, int which) {
+    if (!s || !path || which < UPF_TEXT_LIQUID || which > UPF_TEXT_RESTART) return UPF_ERR_ARG;
+    if (!s->uploaded) return UPF_ERR_STATE;
+    const int n = s->n;
+    const Slots& S = s->S[s->cur];
+    HIPCHK(hipStreamSynchronize(s->st));
+    // slot order: the arrays as the last sort (and the step after it) left them
+    std::vector<float> pos((size_t)4 * n), vel, p, rho;
+    HIPCHK(hipMemcpy(pos.data(), S.pos, (size_t)n * 16, hipMemcpyDeviceToHost));
+    if (which == UPF_TEXT_RESTART) {
+        vel.resize((size_t)4 * n);
+        p.resize(n);
+        rho.resize(n);
+        HIPCHK(hipMemcpy(vel.data(), S.vel, (size_t)n * 16, hipMemcpyDeviceToHost));
+        const float* src_p = s->last_method == UPF_WCSPH_STATIC ? s->W.pres : s->W.p;
+        HIPCHK(hipMemcpy(p.data(), src_p, (size_t)n * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(rho.data(), s->W.rho, (size_t)n * 4, hipMemcpyDeviceToHost));
+    }
+    std::vector<int> type(n);
+    for (int i = 0; i < n; ++i) std::memcpy(&type[i], &pos[4 * (size_t)i + 3], 4);
+    return write_text(path, which, n, pos.data(), type.data(), vel.data(), p.data(), rho.data());
 }
 
 int upf_download_bodies(upf_sim* s, upf_body* out) {
