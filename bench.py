@@ -384,6 +384,7 @@ def run_state_io(args):
                "liquid_bytes": os.path.getsize(lpath), "save_restart_MBps": size / t_save / 1e6,
                "read_restart_MBps": size / t_read / 1e6}
         if not args.no_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
             import oracle_lib as ol
             opath = os.path.join(d, "oracle.txt")
             t0 = time.perf_counter()

@@ -401,6 +401,18 @@ int upf_save_text(upf_sim* s, const char* path, int which);
 int upf_write_text(const char* path, int which, int n, const float* pos4, const int* type, const float* vel4,
                    const float* corr_pressure, const float* predicted_density);
 
+/* DumpGPUParamsToFile (src/cpu/UnifiedPhysics.cpp:411-474), the binary
+ * cross-check dump of the particles and the block (cell) table: sorts the
+ * current state and builds its cell table as the next step's start does
+ * (the state and the next steps are unchanged), then appends one record to
+ * each file.  particles_path: uint32 n, then n float4 (x, y, z, w) in slot
+ * order, w = the z-index of the particle's uploaded position as a float (the
+ * reference's w, set at upload and never rewritten).  blocks_path: uint32
+ * number of occupied blocks, then per occupied block, by ascending block
+ * index (grid_dim^2 * bz + grid_dim * by + bx), uint32 (block index, first
+ * slot, particle count).  Native-endian.  Single-scene simulations.     */
+int upf_dump_cells(upf_sim* s, const char* particles_path, const char* blocks_path);
+
 /* ReadInFluidParticleNum + ReadInFluidParticlesPCISPH (src/cpu/UnifiedIO.cpp:168-220):
  * the count, then 9 whitespace-separated fields per particle.  All output
  * pointers NULL: count only (n_out).  cap < count: UPF_ERR_RANGE with n_out

@@ -249,7 +249,7 @@ template <class T> struct Particle {
     V3<T> pos, vel, force, pos_pred, vel_pred, fp, sum_grad_w, rel0;
     T density, pressure, corr_p, prev_corr_p, rho_pred, rho_err, psi, mass, sum_grad_w_dot;
     int type, body, id;
-    uint32_t key;
+    uint32_t key, key0;  // key0: z-index of the uploaded position (UnifiedParticle::index_)
 };
 
 struct Body {  // RigidBody (src/cpu/RigidBody.h) state, float like the reference
@@ -1245,6 +1245,7 @@ static void load_particles(orc::Sim<T>& s, int n, const float* pos4, const float
         p.body = -1;
         p.id = i;
         p.key = s.key_of(p.pos);
+        p.key0 = p.key;
     }
     // Scene::addAllTo sorts before the first step (Scene.cpp:190-191)
     s.sort_particles();
@@ -1963,4 +1964,52 @@ extern "C" int orc_sim_set_restart(orc_sim* s, const float* cp, const float* rho
     if (s->prec == 64) set_restart(s->d, cp, rho);
     else set_restart(s->f, cp, rho);
     return 0;
+}
+
+// DumpGPUParamsToFile (src/cpu/UnifiedPhysics.cpp:411-474) on the oracle's
+// particles (sorted by key): the particle record (count, float4 with w = the
+// uploaded z-index) and the block table, whose block of a particle is
+// CalculateBlocksKernel's (src/gpu/particles_kernel.cu:671-710):
+// (int)(p * scales * invblock_size) per axis, row-major over
+// (1024 / block_size)^3 blocks; per occupied block (index, first, count),
+// looped over block indices in ascending order.
+template <class T>
+static int dump_cells(orc::Sim<T>& s, const char* ppath, const char* bpath) {
+    const int n = (int)s.ps.size();
+    const uint32_t dim = 1024u / s.P.block_size;
+    const float inv = 1.0f / (float)s.P.block_size;
+    std::vector<float> rec((size_t)4 * n);
+    std::vector<uint32_t> first((size_t)dim * dim * dim, 0xFFFFFFFFu), count((size_t)dim * dim * dim, 0);
+    for (int i = 0; i < n; ++i) {
+        const orc::Particle<T>& q = s.ps[i];
+        const float x = (float)q.pos.x, y = (float)q.pos.y, z = (float)q.pos.z;
+        rec[4 * i] = x;
+        rec[4 * i + 1] = y;
+        rec[4 * i + 2] = z;
+        rec[4 * i + 3] = (float)q.key0;
+        const int bx = (int)(x * s.P.scales[0] * inv), by = (int)(y * s.P.scales[1] * inv),
+                  bz = (int)(z * s.P.scales[2] * inv);
+        const uint32_t b = (dim * dim) * (uint32_t)bz + ((uint32_t)by * dim + (uint32_t)bx);
+        first[b] = std::min(first[b], (uint32_t)i);
+        ++count[b];
+    }
+    std::ofstream pf(ppath, std::ios::app | std::ios::binary);
+    const uint32_t cnt = (uint32_t)n;
+    pf.write((const char*)&cnt, 4);
+    pf.write((const char*)rec.data(), (std::streamsize)rec.size() * 4);
+    uint32_t filled = 0;
+    for (size_t b = 0; b < count.size(); ++b) filled += count[b] > 0;
+    std::ofstream bf(bpath, std::ios::app | std::ios::binary);
+    bf.write((const char*)&filled, 4);
+    for (uint32_t b = 0; b < (uint32_t)count.size(); ++b) {
+        if (!count[b]) continue;
+        bf.write((const char*)&b, 4);
+        bf.write((const char*)&first[b], 4);
+        bf.write((const char*)&count[b], 4);
+    }
+    return (pf && bf) ? 0 : UPF_ERR_IO;
+}
+
+extern "C" int orc_dump_cells(orc_sim* s, const char* ppath, const char* bpath) {
+    return s->prec == 64 ? dump_cells(s->d, ppath, bpath) : dump_cells(s->f, ppath, bpath);
 }

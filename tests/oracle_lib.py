@@ -49,6 +49,7 @@ def _bind(lib):
     lib.orc_write_text.argtypes = [C.c_char_p, C.c_int, C.c_int, pf, pi, pf, pf, pf]
     lib.orc_read_restart.argtypes = [C.c_char_p, C.c_int, pi, pf, pf, pi, pf, pf]
     lib.orc_sim_set_restart.argtypes = [C.c_void_p, pf, pf]
+    lib.orc_dump_cells.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
     return lib
 
 
@@ -243,6 +244,12 @@ class OracleSim:
         cp = np.ascontiguousarray(corr_pressure, np.float32)
         rho = np.ascontiguousarray(predicted_density, np.float32)
         self.lib.orc_sim_set_restart(self.h, ptr(cp), ptr(rho))
+
+    def dump_cells(self, particles_path, blocks_path):
+        """The oracle's DumpGPUParamsToFile records, appended to the two files."""
+        rc = self.lib.orc_dump_cells(self.h, os.fsencode(particles_path), os.fsencode(blocks_path))
+        if rc:
+            raise OSError("orc_dump_cells: %d" % rc)
 
     def slot_state(self, pcisph=True, ids=None):
         """state() in the order of the oracle's particles (the CPU path's

@@ -138,3 +138,35 @@ def test_state_file_call_errors(gpu, tmp_path):
     assert int(lines[0]) == len(typ)
     first = np.float32([float(v) for v in lines[1].split()[:3]])
     assert np.array_equal(first, pos4[0, :3])
+
+
+@pytest.mark.parametrize("steps", [0, 3])
+def test_cell_dump_equals_oracle_and_leaves_the_run_unchanged(gpu, tmp_path, steps):
+    """upf_dump_cells (DumpGPUParamsToFile) after `steps` steps: the same
+    bytes as the oracle's dump of its state after the same steps; the run
+    that dumped continues exactly as one that did not."""
+    p, pos, typ, nf = scenes.dam_break(16)
+    a = upf.Simulation(p, len(typ))
+    a.upload(pos, typ)
+    if steps:
+        a.step(steps, upf.UPF_PCISPH_STATIC)
+    gp, gb = str(tmp_path / "g_particles.bin"), str(tmp_path / "g_blocks.bin")
+    a.dump_cells(gp, gb)
+    o = ol.OracleSim(p, pos, None, typ)
+    if steps:
+        o.step(steps, upf.UPF_PCISPH_STATIC)
+    op, ob = str(tmp_path / "o_particles.bin"), str(tmp_path / "o_blocks.bin")
+    o.dump_cells(op, ob)
+    assert read_bytes(gp) == read_bytes(op)
+    assert read_bytes(gb) == read_bytes(ob)
+    a.dump_cells(gp, gb)  # a second dump of the same state appends the same record
+    recs = upf.read_cell_dump(gp, gb)
+    assert len(recs) == 2 and recs[0][0].tobytes() == recs[1][0].tobytes()
+    assert recs[0][1].tobytes() == recs[1][1].tobytes()
+    a.step(3, upf.UPF_PCISPH_STATIC)
+    b = upf.Simulation(p, len(typ))
+    b.upload(pos, typ)
+    b.step(steps + 3, upf.UPF_PCISPH_STATIC)
+    fa = sim_files(a, tmp_path, "a", (UPF_TEXT_RESTART,))
+    fb = sim_files(b, tmp_path, "b", (UPF_TEXT_RESTART,))
+    assert fa[UPF_TEXT_RESTART] == fb[UPF_TEXT_RESTART]

@@ -334,3 +334,54 @@ def test_reader_rounds_every_decimal_form_as_an_istream(tmp_path):
     ref = np.concatenate([orc["pos"], orc["vel"], orc["corr_pressure"][:, None], orc["predicted_density"][:, None]], 1)
     bad = np.flatnonzero(lib.view(np.uint32).reshape(-1) != ref.view(np.uint32).reshape(-1))
     assert bad.size == 0, [toks[i] for i in bad[:5]]
+
+
+def _dump_by_numpy(p, pos_sorted, key0_sorted):
+    """DumpGPUParamsToFile's two records rebuilt with numpy from sorted
+    positions: w = the uploaded z-index as float32; a particle's block is
+    (int)(x * scales * (1 / block_size)) per axis in float32, row-major."""
+    n = len(pos_sorted)
+    rec = np.zeros((n, 4), np.float32)
+    rec[:, :3] = pos_sorted[:, :3]
+    rec[:, 3] = key0_sorted.astype(np.float32)
+    dim = 1024 // p.block_size
+    inv = np.float32(1.0) / np.float32(p.block_size)
+    b = [(pos_sorted[:, a].astype(np.float32) * np.float32(p.scales[a]) * inv).astype(np.int64) for a in range(3)]
+    lin = (dim * dim) * b[2] + dim * b[1] + b[0]
+    u, first, cnt = np.unique(lin, return_index=True, return_counts=True)
+    blocks = np.stack([u, first, cnt], 1).astype(np.uint32)
+    return (np.uint32([n]).tobytes() + rec.tobytes(), np.uint32([len(u)]).tobytes() + blocks.tobytes())
+
+
+def _morton_keys(p, pos):
+    c = [np.floor(pos[:, a].astype(np.float32) * np.float32(p.scales[a])).astype(np.uint64) for a in range(3)]
+    key = np.zeros(len(pos), np.uint64)
+    for bit in range(10):
+        for a in range(3):
+            key |= ((c[a] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(3 * bit + a)
+    return key
+
+
+@pytest.mark.parametrize("steps", [0, 2])
+def test_oracle_cell_dump_equals_numpy_restatement(tmp_path, steps):
+    """The oracle's binary dump against a numpy restatement from its sorted
+    state; two dumps append two records."""
+    import scenes
+    p, pos, typ, nf = scenes.dam_break(8)
+    o = ol.OracleSim(p, pos, None, typ, omp=False)
+    o.step(steps, upf.UPF_PCISPH_STATIC) if steps else None
+    pp, bp = str(tmp_path / "particles.bin"), str(tmp_path / "blocks.bin")
+    o.dump_cells(pp, bp)
+    st = o.slot_state()
+    key0 = _morton_keys(p, np.asarray(pos))[st["ids"]]
+    exp_p, exp_b = _dump_by_numpy(p, st["pos"], key0)
+    assert read_bytes(pp) == exp_p and read_bytes(bp) == exp_b
+    o.dump_cells(pp, bp)
+    recs = upf.read_cell_dump(pp, bp)
+    assert len(recs) == 2
+    parts, blocks = recs[1]
+    assert parts.tobytes() == exp_p[4:] and blocks.tobytes() == exp_b[4:]
+    # the blocks tile the sorted particles: contiguous ranges covering every slot once
+    order = np.argsort(blocks[:, 1])
+    ends = blocks[order, 1] + blocks[order, 2]
+    assert blocks[order[0], 1] == 0 and np.array_equal(ends[:-1], blocks[order[1:], 1]) and ends[-1] == len(typ)

@@ -80,6 +80,7 @@ def lib():
         L.upf_write_text.argtypes = [C.c_char_p, C.c_int, C.c_int, pf, pi, pf, pf, pf]
         L.upf_read_restart.argtypes = [C.c_char_p, C.c_int, pi, pf, pf, pi, pf, pf]
         L.upf_upload_restart.argtypes = [vp, C.c_int, pf, pf, pi, pf, pf]
+        L.upf_dump_cells.argtypes = [vp, C.c_char_p, C.c_char_p]
         for name, args in (("upf_upload_bodies", [vp, C.c_int, C.POINTER(Body), pi, pf]),
                            ("upf_download_bodies", [vp, C.POINTER(Body)])):
             if hasattr(L, name):
@@ -177,6 +178,23 @@ def read_restart(path):
         check("upf_read_restart", L.upf_read_restart(os.fsencode(path), m, C.byref(n), ptr(out["pos"]),
                                                      ptr(out["vel"]), ptr(out["type"], C.c_int),
                                                      ptr(out["corr_pressure"]), ptr(out["predicted_density"])))
+    return out
+
+
+def read_cell_dump(particles_path, blocks_path):
+    """The records of the binary dumps (DumpGPUParamsToFile): a list of
+    (particles (n, 4) float32, blocks (m, 3) uint32 as (block index, first, count))."""
+    pb = np.fromfile(particles_path, np.uint8)
+    bb = np.fromfile(blocks_path, np.uint8)
+    out, i, j = [], 0, 0
+    while i < len(pb):
+        n = int(pb[i:i + 4].view(np.uint32)[0])
+        parts = pb[i + 4:i + 4 + 16 * n].view(np.float32).reshape(n, 4)
+        i += 4 + 16 * n
+        m = int(bb[j:j + 4].view(np.uint32)[0])
+        blocks = bb[j + 4:j + 4 + 12 * m].view(np.uint32).reshape(m, 3)
+        j += 4 + 12 * m
+        out.append((parts, blocks))
     return out
 
 
@@ -304,6 +322,12 @@ class Simulation:
         (UnifiedIO, src/cpu/UnifiedIO.cpp); UPF_TEXT_RESTART reloads with
         upload_restart and continues exactly."""
         check("upf_save_text", lib().upf_save_text(self.h, os.fsencode(path), int(which)))
+
+    def dump_cells(self, particles_path, blocks_path):
+        """DumpGPUParamsToFile (src/cpu/UnifiedPhysics.cpp:411-474): appends the
+        sorted particles and the block table of the current state to the two
+        binary files (read them with read_cell_dump)."""
+        check("upf_dump_cells", lib().upf_dump_cells(self.h, os.fsencode(particles_path), os.fsencode(blocks_path)))
 
     def upload_bodies(self, bodies, body_id, rel4):
         arr = (Body * len(bodies))(*bodies)

@@ -3,6 +3,7 @@
 #define UPF_INTERNAL_H_
 
 #include <cstdint>
+#include <vector>
 
 #include "upf.h"
 
@@ -21,6 +22,11 @@ int choose_domain(upf_params* p, const float box[3], float pad);
 // read for UPF_TEXT_RESTART only.
 int write_text(const char* path, int which, int n, const float* pos4, const int* type, const float* vel4,
                const float* p, const float* rho);
+// Appends one record to each binary dump: n and the float4 records to
+// particles_path; the number of blocks and their (block index, first, count)
+// triples, sorted by block index, to blocks_path.
+int write_cell_dump(const char* particles_path, const char* blocks_path, int n, const float* rec4,
+                    std::vector<uint32_t>& blocks);
 
 }  // namespace upf
 

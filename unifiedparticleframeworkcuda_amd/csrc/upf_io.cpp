@@ -104,6 +104,32 @@ int write_text(const char* path, int which, int n, const float* pos4, const int*
     return ok ? UPF_OK : UPF_ERR_IO;
 }
 
+// DumpGPUParamsToFile (src/cpu/UnifiedPhysics.cpp:411-474): raw native-endian
+// uint32 / float records appended to the two files.
+int write_cell_dump(const char* particles_path, const char* blocks_path, int n, const float* rec4,
+                    std::vector<uint32_t>& blocks) {
+    const size_t nb = blocks.size() / 3;
+    std::vector<uint32_t> order(nb);
+    for (size_t b = 0; b < nb; ++b) order[b] = (uint32_t)b;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return blocks[3 * a] < blocks[3 * b]; });
+    std::vector<uint32_t> out(1 + 3 * nb);
+    out[0] = (uint32_t)nb;
+    for (size_t b = 0; b < nb; ++b)
+        for (int f = 0; f < 3; ++f) out[1 + 3 * b + f] = blocks[3 * order[b] + f];
+
+    FILE* fp = std::fopen(particles_path, "ab");
+    if (!fp) return UPF_ERR_IO;
+    const uint32_t count = (uint32_t)n;
+    bool ok = std::fwrite(&count, 4, 1, fp) == 1 && std::fwrite(rec4, 16, (size_t)n, fp) == (size_t)n;
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok) return UPF_ERR_IO;
+    fp = std::fopen(blocks_path, "ab");
+    if (!fp) return UPF_ERR_IO;
+    ok = std::fwrite(out.data(), 4, out.size(), fp) == out.size();
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? UPF_OK : UPF_ERR_IO;
+}
+
 }  // namespace upf
 
 namespace {

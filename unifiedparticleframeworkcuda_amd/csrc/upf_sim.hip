@@ -1890,6 +1890,7 @@ struct upf_sim {
     void* sort_tmp = nullptr;
     uint32_t ncells = 0;
     std::vector<int> k_hist;
+    std::vector<uint32_t> key0;  // z-index of every particle's uploaded position, by upload id
     std::vector<float> rho_hist;
     std::vector<void*> allocs;
     // rigid bodies
@@ -2438,6 +2439,7 @@ static int upload_common(upf_sim* s, int n, const float* pos4, const float* vel4
     if (!s || n <= 0 || n > s->cap || !pos4 || !type) return UPF_ERR_ARG;
     std::vector<float> pos((size_t)4 * n), vel((size_t)4 * n, 0.0f), x0((size_t)4 * n);
     std::vector<int> id(n), body(n, -1);
+    std::vector<uint32_t> key0(n);
     std::vector<float> rho(n, s->P.rest_density), zero(n, 0.0f), psi(n, s->P.initial_mass / s->P.rest_density);
     for (int i = 0; i < n; ++i) {
         const upf_params& Q = scene_of ? scene_params[scene_of[i]] : s->P;
@@ -2457,6 +2459,7 @@ static int upload_common(upf_sim* s, int n, const float* pos4, const float* vel4
         id[i] = i;
         uint32_t k;
         if (!host_key(Q, pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2], k)) return UPF_ERR_RANGE;
+        key0[i] = k;
         // the query box of every particle must stay inside the grid (beyond it the
         // reference's cell ranges overflow, src/cpu/zIndex.cpp:66)
         for (int a = 0; a < 3; ++a)
@@ -2485,6 +2488,7 @@ static int upload_common(upf_sim* s, int n, const float* pos4, const float* vel4
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->st));
     s->uploaded = true;
+    s->key0.swap(key0);
     s->psi_ready = false;
     s->k_hist.clear();
     s->rho_hist.clear();
@@ -2737,6 +2741,37 @@ int upf_save_text(upf_sim* s, const char* path, int which) {
     std::vector<int> type(n);
     for (int i = 0; i < n; ++i) std::memcpy(&type[i], &pos[4 * (size_t)i + 3], 4);
     return write_text(path, which, n, pos.data(), type.data(), vel.data(), p.data(), rho.data());
+}
+
+int upf_dump_cells(upf_sim* s, const char* particles_path, const char* blocks_path) {
+    if (!s || !particles_path || !blocks_path) return UPF_ERR_ARG;
+    if (!s->uploaded || s->n_scenes != 1) return UPF_ERR_STATE;
+    int rc = upf_sort_and_bin(s);
+    if (rc) return rc;
+    const int n = s->n;
+    const Slots& S = s->S[s->cur];
+    std::vector<float> rec((size_t)4 * n);
+    std::vector<int> id(n);
+    std::vector<uint2> cells(s->ncells);
+    HIPCHK(hipMemcpy(rec.data(), S.pos, (size_t)n * 16, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(id.data(), S.id, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cells.data(), s->W.cells, (size_t)s->ncells * sizeof(uint2), hipMemcpyDeviceToHost));
+    // w: the uploaded particle's z-index as a float (the reference uploads
+    // particles_h[i].index into w and never rewrites it)
+    for (int k = 0; k < n; ++k) rec[4 * (size_t)k + 3] = (float)s->key0[id[k]];
+    // occupied cells as (row-major block index, first slot, count), by block index
+    const uint32_t dim = s->P.grid_dim;
+    std::vector<uint32_t> blocks;
+    for (uint32_t c = 0; c < s->ncells; ++c) {
+        if (cells[c].y <= cells[c].x) continue;
+        uint32_t b[3] = {0, 0, 0};
+        for (int bit = 0; bit < 10; ++bit)
+            for (int a = 0; a < 3; ++a) b[a] |= ((c >> (3 * bit + a)) & 1u) << bit;
+        blocks.push_back((dim * dim) * b[2] + dim * b[1] + b[0]);
+        blocks.push_back(cells[c].x);
+        blocks.push_back(cells[c].y - cells[c].x);
+    }
+    return write_cell_dump(particles_path, blocks_path, n, rec.data(), blocks);
 }
 
 int upf_download_bodies(upf_sim* s, upf_body* out) {
