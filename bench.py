@@ -282,6 +282,35 @@ def run_wcsph(args):
     return out
 
 
+def run_wall_weight(args):
+    """PCISPH on the 1M dam break's fluid without its wall particles, the
+    container's walls entering the predicted density through the wall weight
+    function (add_wall_weight), against the same steps without it: ms per
+    step and per correction iteration (the term costs one distance and six
+    table reads per liquid particle and iteration)."""
+    import unifiedparticleframeworkcuda_amd as upf
+    nx, ny, nz = args.lattice
+    out = {}
+    for on in (1, 0):
+        p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz)
+        f = typ == upf.UPF_LIQUID
+        pos, typ = pos[f], typ[f]
+        p.add_wall_weight = on
+        sim = upf.Simulation(p, len(typ))
+        sim.upload(pos, typ)
+        sim.step(3, upf.UPF_PCISPH_STATIC)
+        steps = 20
+        sec = _timed_steps(sim, steps, upf.UPF_PCISPH_STATIC)
+        K, rmax = sim.stats()
+        sim.close()
+        k = float(np.mean(K[-steps:]))
+        out["on" if on else "off"] = {"ms_per_step": sec / steps * 1e3, "iterations_mean": k,
+                                      "ms_per_iteration": sec / steps * 1e3 / k,
+                                      "updates_per_s": nf * steps / sec,
+                                      "max_density_error_pct": float(max(0.0, 0.1 * float(np.max(rmax[-steps:])) - 100.0))}
+    return out
+
+
 def run_rigid(args):
     """Rigid-fluid coupling: 524,288 fluid particles (64 x 128 x 64 block),
     64 seeded boxes and spheres sampled at half spacing, Akinci boundary
@@ -440,6 +469,7 @@ def main():
     ap.add_argument("--no-rigid", action="store_true")
     ap.add_argument("--no-model-fill", action="store_true")
     ap.add_argument("--no-state-io", action="store_true")
+    ap.add_argument("--no-wall-weight", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--show-trials", action="store_true", help="print every tuning trial to stderr")
@@ -515,6 +545,8 @@ def main():
         out["model_fill"] = run_model_fill(args)
     if not args.no_state_io:
         out["state_io"] = run_state_io(args)
+    if not args.no_wall_weight:
+        out["wall_weight"] = run_wall_weight(args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, args.cpu_budget)
     print(json.dumps(out))

@@ -119,6 +119,18 @@ typedef struct upf_params {
                                       is a neighbour iff |x_i-x_j|^2 < T, which is
                                       exactly `sqrtf(d2) < h` of the CPU path      */
     int   max_neighbors;           /* neighbour list capacity per particle         */
+    /* wall weight function (addWallWeightFunction, UnifiedConstants.cpp:144;
+     * WallWeight / DistanceToWall, src/cpu/UnifiedPhysics.cpp:5926-6060): a
+     * density contribution of the container's walls as a function of the
+     * distance to the nearest wall face.  WCSPH adds it after the neighbour
+     * sum, as the CPU path's CalculateDensityPressure (:2339-2406); PCISPH
+     * adds it to the predicted density before the neighbour sum, as the GPU
+     * path's ComputePredictedDensityAndPressureBlocksKernelPCISPHWallWeight
+     * (src/gpu/particles_kernel.cu:3845-3990, the CPU path's PCISPH has none).
+     * Single-scene PCISPH_STATIC and WCSPH_STATIC steps.                    */
+    int   add_wall_weight;         /* 0 off (upf_params_init), 1 on                */
+    float container_min[3];        /* realBoxContainer (upf_params_set_domain: the */
+    float container_max[3];        /* collision box widened by particle_radius)    */
 } upf_params;
 
 /* ------------------------------------------------------ host helpers */

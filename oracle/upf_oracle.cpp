@@ -571,13 +571,53 @@ template <class T> class Sim {
         return (T)P.initial_mass;
     }
 
-    // ComputePredictedDensityAndPressure (:5456-5491); returns rho* (liquid) or 0
+    // DistanceToWall (:5926-5988): the distance to the nearest face of the
+    // real box container, clamped at 0, at most the box's longest side.
+    T distance_to_wall(const V3<T>& x) const {
+        const T lo[3] = {(T)P.container_min[0], (T)P.container_min[1], (T)P.container_min[2]};
+        const T hi[3] = {(T)P.container_max[0], (T)P.container_max[1], (T)P.container_max[2]};
+        const T len = hi[0] - lo[0], height = hi[1] - lo[1], width = hi[2] - lo[2];
+        T result = MAXr<T>(len, MAXr<T>(height, width));
+        const T d[6] = {x.y - lo[1], hi[1] - x.y, x.x - lo[0], hi[0] - x.x, x.z - lo[2], hi[2] - x.z};
+        for (int k = 0; k < 6; ++k) {  // ground, ceiling, left, right, back, front
+            const T dist = MAXr<T>(d[k], T(0));
+            const T temp = dist < result ? dist : result;
+            if (temp < result) result = temp;
+        }
+        return result;
+    }
+
+    // WallWeight (:6032-6060): m (W(d1) + W(d2) + 2 (W(d3) + W(d4) + W(d5) + W(d6)))
+    // over the virtual wall particles of Kulasegaram's wall weight.  cpu_form:
+    // kernelM4Lut's index (d * (1/h)) * 300; else WallWeightDevice's
+    // (d * 300) * (1/h) (src/gpu/particles_kernel.cuh:652-693).
+    T wall_weight(T dist, bool cpu_form) const {
+        const T r = (T)P.particle_radius, sp = (T)P.spacing;
+        const T effective = (T)P.support_radius - r;
+        if (dist >= effective) return T(0);
+        auto pyth = [](T a, T b) { return (T)std::sqrt(a * a + b * b); };  // PythagoreanDist (UnifiedPhysics.h:161)
+        const T d1 = dist + r;
+        const T d2 = d1 + sp;
+        const T dd[6] = {d1, d2, pyth(d1, sp), pyth(d1, sp * 2), pyth(d2, sp), pyth(d2, sp * 2)};
+        T kv[6];
+        for (int k = 0; k < 6; ++k) {
+            const int idx = cpu_form ? (int)(dd[k] * L.factor * 300) : (int)(dd[k] * 300 * L.factor);
+            kv[k] = idx >= 300 ? T(0) : L.m4[idx];
+        }
+        return (T)P.initial_mass * (kv[0] + kv[1] + 2 * (kv[2] + kv[3] + kv[4] + kv[5]));
+    }
+
+    // ComputePredictedDensityAndPressure (:5456-5491); returns rho* (liquid) or 0.
+    // With the wall weight on, its term comes first, as in the GPU path's
+    // ComputePredictedDensityAndPressureBlocksKernelPCISPHWallWeight
+    // (src/gpu/particles_kernel.cu:3845-3990), at the predicted position.
     T pred_density(int i) {
         Particle<T>& p = ps[i];
         if (p.type != UPF_LIQUID) return T(0);
         const std::vector<int>& nb = nbrs[i];
         const T m = (T)P.initial_mass;
         p.rho_pred = kernel_self * m;
+        if (P.add_wall_weight) p.rho_pred += wall_weight(distance_to_wall(p.pos_pred), false);
         for (size_t k = 0; k < nb.size(); ++k) {
             const Particle<T>& q = ps[nb[k]];
             T dist = dist3<T>(p.pos_pred, q.pos_pred);
@@ -673,7 +713,8 @@ template <class T> class Sim {
         }
     }
 
-    // CalculateDensityPressure (:2339-2406) with physicsType 'w', wall weight off
+    // CalculateDensityPressure (:2339-2406) with physicsType 'w' (the wall
+    // weight at the end when P.add_wall_weight)
     void density_pressure_wcsph(int i) {
         Particle<T>& p = ps[i];
         if (p.type != UPF_LIQUID) return;
@@ -684,6 +725,7 @@ template <class T> class Sim {
             T dist = dist3<T>(p.pos, ps[nb[k]].pos);
             if (dist < h()) { T w = L.w(dist); p.density += w * m; }
         }
+        if (P.add_wall_weight) p.density += wall_weight(distance_to_wall(p.pos), true);
         // CalculatePressureWCSPH (:2491-2501)
         const T gamma = T(7.0);
         const T b_i = (T)P.rest_density * (T)P.gas_const_wcsph / gamma;
@@ -2012,4 +2054,20 @@ static int dump_cells(orc::Sim<T>& s, const char* ppath, const char* bpath) {
 
 extern "C" int orc_dump_cells(orc_sim* s, const char* ppath, const char* bpath) {
     return s->prec == 64 ? dump_cells(s->d, ppath, bpath) : dump_cells(s->f, ppath, bpath);
+}
+
+// DistanceToWall and WallWeight of the float oracle for n points: dist,
+// the CPU path's weight and the GPU path's weight (table index order).
+extern "C" int orc_wall_weight(const upf_params* p, int n, const float* pos4, float* dist, float* w_cpu,
+                               float* w_gpu) {
+    orc::Sim<float> s;
+    s.init(*p);
+    for (int i = 0; i < n; ++i) {
+        const orc::V3<float> x(pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2]);
+        const float d = s.distance_to_wall(x);
+        if (dist) dist[i] = d;
+        if (w_cpu) w_cpu[i] = s.wall_weight(d, true);
+        if (w_gpu) w_gpu[i] = s.wall_weight(d, false);
+    }
+    return 0;
 }

@@ -50,6 +50,7 @@ def _bind(lib):
     lib.orc_read_restart.argtypes = [C.c_char_p, C.c_int, pi, pf, pf, pi, pf, pf]
     lib.orc_sim_set_restart.argtypes = [C.c_void_p, pf, pf]
     lib.orc_dump_cells.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
+    lib.orc_wall_weight.argtypes = [pp, C.c_int, pf, pf, pf, pf]
     return lib
 
 
@@ -150,6 +151,16 @@ def read_restart(path, cap=1 << 20):
     if done < 0:
         raise OSError("orc_read_restart: cannot open %s" % path)
     return n.value, {k: v[:done] for k, v in out.items()}
+
+
+def wall_weight(p, pos4):
+    """The oracle's DistanceToWall and WallWeight (CPU-path and GPU-path table
+    index) at the points pos4: (dist, w_cpu, w_gpu) float32 arrays."""
+    x = np.ascontiguousarray(pos4, np.float32)
+    n = len(x)
+    d, wc, wg = (np.zeros(n, np.float32) for _ in range(3))
+    oracle().orc_wall_weight(C.byref(p), n, ptr(x), ptr(d), ptr(wc), ptr(wg))
+    return d, wc, wg
 
 
 REF_TEXT_FILES = {1: "FluidParticles/particles%04d.txt", 2: "RigidParticles/rigid1Particles%04d.txt",

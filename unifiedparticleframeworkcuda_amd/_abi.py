@@ -82,6 +82,9 @@ class Params(C.Structure):
         ("collision_max", F3),
         ("support_radius_sq_limit", C.c_float),
         ("max_neighbors", C.c_int),
+        ("add_wall_weight", C.c_int),
+        ("container_min", F3),
+        ("container_max", F3),
     ]
 
     def copy(self):

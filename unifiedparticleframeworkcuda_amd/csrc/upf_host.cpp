@@ -160,6 +160,9 @@ int upf_params_set_domain(upf_params* p, const float vmax[3], const float cmin[3
         p->scales[a] = 1024.0f / (vmax[a] - 0.0f);
         p->collision_min[a] = cmin[a];
         p->collision_max[a] = cmax[a];
+        // setBoundingBoxValues: collisionBox = realBoxContainer -/+ particleRadius (:374-375)
+        p->container_min[a] = cmin[a] - p->particle_radius;
+        p->container_max[a] = cmax[a] + p->particle_radius;
         const float s = p->scales[a] * p->support_radius;
         bs = a == 0 ? s : (bs >= s ? bs : s);
     }

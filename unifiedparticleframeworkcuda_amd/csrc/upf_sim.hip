@@ -38,6 +38,41 @@ __device__ inline float lut_at(const float* lut, float dist, float factor) {
     return idx >= 300 ? 0.0f : lut[idx];
 }
 
+// DistanceToWall (src/cpu/UnifiedPhysics.cpp:5926-5988) = DistanceToWallDevice
+// (src/gpu/particles_kernel.cuh:612-650): distance to the nearest face of the
+// container box (0 outside it), at most the box's longest side.
+__device__ inline float wall_distance(const upf_params& P, float x, float y, float z) {
+    const float* lo = P.container_min;
+    const float* hi = P.container_max;
+    float r = fmaxf(hi[0] - lo[0], fmaxf(hi[1] - lo[1], hi[2] - lo[2]));
+    r = fminf(r, fmaxf(y - lo[1], 0.0f));
+    r = fminf(r, fmaxf(hi[1] - y, 0.0f));
+    r = fminf(r, fmaxf(x - lo[0], 0.0f));
+    r = fminf(r, fmaxf(hi[0] - x, 0.0f));
+    r = fminf(r, fmaxf(z - lo[2], 0.0f));
+    r = fminf(r, fmaxf(hi[2] - z, 0.0f));
+    return r;
+}
+
+// WallWeight (src/cpu/UnifiedPhysics.cpp:6032-6060) / WallWeightDevice
+// (src/gpu/particles_kernel.cuh:652-693): m times the kernel sum over ten
+// virtual wall particles, two layers below the wall (d1, d2) and one and two
+// spacings aside (both sides).  The table index is the CPU path's
+// (int)(d * (1/h) * 300) when CPU_FORM, else the GPU path's (d * 300) * (1/h).
+template <bool CPU_FORM>
+__device__ inline float wall_weight(const upf_params& P, float dist, float inv_h, const float* lw) {
+    if (dist >= P.support_radius - P.particle_radius) return 0.0f;
+    const float s = P.spacing, s2 = s * 2.0f;
+    const float d1 = dist + P.particle_radius, d2 = d1 + s;
+    const float d3 = sqrtf(d1 * d1 + s * s), d4 = sqrtf(d1 * d1 + s2 * s2);
+    const float d5 = sqrtf(d2 * d2 + s * s), d6 = sqrtf(d2 * d2 + s2 * s2);
+    auto k = [&](float d) {
+        const int idx = CPU_FORM ? (int)(d * inv_h * 300.0f) : (int)(d * 300.0f * inv_h);
+        return idx >= 300 ? 0.0f : lw[idx];
+    };
+    return P.initial_mass * (k(d1) + k(d2) + 2.0f * (k(d3) + k(d4) + k(d5) + k(d6)));
+}
+
 __device__ inline int ptype(const float4& x) { return __float_as_int(x.w); }
 
 __device__ inline void set_err(Ctl* ctl, int code) { atomicOr(&ctl->err, 1u << code); }
@@ -961,6 +996,8 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 const float m = P.initial_mass;
                 const float T = P.support_radius_sq_limit;
                 float rho = P.kernel_self * m;
+                if (P.add_wall_weight)  // the GPU path's order: walls before neighbours
+                    rho = rho + wall_weight<false>(P, wall_distance(P, xi.x, xi.y, xi.z), A.lut_factor, lw);
                 const int cnt = W.ncount[i];
                 const size_t st = (size_t)A.stride;
                 // G neighbours per round: their loads are issued together,
@@ -1238,8 +1275,8 @@ __global__ void k_integrate(StepArgs A, Slots S, Work W, int pcisph, int pure_fl
 }
 
 // ------------------------------------------------------------------- WCSPH
-// CalculateDensityPressure with physicsType 'w' and no wall weight
-// (:2339-2406) + CalculatePressureWCSPH (:2491-2501), over the lists
+// CalculateDensityPressure with physicsType 'w' (:2339-2406), the wall weight
+// when P.add_wall_weight, + CalculatePressureWCSPH (:2491-2501), over the lists
 __global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                          Ctl* ctl) {
     __shared__ float lw[LUT_N];
@@ -1260,6 +1297,7 @@ __global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Wo
     float rho = P.kernel_self * m;
     const int cnt = W.ncount[i];
     for (int k = 0; k < cnt; ++k) rho = rho + lw[W.nbr[(size_t)k * A.stride + i] & NB_BIN_MASK] * m;
+    if (P.add_wall_weight) rho = rho + wall_weight<true>(P, wall_distance(P, xi.x, xi.y, xi.z), A.lut_factor, lw);
     const float gamma = 7.0f;
     const float b = P.rest_density * P.gas_const_wcsph / gamma;
     const float frac = rho / P.rest_density;
@@ -1960,6 +1998,10 @@ static int check_params(const upf_params* p) {
         if (!((float)p->block_size > p->scales[a] * p->support_radius * 1.0001f)) return UPF_ERR_ARG;
     if (p->vbox_min[0] != 0.0f || p->vbox_min[1] != 0.0f || p->vbox_min[2] != 0.0f) return UPF_ERR_ARG;
     if (p->max_neighbors <= 0 || p->max_neighbors > 4096) return UPF_ERR_ARG;
+    if (p->add_wall_weight != 0 && p->add_wall_weight != 1) return UPF_ERR_ARG;
+    if (p->add_wall_weight)
+        for (int a = 0; a < 3; ++a)
+            if (!(p->container_min[a] < p->container_max[a])) return UPF_ERR_ARG;
     return UPF_OK;
 }
 
@@ -2508,6 +2550,7 @@ int upf_step(upf_sim* s, int nsteps, int method) {
     if (!s || nsteps < 0) return UPF_ERR_ARG;
     if (!s->uploaded) return UPF_ERR_STATE;
     if (method == UPF_PCISPH_RIGID && !s->rigid) return UPF_ERR_STATE;
+    if (method == UPF_PCISPH_RIGID && s->P.add_wall_weight) return UPF_ERR_ARG;
     for (int k = 0; k < nsteps; ++k) {
         int rc;
         if (method == UPF_PCISPH_STATIC) rc = step_pcisph_static(s, false);
@@ -2883,6 +2926,8 @@ static bool same_physics(const upf_params& a, const upf_params& b) {
         x.scales[k] = a.scales[k];
         x.collision_min[k] = a.collision_min[k];
         x.collision_max[k] = a.collision_max[k];
+        x.container_min[k] = a.container_min[k];
+        x.container_max[k] = a.container_max[k];
     }
     x.block_size = a.block_size;
     x.block_shift = a.block_shift;
@@ -2902,6 +2947,7 @@ int upf_batch_create(int n_scenes, const upf_params* scene_params, const int* co
     for (int k = 0; k < n_scenes; ++k) {
         if (counts[k] <= 0) return UPF_ERR_ARG;
         int rc = check_params(&scene_params[k]);
+        if (!rc && scene_params[k].add_wall_weight) rc = UPF_ERR_ARG;  // single-scene steps only
         if (rc) return rc;
         if (!same_physics(scene_params[0], scene_params[k])) return UPF_ERR_ARG;
         total += counts[k];
