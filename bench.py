@@ -258,19 +258,21 @@ def _timed_steps(sim, steps, method):
 def run_wcsph(args):
     """The WCSPH variant on the same 1M inputs: Tait EOS, stiffness 1000,
     gamma 7, at the CFL step 0.4 h / c = 0.253 ms; the 1 ms step of the
-    PCISPH runs is also tried (it violates the CFL condition)."""
+    PCISPH runs is also tried (it violates the CFL condition).  Also the
+    original SPH (linear EOS, k = 1000) at 0.253 ms ("sph_...")."""
     import unifiedparticleframeworkcuda_amd as upf
     nx, ny, nz = args.lattice
     out = {}
-    for dt, steps in ((0.000253, 20), (0.001, 20)):
+    for method, dt, steps in ((upf.UPF_WCSPH_STATIC, 0.000253, 20), (upf.UPF_WCSPH_STATIC, 0.001, 20),
+                              (upf.UPF_SPH_STATIC, 0.000253, 20)):
         p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz, dt=dt)
         p.gas_const_wcsph = 1000.0
         sim = upf.Simulation(p, len(typ))
         sim.upload(pos, typ)
-        key = "dt_%gms" % (dt * 1e3)
+        key = ("sph_" if method == upf.UPF_SPH_STATIC else "") + "dt_%gms" % (dt * 1e3)
         try:
-            sim.step(3, upf.UPF_WCSPH_STATIC)
-            sec = _timed_steps(sim, steps, upf.UPF_WCSPH_STATIC)
+            sim.step(3, method)
+            sec = _timed_steps(sim, steps, method)
             st = sim.state()
             f = typ == upf.UPF_LIQUID
             out[key] = {"updates_per_s": nf * steps / sec, "ms_per_step": sec / steps * 1e3, "stable": True,

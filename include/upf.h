@@ -56,6 +56,9 @@ extern "C" {
                                  ParticlePhysicsOnDeviceSPH src/gpu/particles.cu:1281 */
 #define UPF_PCISPH_RIGID   2  /* CpuParticlePhysicsVersatileCouplingPCISPH src/cpu/UnifiedPhysics.cpp:4991 /
                                  ParticlePhysicsOnDeviceFluidRigidCouplingPCISPH src/gpu/particles.cu:2515 */
+#define UPF_SPH_STATIC     3  /* CpuParticlePhysicsStaticBoundariesSPH src/cpu/UnifiedPhysics.cpp:4180 with
+                                 physicsType 'o': original SPH, p = max(0, k (rho - rho0))
+                                 (CalculatePressureSPH :2479-2489), k = gas_const_sph */
 
 /* Boundary-pressure mode (Akinci boundary pressure force on liquid from
  * boundary particles).  The reference CPU path passes an integer LUT index to
@@ -131,6 +134,8 @@ typedef struct upf_params {
     int   add_wall_weight;         /* 0 off (upf_params_init), 1 on                */
     float container_min[3];        /* realBoxContainer (upf_params_set_domain: the */
     float container_max[3];        /* collision box widened by particle_radius)    */
+    float gas_const_sph;           /* fluidGasConst of original SPH (UPF_SPH_STATIC),
+                                      1000 (UnifiedConstants.cpp:260)              */
 } upf_params;
 
 /* ------------------------------------------------------ host helpers */

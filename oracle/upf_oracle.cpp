@@ -715,7 +715,7 @@ template <class T> class Sim {
 
     // CalculateDensityPressure (:2339-2406) with physicsType 'w' (the wall
     // weight at the end when P.add_wall_weight)
-    void density_pressure_wcsph(int i) {
+    void density_pressure_wcsph(int i, bool tait = true) {
         Particle<T>& p = ps[i];
         if (p.type != UPF_LIQUID) return;
         const std::vector<int>& nb = nbrs[i];
@@ -726,6 +726,14 @@ template <class T> class Sim {
             if (dist < h()) { T w = L.w(dist); p.density += w * m; }
         }
         if (P.add_wall_weight) p.density += wall_weight(distance_to_wall(p.pos), true);
+        if (!tait) {  // CalculatePressureSPH (:2479-2489), physicsType 'o'
+            const T gamma = T(1.0);
+            const T b_i = (T)P.rest_density * (T)P.gas_const_sph / gamma;
+            const T densityFrac = p.density / (T)P.rest_density;
+            const T powGamma = densityFrac;
+            p.pressure = MAXr<T>(T(0), b_i * (powGamma - T(1)));
+            return;
+        }
         // CalculatePressureWCSPH (:2491-2501)
         const T gamma = T(7.0);
         const T b_i = (T)P.rest_density * (T)P.gas_const_wcsph / gamma;
@@ -869,10 +877,12 @@ template <class T> class Sim {
             for (int i = 0; i < n; ++i) integrate(i, true);
             boundary_handling(false);
             sort_particles();
-        } else if (method == UPF_WCSPH_STATIC) {  // CpuParticlePhysicsStaticBoundariesSPH (:4180-4202)
+        } else if (method == UPF_WCSPH_STATIC || method == UPF_SPH_STATIC) {
+            // CpuParticlePhysicsStaticBoundariesSPH (:4180-4202), physicsType 'w' or 'o'
+            const bool tait = method == UPF_WCSPH_STATIC;
             neighbor_search();
 #pragma omp parallel for schedule(dynamic, 100)
-            for (int i = 0; i < n; ++i) density_pressure_wcsph(i);
+            for (int i = 0; i < n; ++i) density_pressure_wcsph(i, tait);
 #pragma omp parallel for schedule(dynamic, 100)
             for (int i = 0; i < n; ++i) forces_wcsph(i);
 #pragma omp parallel for schedule(dynamic, 100)

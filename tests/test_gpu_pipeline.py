@@ -207,3 +207,20 @@ def test_wcsph_steps(gpu, box_force):
     _close(g["pos"][:, :3], r["pos"][:, :3], "pos", rtol=1e-6)
     _close(g["vel"][f, :3], r["vel"][f, :3], "vel", rtol=1e-4, atol=1e-6)
     _close(g["rho"][f], r["rho"][f], "rho", rtol=1e-6)
+
+
+@pytest.mark.parametrize("box_force,wall_weight", [(0, 0), (1, 0), (0, 1)])
+def test_sph_steps_bit_exact(gpu, box_force, wall_weight):
+    """Original SPH (physicsType 'o', linear equation of state): no powf on
+    the path, so the steps are bit-exact against the oracle."""
+    p, pos, typ, nf = scenes.dam_break(16)
+    p.add_boundary_force = box_force
+    p.add_wall_weight = wall_weight
+    p.dt = 0.000253
+    f = typ == upf.UPF_LIQUID
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    o = ol.OracleSim(p, pos, None, typ)
+    sim.step(3, upf.UPF_SPH_STATIC)
+    o.step(3, upf.UPF_SPH_STATIC)
+    _compare(sim, o, f, pcisph=False)

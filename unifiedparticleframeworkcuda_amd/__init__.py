@@ -14,7 +14,7 @@ import numpy as np
 
 from ._abi import (Params, DamBreakDesc, RigidSceneDesc, Body, Tuning, ptr, ERROR_NAMES,  # noqa: F401
                    UPF_OK, UPF_ERR_RANGE, UPF_LIQUID, UPF_FROZEN, UPF_RIGID, UPF_PCISPH_STATIC,
-                   UPF_WCSPH_STATIC, UPF_PCISPH_RIGID, UPF_BOUNDARY_CPU_REF, UPF_BOUNDARY_AKINCI,
+                   UPF_WCSPH_STATIC, UPF_PCISPH_RIGID, UPF_SPH_STATIC, UPF_BOUNDARY_CPU_REF, UPF_BOUNDARY_AKINCI,
                    UPF_TEXT_LIQUID, UPF_TEXT_RIGID, UPF_TEXT_POSITIONS, UPF_TEXT_RESTART)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))

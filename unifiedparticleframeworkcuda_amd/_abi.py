@@ -35,6 +35,7 @@ UPF_TEXT_RESTART = 4
 UPF_PCISPH_STATIC = 0
 UPF_WCSPH_STATIC = 1
 UPF_PCISPH_RIGID = 2
+UPF_SPH_STATIC = 3
 
 UPF_BOUNDARY_CPU_REF = 0
 UPF_BOUNDARY_AKINCI = 1
@@ -85,6 +86,7 @@ class Params(C.Structure):
         ("add_wall_weight", C.c_int),
         ("container_min", F3),
         ("container_max", F3),
+        ("gas_const_sph", C.c_float),
     ]
 
     def copy(self):

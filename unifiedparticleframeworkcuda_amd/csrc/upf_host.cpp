@@ -128,6 +128,7 @@ int upf_params_init(float spacing, upf_params* p) {
     p->max_density_error_allowed = 1.0f;
     p->dt = 0.0012f;                       // deltaT_pcisph
     p->gas_const_wcsph = 2000.0f;
+    p->gas_const_sph = 1000.0f;  // setSimulationMethod (UnifiedConstants.cpp:260,280)
     {
         const float f = p->dt / 0.0002f;
         p->density_error_factor_parameter = (float)(0.05 * f * f);

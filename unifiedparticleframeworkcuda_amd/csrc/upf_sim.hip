@@ -1275,8 +1275,11 @@ __global__ void k_integrate(StepArgs A, Slots S, Work W, int pcisph, int pure_fl
 }
 
 // ------------------------------------------------------------------- WCSPH
-// CalculateDensityPressure with physicsType 'w' (:2339-2406), the wall weight
-// when P.add_wall_weight, + CalculatePressureWCSPH (:2491-2501), over the lists
+// CalculateDensityPressure (:2339-2406), the wall weight when
+// P.add_wall_weight, then the pressure of physicsType 'w' (TAIT,
+// CalculatePressureWCSPH :2491-2501) or 'o' (CalculatePressureSPH
+// :2479-2489), over the lists
+template <bool TAIT>
 __global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                          Ctl* ctl) {
     __shared__ float lw[LUT_N];
@@ -1298,11 +1301,18 @@ __global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Wo
     const int cnt = W.ncount[i];
     for (int k = 0; k < cnt; ++k) rho = rho + lw[W.nbr[(size_t)k * A.stride + i] & NB_BIN_MASK] * m;
     if (P.add_wall_weight) rho = rho + wall_weight<true>(P, wall_distance(P, xi.x, xi.y, xi.z), A.lut_factor, lw);
-    const float gamma = 7.0f;
-    const float b = P.rest_density * P.gas_const_wcsph / gamma;
-    const float frac = rho / P.rest_density;
-    const float pg = powf(frac, gamma);
-    const float pres = fmax_ref(0.0f, b * (pg - 1.0f));
+    float pres;
+    if constexpr (TAIT) {
+        const float gamma = 7.0f;
+        const float b = P.rest_density * P.gas_const_wcsph / gamma;
+        const float frac = rho / P.rest_density;
+        const float pg = powf(frac, gamma);
+        pres = fmax_ref(0.0f, b * (pg - 1.0f));
+    } else {  // gamma = 1: p = k (rho - rho0) written as the reference writes it
+        const float b = P.rest_density * P.gas_const_sph / 1.0f;
+        const float frac = rho / P.rest_density;
+        pres = fmax_ref(0.0f, b * (frac - 1.0f));
+    }
     W.rho[i] = rho;
     W.pres[i] = pres;
     // what the force pass needs of this particle as a neighbour, in two
@@ -2276,7 +2286,10 @@ static int step_pcisph_rigid(upf_sim* s) {
     return UPF_OK;
 }
 
-static int step_wcsph_static(upf_sim* s) {
+// The methods whose pressure is the state pressure W.pres (no correction loop).
+static inline bool sph_family(int method) { return method == UPF_WCSPH_STATIC || method == UPF_SPH_STATIC; }
+
+static int step_wcsph_static(upf_sim* s, bool tait) {
     int rc = sort_and_bin(s);
     if (rc) return rc;
     const StepArgs A = s->args();
@@ -2284,7 +2297,8 @@ static int step_wcsph_static(upf_sim* s) {
     const dim3 g = grid_for(s->n);
     build_lists(s, A);
     s->prof.begin(PC_WCSPH_DENSITY, s->st);
-    hipLaunchKernelGGL(k_density_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+    if (tait) hipLaunchKernelGGL(k_density_wcsph<true>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+    else hipLaunchKernelGGL(k_density_wcsph<false>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
     s->prof.begin(PC_WCSPH_FORCE, s->st);
     hipLaunchKernelGGL(k_force_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
@@ -2554,7 +2568,8 @@ int upf_step(upf_sim* s, int nsteps, int method) {
     for (int k = 0; k < nsteps; ++k) {
         int rc;
         if (method == UPF_PCISPH_STATIC) rc = step_pcisph_static(s, false);
-        else if (method == UPF_WCSPH_STATIC) rc = step_wcsph_static(s);
+        else if (method == UPF_WCSPH_STATIC) rc = step_wcsph_static(s, true);
+        else if (method == UPF_SPH_STATIC) rc = step_wcsph_static(s, false);
         else if (method == UPF_PCISPH_RIGID) rc = step_pcisph_rigid(s);
         else return UPF_ERR_ARG;
         s->last_method = method;
@@ -2711,7 +2726,7 @@ int upf_download(upf_sim* s, float* pos4, float* vel4, float* rho, float* p, flo
     if (get4(S.vel, vel4, false)) return UPF_ERR_HIP;
     if (get4(s->W.force, force4, false)) return UPF_ERR_HIP;
     if (get1(s->W.rho, rho)) return UPF_ERR_HIP;
-    if (get1(s->last_method == UPF_WCSPH_STATIC ? s->W.pres : s->W.p, p)) return UPF_ERR_HIP;
+    if (get1(sph_family(s->last_method) ? s->W.pres : s->W.p, p)) return UPF_ERR_HIP;
     if (get1(S.psi, psi)) return UPF_ERR_HIP;
     return UPF_OK;
 }
@@ -2777,7 +2792,7 @@ int upf_save_text(upf_sim* s, const char* path, int which) {
         p.resize(n);
         rho.resize(n);
         HIPCHK(hipMemcpy(vel.data(), S.vel, (size_t)n * 16, hipMemcpyDeviceToHost));
-        const float* src_p = s->last_method == UPF_WCSPH_STATIC ? s->W.pres : s->W.p;
+        const float* src_p = sph_family(s->last_method) ? s->W.pres : s->W.p;
         HIPCHK(hipMemcpy(p.data(), src_p, (size_t)n * 4, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(rho.data(), s->W.rho, (size_t)n * 4, hipMemcpyDeviceToHost));
     }
