@@ -449,6 +449,28 @@ int upf_read_restart(const char* path, int cap, int* n_out, float* pos4, float* 
 int upf_upload_restart(upf_sim* s, int n, const float* pos4, const float* vel4, const int* type,
                        const float* corr_pressure, const float* predicted_density);
 
+/* Rigid bodies from particles: Scene::addAllTo -> SolidifyParticle ->
+ * FreezeParticle (src/sceneConfigure/Scene.cpp:149-208,
+ * src/cpu/UnifiedPhysics.cpp:810-813,1657-1847) with AddToRigidBody
+ * (:825-883), finished as UnifiedPhysics' constructor finishes them
+ * (:141-160).  The n particles given are the solid ones (e.g. an OBJ fill of
+ * upf_mesh_fill); those closer than the support radius (|x_i - x_j| < h, the
+ * CPU path's neighbour test) belong to one body, and a body needs two
+ * particles (a lone particle gets body_id -1).  Bodies are numbered by their
+ * first particle and accumulate their particles in index order with
+ * AddToRigidBody's float arithmetic: mass, centre of mass (pos), velocity
+ * (from vel4, NULL = at rest), inertia about the centre, angular momentum;
+ * inv_inertia_local = Matrix3x3::CalculateInverse of the inertia (the zero
+ * matrix if it is singular, as there); rotation identity, quaternion
+ * (1,0,0,0); linear momentum, angular velocity and inv_inertia_world 0, as
+ * RigidBody() leaves them until the first rigid step.  rel4 (may be NULL):
+ * position - centre of mass (init_relative_pos_), w = the particle's mass.
+ * bodies == NULL: body ids and count only; cap_bodies < count: UPF_ERR_RANGE
+ * with n_bodies set.  Host code.                                           */
+int upf_bodies_from_particles(const upf_params* p, int n, const float* pos4, const float* vel4,
+                              const float* mass, int* body_id, float* rel4, upf_body* bodies,
+                              int cap_bodies, int* n_bodies);
+
 /* ------------------------------------------------------ launch tuning */
 /* Launch shapes that change speed only, never results (no reference
  * counterpart; the reference fixes them at compile time).  Defaults

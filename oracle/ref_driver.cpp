@@ -182,6 +182,17 @@ int ref_obj_particles(const char* path, float sx, float sy, float sz, float spac
     return n;
 }
 
+// Matrix3x3::CalculateInverse (src/cpu/Matrix3x3.cpp:121-158) on a row-major
+// 3x3; returns 1 if it inverted, 0 if it found the matrix singular.
+int ref_inverse3(const float* m9, float* out9) {
+    Matrix3x3 m, r;
+    r.SetZero();
+    for (int k = 0; k < 9; ++k) m.elements[k / 3][k % 3] = m9[k];
+    const bool ok = Matrix3x3::CalculateInverse(m, r);
+    for (int k = 0; k < 9; ++k) out9[k] = r.elements[k / 3][k % 3];
+    return ok ? 1 : 0;
+}
+
 // One call of a UnifiedIO writer (src/cpu/UnifiedIO.cpp:13-166) with `dir` as
 // the working directory: the writers open "K:/SimulationData/output/..."
 // relative to it, so dir must hold K:/SimulationData/output/FluidParticles and

@@ -2081,3 +2081,157 @@ extern "C" int orc_wall_weight(const upf_params* p, int n, const float* pos4, fl
     }
     return 0;
 }
+
+// ------------------------------------------------ bodies from particles
+namespace orc {
+
+// Matrix3x3::CalculateInverse (src/cpu/Matrix3x3.cpp:121-158).
+static bool calc_inverse(const float (&m)[3][3], float (&r)[3][3]) {
+    float det = m[0][0] * m[1][1] * m[2][2] - m[2][0] * m[1][1] * m[0][2] + m[0][1] * m[1][2] * m[2][0] -
+                m[2][1] * m[1][2] * m[0][0] + m[0][2] * m[1][0] * m[2][1] - m[2][2] * m[1][0] * m[0][1];
+    if (det < 1e-20 && det > -1e-20) return false;
+    float detFactor = 1 / det;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            float sub[4];
+            int e = 0;
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l)
+                    if (i != k && j != l) sub[e++] = m[k][l];
+            r[j][i] = sub[0] * sub[3] - sub[2] * sub[1];
+            r[j][i] *= ((i + j) & 0x1) ? -detFactor : detFactor;
+        }
+    return true;
+}
+
+struct BuildBody {  // the RigidBody fields AddToRigidBody touches
+    float mass = 0;
+    V3<float> pos, vel, L;
+    float I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+};
+
+// AddToRigidBody (src/cpu/UnifiedPhysics.cpp:825-883)
+static void add_to_rigid_body(BuildBody& rb, const V3<float>& ppos, const V3<float>& pvel, float partMass) {
+    float newMass = rb.mass + partMass;
+    V3<float> newCenterOfMass = (rb.pos * rb.mass + ppos * partMass).div(newMass);
+    V3<float> distCenters = rb.pos - newCenterOfMass;
+    V3<float> distCenterPart = ppos - newCenterOfMass;
+    float centersSq0 = distCenters[0] * distCenters[0];
+    float centersSq1 = distCenters[1] * distCenters[1];
+    float centersSq2 = distCenters[2] * distCenters[2];
+    float partSq0 = distCenterPart[0] * distCenterPart[0];
+    float partSq1 = distCenterPart[1] * distCenterPart[1];
+    float partSq2 = distCenterPart[2] * distCenterPart[2];
+    float (&t)[3][3] = rb.I;
+    t[0][0] += rb.mass * (centersSq1 + centersSq2) + partMass * (partSq1 + partSq2);
+    t[1][1] += rb.mass * (centersSq0 + centersSq2) + partMass * (partSq0 + partSq2);
+    t[2][2] += rb.mass * (centersSq1 + centersSq0) + partMass * (partSq1 + partSq0);
+    t[0][1] -= rb.mass * distCenters[0] * distCenters[1] + partMass * distCenterPart[0] * distCenterPart[1];
+    t[0][2] -= rb.mass * distCenters[0] * distCenters[2] + partMass * distCenterPart[0] * distCenterPart[2];
+    t[1][2] -= rb.mass * distCenters[1] * distCenters[2] + partMass * distCenterPart[1] * distCenterPart[2];
+    t[1][0] = t[0][1];
+    t[2][0] = t[0][2];
+    t[2][1] = t[1][2];
+    rb.L = rb.L + distCenters.cross(rb.vel * rb.mass);
+    rb.L += distCenterPart.cross(pvel * partMass);
+    rb.vel = (rb.vel * rb.mass + pvel * partMass).div(newMass);
+    rb.pos = newCenterOfMass;
+    rb.mass = newMass;
+}
+
+}  // namespace orc
+
+// Solidified particles -> rigid bodies (Scene::addAllTo, FreezeParticle,
+// AddToRigidBody, the body finalisation of UnifiedPhysics' constructor
+// :141-160): bodies are the connected sets of particles closer than h (every
+// pair tested), numbered by first particle, filled in particle order.
+extern "C" int orc_bodies_from_particles(const upf_params* p, int n, const float* pos4, const float* vel4,
+                                         const float* mass, int* body_id, float* rel4, upf_body* bodies,
+                                         int* n_bodies) {
+    std::vector<int> comp(n, -1);
+    int nc = 0;
+    for (int s = 0; s < n; ++s) {
+        if (comp[s] >= 0) continue;
+        std::vector<int> stack{s};
+        comp[s] = nc;
+        while (!stack.empty()) {
+            const int i = stack.back();
+            stack.pop_back();
+            for (int j = 0; j < n; ++j) {
+                if (comp[j] >= 0) continue;
+                const orc::V3<float> d(pos4[4 * i] - pos4[4 * j], pos4[4 * i + 1] - pos4[4 * j + 1],
+                                       pos4[4 * i + 2] - pos4[4 * j + 2]);
+                if (d.length_sq() < p->support_radius_sq_limit) {
+                    comp[j] = nc;
+                    stack.push_back(j);
+                }
+            }
+        }
+        ++nc;
+    }
+    std::vector<int> count(nc, 0), number(nc, -1);
+    for (int i = 0; i < n; ++i) ++count[comp[i]];
+    int nb = 0;
+    for (int i = 0; i < n; ++i) {
+        if (count[comp[i]] < 2) { body_id[i] = -1; continue; }
+        if (number[comp[i]] < 0) number[comp[i]] = nb++;
+        body_id[i] = number[comp[i]];
+    }
+    *n_bodies = nb;
+    if (!bodies) return 0;
+    std::vector<orc::BuildBody> rb(nb);
+    for (int i = 0; i < n; ++i) {
+        if (body_id[i] < 0) continue;
+        const orc::V3<float> x(pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2]);
+        const orc::V3<float> v = vel4 ? orc::V3<float>(vel4[4 * i], vel4[4 * i + 1], vel4[4 * i + 2]) : orc::V3<float>();
+        orc::add_to_rigid_body(rb[body_id[i]], x, v, mass[i]);
+    }
+    for (int b = 0; b < nb; ++b) {
+        upf_body& o = bodies[b];
+        std::memset(&o, 0, sizeof(o));
+        for (int a = 0; a < 3; ++a) {
+            o.pos[a] = rb[b].pos[a];
+            o.vel[a] = rb[b].vel[a];
+            o.angular_momentum[a] = rb[b].L[a];
+        }
+        o.mass = rb[b].mass;
+        o.quaternion[0] = 1.0f;
+        o.rotation[0] = o.rotation[4] = o.rotation[8] = 1.0f;
+        float inv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        orc::calc_inverse(rb[b].I, inv);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) o.inv_inertia_local[3 * r + c] = inv[r][c];
+    }
+    if (rel4)
+        for (int i = 0; i < n; ++i) {
+            const int b = body_id[i];
+            for (int a = 0; a < 3; ++a) rel4[4 * i + a] = b >= 0 ? pos4[4 * i + a] - rb[b].pos[a] : 0.0f;
+            rel4[4 * i + 3] = mass[i];
+        }
+    return 0;
+}
+
+// Matrix3x3::CalculateInverse of the oracle (row-major 9 floats); 1 if invertible.
+extern "C" int orc_inverse3(const float* m9, float* out9) {
+    float m[3][3], r[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int k = 0; k < 9; ++k) m[k / 3][k % 3] = m9[k];
+    const bool ok = orc::calc_inverse(m, r);
+    for (int k = 0; k < 9; ++k) out9[k] = r[k / 3][k % 3];
+    return ok ? 1 : 0;
+}
+
+// The inertia of a body (row-major 9 floats) as add_to_rigid_body builds it,
+// for the closed-form checks.
+extern "C" int orc_body_inertia(int n, const float* pos4, const float* vel4, const float* mass, float* inertia9,
+                                float* com3, float* vel3, float* L3, float* total_mass) {
+    orc::BuildBody rb;
+    for (int i = 0; i < n; ++i) {
+        const orc::V3<float> x(pos4[4 * i], pos4[4 * i + 1], pos4[4 * i + 2]);
+        const orc::V3<float> v = vel4 ? orc::V3<float>(vel4[4 * i], vel4[4 * i + 1], vel4[4 * i + 2]) : orc::V3<float>();
+        orc::add_to_rigid_body(rb, x, v, mass[i]);
+    }
+    for (int k = 0; k < 9; ++k) inertia9[k] = rb.I[k / 3][k % 3];
+    for (int a = 0; a < 3; ++a) { com3[a] = rb.pos[a]; vel3[a] = rb.vel[a]; L3[a] = rb.L[a]; }
+    *total_mass = rb.mass;
+    return 0;
+}

@@ -51,6 +51,9 @@ def _bind(lib):
     lib.orc_sim_set_restart.argtypes = [C.c_void_p, pf, pf]
     lib.orc_dump_cells.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p]
     lib.orc_wall_weight.argtypes = [pp, C.c_int, pf, pf, pf, pf]
+    lib.orc_bodies_from_particles.argtypes = [pp, C.c_int, pf, pf, pf, pi, pf, C.POINTER(Body), pi]
+    lib.orc_inverse3.argtypes = [pf, pf]
+    lib.orc_body_inertia.argtypes = [C.c_int, pf, pf, pf, pf, pf, pf, pf, pf]
     return lib
 
 
@@ -81,6 +84,7 @@ def ref_lib():
         lib.ref_box_query.argtypes = [C.c_int, C.POINTER(C.c_ushort), C.POINTER(C.c_ushort), pu, C.c_int, pu]
         lib.ref_rigid_step.argtypes = [pf, C.c_int, pf, pf, pf, C.c_float, C.c_float, C.c_float]
         lib.ref_obj_particles.argtypes = [C.c_char_p, C.c_float, C.c_float, C.c_float, C.c_float, pf, C.c_int]
+        lib.ref_inverse3.argtypes = [pf, pf]
         lib.ref_save_text.argtypes = [C.c_char_p, C.c_int, C.c_int, pf, C.POINTER(C.c_int), C.c_int]
         lib.ref_read_restart.argtypes = [C.c_char_p, C.c_int, pf, pf, C.POINTER(C.c_int), pf, pf]
         _libs["ref"] = lib
@@ -161,6 +165,46 @@ def wall_weight(p, pos4):
     d, wc, wg = (np.zeros(n, np.float32) for _ in range(3))
     oracle().orc_wall_weight(C.byref(p), n, ptr(x), ptr(d), ptr(wc), ptr(wg))
     return d, wc, wg
+
+
+def bodies_from_particles(p, pos4, mass, vel4=None):
+    """The oracle's bodies of solid particles: (body_id, rel4, list of Body)."""
+    x = np.ascontiguousarray(pos4, np.float32)
+    n = len(x)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (n,)), np.float32)
+    v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+    bid = np.zeros(n, np.int32)
+    rel = np.zeros((n, 4), np.float32)
+    nb = C.c_int()
+    L = oracle()
+    L.orc_bodies_from_particles(C.byref(p), n, ptr(x), ptr(v), ptr(m), ptr(bid, C.c_int), None, None, C.byref(nb))
+    arr = (Body * max(nb.value, 1))()
+    L.orc_bodies_from_particles(C.byref(p), n, ptr(x), ptr(v), ptr(m), ptr(bid, C.c_int), ptr(rel), arr, C.byref(nb))
+    return bid, rel, list(arr)[:nb.value]
+
+
+def inverse3(m9, ref=False):
+    """Matrix3x3::CalculateInverse: the oracle's, or the reference's own
+    (oracle/_ref; None without it).  Returns (ok, 3x3 float32)."""
+    L = ref_lib() if ref else oracle()
+    if L is None:
+        return None
+    m = np.ascontiguousarray(m9, np.float32).reshape(9)
+    out = np.zeros(9, np.float32)
+    ok = (L.ref_inverse3 if ref else L.orc_inverse3)(ptr(m), ptr(out))
+    return bool(ok), out.reshape(3, 3)
+
+
+def body_accumulate(pos4, mass, vel4=None):
+    """AddToRigidBody over all given particles in order: (inertia 3x3, com,
+    vel, angular momentum, mass), float32."""
+    x = np.ascontiguousarray(pos4, np.float32)
+    n = len(x)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (n,)), np.float32)
+    v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+    I, c, vv, L3, M = np.zeros(9, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32), np.zeros(1, np.float32)
+    oracle().orc_body_inertia(n, ptr(x), ptr(v), ptr(m), ptr(I), ptr(c), ptr(vv), ptr(L3), ptr(M))
+    return I.reshape(3, 3), c, vv, L3, float(M[0])
 
 
 REF_TEXT_FILES = {1: "FluidParticles/particles%04d.txt", 2: "RigidParticles/rigid1Particles%04d.txt",

@@ -69,3 +69,35 @@ def test_rigid_contact_steps(gpu, mode):
     for k in gb:
         scale = np.abs(rb[k]).max() + 1e-12
         assert np.abs(gb[k] - rb[k]).max() <= 1e-5 * scale, k
+
+
+@pytest.mark.parametrize("mode", [upf.UPF_BOUNDARY_CPU_REF, upf.UPF_BOUNDARY_AKINCI])
+def test_bodies_built_from_particles_drive_the_coupling_step(gpu, mode):
+    """The scene's rigid particles solidified into bodies as the reference's
+    scene set-up does (upf_bodies_from_particles: AddToRigidBody's centre of
+    mass, inertia and momenta, RigidBody()'s zero momentum and world inverse
+    inertia): 5 coupled steps agree with the oracle bit for bit."""
+    p, pos, typ, bid0, rel0, bodies0 = _scene(mode, nb=4)
+    r = typ == upf.UPF_RIGID
+    b_r, rel_r, bodies = upf.bodies_from_particles(p, pos[r], rel0[r, 3])
+    assert len(bodies) == len(bodies0) and np.all(b_r >= 0)
+    for b, b0 in zip(bodies, bodies0):
+        assert b.mass == pytest.approx(b0.mass, rel=1e-4)
+        assert np.allclose(list(b.pos), list(b0.pos), atol=1e-3)
+    bid = np.full(len(typ), -1, np.int32)
+    bid[r] = b_r
+    rel = np.zeros((len(typ), 4), np.float32)
+    rel[r] = rel_r
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    sim.upload_bodies(bodies, bid, rel)
+    o = ol.OracleSim(p, pos, None, typ)
+    o.upload_bodies(bodies, bid, rel, rel[:, 3].copy())
+    sim.step(5, upf.UPF_PCISPH_RIGID)
+    o.step(5, upf.UPF_PCISPH_RIGID)
+    g, rr = sim.state(), o.state()
+    assert np.array_equal(g["pos"][:, :3], rr["pos"][:, :3])
+    assert np.array_equal(g["vel"][:, :3], rr["vel"][:, :3])
+    gb, rb = _body_arrays(sim.bodies(len(bodies))), _body_arrays(o.bodies(len(bodies)))
+    for k in gb:
+        assert np.array_equal(gb[k], rb[k]), k

@@ -81,6 +81,7 @@ def lib():
         L.upf_read_restart.argtypes = [C.c_char_p, C.c_int, pi, pf, pf, pi, pf, pf]
         L.upf_upload_restart.argtypes = [vp, C.c_int, pf, pf, pi, pf, pf]
         L.upf_dump_cells.argtypes = [vp, C.c_char_p, C.c_char_p]
+        L.upf_bodies_from_particles.argtypes = [pp, C.c_int, pf, pf, pf, pi, pf, C.POINTER(Body), C.c_int, pi]
         for name, args in (("upf_upload_bodies", [vp, C.c_int, C.POINTER(Body), pi, pf]),
                            ("upf_download_bodies", [vp, C.POINTER(Body)])):
             if hasattr(L, name):
@@ -196,6 +197,28 @@ def read_cell_dump(particles_path, blocks_path):
         j += 4 + 12 * m
         out.append((parts, blocks))
     return out
+
+
+def bodies_from_particles(p, pos4, mass, vel4=None):
+    """Rigid bodies of solid particles as the reference's scene set-up makes
+    them (FreezeParticle + AddToRigidBody, src/cpu/UnifiedPhysics.cpp:825-883,
+    1657-1847): (body_id per particle (-1: none), rel4 (position - centre of
+    mass, w = mass), list of Body)."""
+    L = lib()
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, np.float32), (len(pos4),)), np.float32)
+    v = None if vel4 is None else np.ascontiguousarray(vel4, np.float32)
+    n = len(pos4)
+    bid = np.zeros(n, np.int32)
+    rel = np.zeros((n, 4), np.float32)
+    nb = C.c_int()
+    check("upf_bodies_from_particles", L.upf_bodies_from_particles(C.byref(p), n, ptr(pos4), ptr(v), ptr(m),
+                                                                   ptr(bid, C.c_int), None, None, 0, C.byref(nb)))
+    arr = (Body * max(nb.value, 1))()
+    check("upf_bodies_from_particles", L.upf_bodies_from_particles(C.byref(p), n, ptr(pos4), ptr(v), ptr(m),
+                                                                   ptr(bid, C.c_int), ptr(rel), arr, nb.value,
+                                                                   C.byref(nb)))
+    return bid, rel, list(arr)[:nb.value]
 
 
 def luts(p):

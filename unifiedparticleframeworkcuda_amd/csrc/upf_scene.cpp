@@ -252,3 +252,172 @@ extern "C" int upf_scene_rigid(const upf_rigid_scene_desc* d, upf_params* p, int
     if (bodies) std::memcpy(bodies, B.data(), B.size() * sizeof(upf_body));
     return UPF_OK;
 }
+
+// ---------------------------------------------------- bodies from particles
+namespace {
+
+// Matrix3x3::CalculateInverse (src/cpu/Matrix3x3.cpp:121-158): the adjugate
+// over the determinant, expression by expression; false (result untouched)
+// when |det| < 1e-20.
+bool invert3(const float m[3][3], float out[9]) {
+    const float det = m[0][0] * m[1][1] * m[2][2] - m[2][0] * m[1][1] * m[0][2] + m[0][1] * m[1][2] * m[2][0] -
+                      m[2][1] * m[1][2] * m[0][0] + m[0][2] * m[1][0] * m[2][1] - m[2][2] * m[1][0] * m[0][1];
+    if ((double)det < 1e-20 && (double)det > -1e-20) return false;
+    const float f = 1 / det;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            float s[4];
+            int e = 0;
+            for (int k = 0; k < 3; ++k)
+                for (int l = 0; l < 3; ++l)
+                    if (k != i && l != j) s[e++] = m[k][l];
+            float v = s[0] * s[3] - s[2] * s[1];
+            v *= ((i + j) & 1) ? -f : f;
+            out[3 * j + i] = v;
+        }
+    return true;
+}
+
+struct Accum {  // RigidBody fields that AddToRigidBody changes
+    float mass = 0.0f;
+    float com[3] = {0, 0, 0}, vel[3] = {0, 0, 0}, L[3] = {0, 0, 0};
+    float I[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+};
+
+void cross3(const float a[3], const float b[3], float o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// AddToRigidBody (src/cpu/UnifiedPhysics.cpp:825-883): one particle joins the
+// body; centre of mass, inertia about it, angular momentum and velocity are
+// updated incrementally in float (vmml's v / s is v * (1 / s)).
+void add_particle(Accum& b, const float x[3], const float v[3], float m) {
+    const float M = b.mass, newMass = M + m, inv = 1.0f / newMass;
+    float c[3], dc[3], dp[3];
+    for (int a = 0; a < 3; ++a) c[a] = (b.com[a] * M + x[a] * m) * inv;
+    for (int a = 0; a < 3; ++a) dc[a] = b.com[a] - c[a];
+    for (int a = 0; a < 3; ++a) dp[a] = x[a] - c[a];
+    const float cs0 = dc[0] * dc[0], cs1 = dc[1] * dc[1], cs2 = dc[2] * dc[2];
+    const float ps0 = dp[0] * dp[0], ps1 = dp[1] * dp[1], ps2 = dp[2] * dp[2];
+    b.I[0][0] += M * (cs1 + cs2) + m * (ps1 + ps2);
+    b.I[1][1] += M * (cs0 + cs2) + m * (ps0 + ps2);
+    b.I[2][2] += M * (cs1 + cs0) + m * (ps1 + ps0);
+    b.I[0][1] -= M * dc[0] * dc[1] + m * dp[0] * dp[1];
+    b.I[0][2] -= M * dc[0] * dc[2] + m * dp[0] * dp[2];
+    b.I[1][2] -= M * dc[1] * dc[2] + m * dp[1] * dp[2];
+    b.I[1][0] = b.I[0][1];
+    b.I[2][0] = b.I[0][2];
+    b.I[2][1] = b.I[1][2];
+    float mom[3], t[3];
+    for (int a = 0; a < 3; ++a) mom[a] = b.vel[a] * M;
+    cross3(dc, mom, t);
+    for (int a = 0; a < 3; ++a) b.L[a] = b.L[a] + t[a];
+    for (int a = 0; a < 3; ++a) mom[a] = v[a] * m;
+    cross3(dp, mom, t);
+    for (int a = 0; a < 3; ++a) b.L[a] += t[a];
+    for (int a = 0; a < 3; ++a) b.vel[a] = (b.vel[a] * M + v[a] * m) * inv;
+    for (int a = 0; a < 3; ++a) b.com[a] = c[a];
+    b.mass = newMass;
+}
+
+int find_root(std::vector<int>& parent, int i) {
+    while (parent[i] != i) {
+        parent[i] = parent[parent[i]];
+        i = parent[i];
+    }
+    return i;
+}
+
+}  // namespace
+
+extern "C" int upf_bodies_from_particles(const upf_params* p, int n, const float* pos4, const float* vel4,
+                                         const float* mass, int* body_id, float* rel4, upf_body* bodies,
+                                         int cap_bodies, int* n_bodies) {
+    if (!p || n < 0 || (n > 0 && (!pos4 || !mass)) || !body_id || !n_bodies || cap_bodies < 0) return UPF_ERR_ARG;
+    const float T = p->support_radius_sq_limit;
+    const double cell = (double)p->support_radius;
+    if (!(T > 0.0f) || !(cell > 0.0)) return UPF_ERR_ARG;
+    // neighbours (|x_i - x_j| < h, the CPU path's lists) through a hash grid
+    // of h-sized cells; components by union-find
+    std::vector<int> parent(n);
+    for (int i = 0; i < n; ++i) parent[i] = i;
+    struct Key {
+        long long x, y, z;
+    };
+    auto key_of = [&](int i) {
+        return Key{(long long)std::floor(pos4[4 * i] / cell), (long long)std::floor(pos4[4 * i + 1] / cell),
+                   (long long)std::floor(pos4[4 * i + 2] / cell)};
+    };
+    auto hash = [](long long x, long long y, long long z) {
+        return (uint64_t)(x * 73856093LL) ^ (uint64_t)(y * 19349663LL) ^ (uint64_t)(z * 83492791LL);
+    };
+    std::vector<std::pair<uint64_t, int>> cells((size_t)n);
+    for (int i = 0; i < n; ++i) {
+        const Key k = key_of(i);
+        cells[i] = {hash(k.x, k.y, k.z), i};
+    }
+    std::sort(cells.begin(), cells.end());
+    for (int i = 0; i < n; ++i) {
+        const Key k = key_of(i);
+        for (int dz = -1; dz <= 1; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const uint64_t h = hash(k.x + dx, k.y + dy, k.z + dz);
+                    auto it = std::lower_bound(cells.begin(), cells.end(), std::make_pair(h, -1));
+                    for (; it != cells.end() && it->first == h; ++it) {
+                        const int j = it->second;
+                        if (j <= i) continue;
+                        const float ex = pos4[4 * i] - pos4[4 * j], ey = pos4[4 * i + 1] - pos4[4 * j + 1],
+                                    ez = pos4[4 * i + 2] - pos4[4 * j + 2];
+                        if (!(ex * ex + ey * ey + ez * ez < T)) continue;
+                        const int a = find_root(parent, i), b = find_root(parent, j);
+                        if (a != b) parent[std::max(a, b)] = std::min(a, b);
+                    }
+                }
+    }
+    // bodies numbered by their first particle; a particle alone forms none
+    std::vector<int> size(n, 0), index(n, -1);
+    for (int i = 0; i < n; ++i) ++size[find_root(parent, i)];
+    int nb = 0;
+    for (int i = 0; i < n; ++i) {
+        const int r = find_root(parent, i);
+        if (size[r] < 2) {
+            body_id[i] = -1;
+            continue;
+        }
+        if (index[r] < 0) index[r] = nb++;
+        body_id[i] = index[r];
+    }
+    *n_bodies = nb;
+    if (!bodies) return UPF_OK;
+    if (nb > cap_bodies) return UPF_ERR_RANGE;
+    std::vector<Accum> acc(nb);
+    const float zero[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i)
+        if (body_id[i] >= 0) add_particle(acc[body_id[i]], &pos4[4 * i], vel4 ? &vel4[4 * i] : zero, mass[i]);
+    for (int b = 0; b < nb; ++b) {
+        upf_body& o = bodies[b];
+        std::memset(&o, 0, sizeof(o));  // RigidBody(): momenta, angular velocity, forces 0
+        const Accum& A = acc[b];
+        for (int a = 0; a < 3; ++a) {
+            o.pos[a] = A.com[a];
+            o.vel[a] = A.vel[a];
+            o.angular_momentum[a] = A.L[a];
+            o.rotation[4 * a] = 1.0f;
+        }
+        o.mass = A.mass;
+        o.quaternion[0] = 1.0f;
+        // UnifiedPhysics' constructor inverts the inertia once the scene is
+        // built (:150); a singular one leaves the zero matrix
+        invert3(A.I, o.inv_inertia_local);
+    }
+    if (rel4)
+        for (int i = 0; i < n; ++i) {  // init_relative_pos_ = position - centre of mass (:158)
+            const int b = body_id[i];
+            for (int a = 0; a < 3; ++a) rel4[4 * i + a] = b >= 0 ? pos4[4 * i + a] - acc[b].com[a] : 0.0f;
+            rel4[4 * i + 3] = mass[i];
+        }
+    return UPF_OK;
+}
