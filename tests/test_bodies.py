@@ -185,3 +185,15 @@ def test_capacity_and_count_only():
     assert L.upf_bodies_from_particles(C.byref(p), n, ptr(x), None, ptr(m), ptr(bid, C.c_int), None, arr, 1,
                                        C.byref(nb)) == UPF_ERR_RANGE
     assert nb.value > 1
+
+
+def test_non_finite_positions_are_rejected():
+    p, _, _, _ = scenes.dam_break(8)
+    x = np.zeros((3, 4), np.float32)
+    x[:, 0] = [0.1, 0.1 + float(p.spacing), np.nan]
+    m = np.full(3, np.float32(p.initial_mass))
+    with pytest.raises(upf.UPFError):
+        upf.bodies_from_particles(p, x, m)
+    x[2, 0] = np.inf
+    with pytest.raises(upf.UPFError):
+        upf.bodies_from_particles(p, x, m)

@@ -339,6 +339,8 @@ extern "C" int upf_bodies_from_particles(const upf_params* p, int n, const float
     const float T = p->support_radius_sq_limit;
     const double cell = (double)p->support_radius;
     if (!(T > 0.0f) || !(cell > 0.0)) return UPF_ERR_ARG;
+    for (int i = 0; i < 3 * n; ++i)  // finite positions, cell indices well inside 64 bits
+        if (!(std::fabs(pos4[4 * (i / 3) + i % 3] / cell) < 1e15)) return UPF_ERR_ARG;
     // neighbours (|x_i - x_j| < h, the CPU path's lists) through a hash grid
     // of h-sized cells; components by union-find
     std::vector<int> parent(n);
