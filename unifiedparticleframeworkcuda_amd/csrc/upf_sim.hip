@@ -14,6 +14,7 @@
 // -ffp-contract=off and correctly rounded division / sqrt, so results match
 // the CPU restatement bit for bit where the reference's arithmetic is IEEE.
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1881,6 +1882,24 @@ struct Prof {
         if (!on || pending.empty()) return;
         (void)hipEventRecord(pool[pending.back().second + 1], st);
     }
+    // start and stop events for a launch that records them itself
+    // (hipExtLaunchKernel: the dispatch's own start and end, no marker
+    // packets around it); false when profiling is off
+    bool span(int cls, hipEvent_t& e0, hipEvent_t& e1) {
+        if (!on) return false;
+        if (used + 2 > (int)pool.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) return false;
+                pool.push_back(e);
+            }
+        }
+        pending.push_back(std::make_pair(cls, used));
+        e0 = pool[used];
+        e1 = pool[used + 1];
+        used += 2;
+        return true;
+    }
     // after the stream has been synchronised
     void collect() {
         for (auto& pr : pending) {
@@ -2099,19 +2118,26 @@ template <int MODE>
 static void enqueue_iteration(upf_sim* s, const StepArgs& A, int t) {
     const dim3 g = grid_persist(s, s->n), b = block_loop(s);
     Slots& S = s->S[s->cur];
-    s->prof.begin(PC_DENSITY, s->st);
     const int lw = s->tune.loop_width, ll = s->tune.loop_list;
-    if (ll && lw == 8) hipLaunchKernelGGL((k_density<MODE, 1, 8>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    else if (ll) hipLaunchKernelGGL((k_density<MODE, 1, 4>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    else if (lw == 8) hipLaunchKernelGGL((k_density<MODE, 0, 8>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    else hipLaunchKernelGGL((k_density<MODE, 0, 4>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    s->prof.end(s->st);
-    s->prof.begin(PC_FORCE, s->st);
-    if (ll && lw == 8) hipLaunchKernelGGL((k_force<MODE, 1, 8>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    else if (ll) hipLaunchKernelGGL((k_force<MODE, 1, 4>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    else if (lw == 8) hipLaunchKernelGGL((k_force<MODE, 0, 8>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    else hipLaunchKernelGGL((k_force<MODE, 0, 4>), g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);
-    s->prof.end(s->st);
+    // the iteration passes are timed by their own dispatches when profiling
+    // (their device time is the figure the roofline is quoted on)
+#define UPF_PASS(KERNEL, CLS)                                                                               \
+    do {                                                                                                   \
+        hipEvent_t e0, e1;                                                                                 \
+        if (s->prof.span(CLS, e0, e1))                                                                     \
+            hipExtLaunchKernelGGL(KERNEL, g, b, 0, s->st, e0, e1, 0, t, A, S, s->W, s->luts, s->ctl);      \
+        else                                                                                               \
+            hipLaunchKernelGGL(KERNEL, g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);                    \
+    } while (0)
+    if (ll && lw == 8) UPF_PASS((k_density<MODE, 1, 8>), PC_DENSITY);
+    else if (ll) UPF_PASS((k_density<MODE, 1, 4>), PC_DENSITY);
+    else if (lw == 8) UPF_PASS((k_density<MODE, 0, 8>), PC_DENSITY);
+    else UPF_PASS((k_density<MODE, 0, 4>), PC_DENSITY);
+    if (ll && lw == 8) UPF_PASS((k_force<MODE, 1, 8>), PC_FORCE);
+    else if (ll) UPF_PASS((k_force<MODE, 1, 4>), PC_FORCE);
+    else if (lw == 8) UPF_PASS((k_force<MODE, 0, 8>), PC_FORCE);
+    else UPF_PASS((k_force<MODE, 0, 4>), PC_FORCE);
+#undef UPF_PASS
 }
 
 // PredictionCorrectionStep (:5367-5424).  The loop test runs on the device;
