@@ -59,6 +59,10 @@ extern "C" {
 #define UPF_SPH_STATIC     3  /* CpuParticlePhysicsStaticBoundariesSPH src/cpu/UnifiedPhysics.cpp:4180 with
                                  physicsType 'o': original SPH, p = max(0, k (rho - rho0))
                                  (CalculatePressureSPH :2479-2489), k = gas_const_sph */
+/* On a scene without boundary particles the two SPH methods are also the
+ * reference's default CPU driver for 'o' / 'w', CpuParticlePhysicsPureFluidsSPH
+ * (src/cpu/UnifiedPhysics.cpp:4252-4330), whose passes differ from the
+ * static-boundary driver's only in not testing the particle type.         */
 
 /* Boundary-pressure mode (Akinci boundary pressure force on liquid from
  * boundary particles).  The reference CPU path passes an integer LUT index to

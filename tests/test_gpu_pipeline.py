@@ -224,3 +224,28 @@ def test_sph_steps_bit_exact(gpu, box_force, wall_weight):
     sim.step(3, upf.UPF_SPH_STATIC)
     o.step(3, upf.UPF_SPH_STATIC)
     _compare(sim, o, f, pcisph=False)
+
+
+@pytest.mark.parametrize("method", [upf.UPF_SPH_STATIC, upf.UPF_WCSPH_STATIC])
+def test_fluid_only_scene_is_the_pure_fluid_driver(gpu, method):
+    """Without boundary particles the static-boundary SPH driver is the
+    reference's default 'o'/'w' CPU driver CpuParticlePhysicsPureFluidsSPH
+    (src/cpu/UnifiedPhysics.cpp:4252-4330: the same density, force,
+    integration and box passes, the density without the type test): the
+    fluid of the dam break alone, in its box, against the oracle."""
+    p, pos, typ, nf = scenes.dam_break(16)
+    f = typ == upf.UPF_LIQUID
+    pos, typ = pos[f], typ[f]
+    p.dt = 0.000253
+    p.gas_const_wcsph = 1000.0
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    o = ol.OracleSim(p, pos, None, typ)
+    sim.step(1, method)
+    o.step(1, method)
+    g, r = sim.state(), o.state(pcisph=False)
+    assert np.array_equal(g["rho"], r["rho"])
+    if method == upf.UPF_SPH_STATIC:
+        sim.step(4, method)
+        o.step(4, method)
+        _compare(sim, o, np.ones(len(typ), bool), pcisph=False)
