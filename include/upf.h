@@ -486,7 +486,9 @@ typedef struct upf_tuning {
     int nbr_rounds;   /* list build: candidates per lane per chunk, 2, 4 or 8 (x 64)    */
     int nbr_stage;    /* list build: entries per list staged in LDS: 0, 32 or 64       */
     int speculate;    /* host loop: iterations enqueued per read-back of the decision (1 .. 64) */
-    int loop_list;    /* correction loop reads: 0 packed 32-bit lists, 1 delta-coded   */
+    int loop_list;    /* correction loop reads: 0 packed 32-bit lists, 1 delta-coded,
+                         2 the neighbours' records from LDS through per-chunk unions
+                         (static PCISPH; needs loop_block 256)                     */
     int loop_width;   /* correction loop: neighbours whose loads are issued together, 4 or 8 */
     int nbr_tile;     /* list build: 0 cell by cell (nbr_rounds, nbr_stage), or the tile
                          form with 256 / 512 candidates staged in LDS at a time        */

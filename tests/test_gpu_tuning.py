@@ -41,6 +41,12 @@ def _run(p, pos, typ, tuning, steps=5):
          spec_first=2, xcd_remap=1),
     dict(loop_block=256, grid_cap=8192, nbr_rounds=8, nbr_stage=0, speculate=4, loop_list=1, loop_width=4, nbr_tile=0,
          spec_first=4, xcd_remap=0),
+    # union lists: neighbour records from LDS (chunks whose union is too large
+    # for it fall back to the delta-coded lists); a small grid re-stages per chunk
+    dict(loop_block=256, grid_cap=8192, nbr_rounds=4, nbr_stage=0, speculate=4, loop_list=2, loop_width=8, nbr_tile=0,
+         spec_first=4, xcd_remap=1),
+    dict(loop_block=256, grid_cap=3, nbr_rounds=4, nbr_stage=0, speculate=1, loop_list=2, loop_width=4, nbr_tile=0,
+         spec_first=0, xcd_remap=0),
 ])
 def test_tunings_give_identical_steps(gpu, tuning):
     p, pos, typ = _compressed_scene()
@@ -57,7 +63,8 @@ def test_invalid_tuning_rejected(gpu):
     sim = upf.Simulation(p, len(typ))
     for d in (dict(autotune.DEFAULT, loop_block=96), dict(autotune.DEFAULT, nbr_rounds=3),
               dict(autotune.DEFAULT, nbr_stage=16), dict(autotune.DEFAULT, grid_cap=0),
-              dict(autotune.DEFAULT, loop_list=2), dict(autotune.DEFAULT, loop_width=6),
+              dict(autotune.DEFAULT, loop_list=3), dict(autotune.DEFAULT, loop_list=2, loop_block=128),
+              dict(autotune.DEFAULT, loop_width=6),
               dict(autotune.DEFAULT, nbr_tile=128)):
         with pytest.raises(upf.UPFError):
             sim.set_tuning(d)
@@ -75,3 +82,20 @@ def test_autotuner_applies_and_caches(gpu, tmp_path):
     again, ms2 = autotune.tune_simulation(sim, pos, typ, "dam_break_test", path=path)
     assert again == best and ms2 is None  # from the cache
     sim.close()
+
+
+@pytest.mark.parametrize("mode", [upf.UPF_BOUNDARY_CPU_REF, upf.UPF_BOUNDARY_AKINCI])
+@pytest.mark.parametrize("lattice_rel", [1.0025, 0.9])
+def test_union_lists_equal_delta_coded_lists(gpu, mode, lattice_rel):
+    """Union lists (loop_list 2) against delta-coded lists (loop_list 1) on the
+    dam break at rest spacing (unions fit in LDS) and compressed (some do
+    not): the same steps bit for bit, in both boundary-pressure modes."""
+    p = upf.make_params(0.01, 0.001)
+    pos, typ, nf = upf.dam_break(p, 24, 16, 20, (0.48, 0.32, 0.4), lattice_rel=lattice_rel)
+    upf.density_error_factor(p)
+    p.boundary_pressure_mode = mode
+    ref, Kr = _run(p, pos, typ, dict(autotune.DEFAULT, loop_list=1), steps=6)
+    got, Kg = _run(p, pos, typ, dict(autotune.DEFAULT, loop_list=2), steps=6)
+    assert list(Kr) == list(Kg)
+    for k in ("pos", "vel", "rho", "p", "force"):
+        assert np.array_equal(ref[k], got[k]), k

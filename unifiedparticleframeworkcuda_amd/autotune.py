@@ -1,7 +1,8 @@
 """Autotuner of the launch shapes (upf_set_tuning): block size and grid cap
 of the grid-stride passes, candidates per lane and LDS staging depth of the
 neighbour-list build, how far the host loop enqueues PCISPH iterations
-ahead, and the list format and load width of the correction loop.  None of them changes results (every per-particle sum keeps its
+ahead, and the list format (packed, delta-coded, or chunk unions staged in
+LDS) and load width of the correction loop.  None of them changes results (every per-particle sum keeps its
 order), so candidates are compared on the same steps from the same initial
 state.  The best choice is kept per (workload, particle count) in a JSON
 cache: $UPF_TUNING_CACHE, else ~/.cache/upf/tuning.json.
@@ -11,13 +12,13 @@ import json
 import os
 import time
 
-from . import UPF_PCISPH_STATIC
+from . import UPF_PCISPH_STATIC, UPFError
 from ._abi import Tuning
 
 DEFAULT = dict(loop_block=256, grid_cap=8192, nbr_rounds=4, nbr_stage=0, speculate=4, loop_list=1, loop_width=8,
                nbr_tile=0, spec_first=4, xcd_remap=1)
 CANDIDATES = dict(loop_block=(64, 128, 256), grid_cap=(2048, 4096, 8192, 16384), nbr_rounds=(2, 4, 8),
-                  nbr_stage=(0, 32, 64), speculate=(1, 2, 4, 8), loop_list=(0, 1),
+                  nbr_stage=(0, 32, 64), speculate=(1, 2, 4, 8), loop_list=(0, 1, 2),
                   loop_width=(4, 8), nbr_tile=(0, 256, 512), spec_first=(0, 2, 4, 8, 64),
                   xcd_remap=(0, 1))
 
@@ -69,7 +70,10 @@ def coordinate_search(measure, start=None, candidates=None, sweeps=1, log=None):
     def run(t):
         k = tuple(sorted(t.items()))
         if k not in seen:
-            seen[k] = measure(dict(t))
+            try:
+                seen[k] = measure(dict(t))
+            except UPFError:  # a combination the library rejects (e.g. union lists need 256-thread blocks)
+                seen[k] = float("inf")
             if log is not None:
                 log.append((dict(t), seen[k]))
         return seen[k]

@@ -13,6 +13,13 @@ namespace upf {
 constexpr int LUT_N = 301;
 constexpr int BLOCK = 256;
 constexpr int NB_ESC = 27;     // escapes per delta-coded list at most (one per gap between stencil cells)
+// neighbour unions of the correction loop: per chunk of UNION_CHUNK
+// consecutive slots, the distinct neighbour slots of its lists (at most
+// UNION_CAP, else the chunk reads through the delta-coded lists), found
+// through an LDS hash table of UNION_TAB entries
+constexpr int UNION_CHUNK = 256;
+constexpr int UNION_CAP = 1024;
+constexpr int UNION_TAB = 2048;
 
 // Loop control of the PCISPH correction loop, kept on the device.
 constexpr int MAX_SLOTS = 64;  // the max reduction spreads its atomics over 64 cache lines
@@ -80,6 +87,13 @@ struct Work {           // per-step arrays, slot order
     uint32_t* nbf;
     uint4* nbw;
     uint32_t* nbe;
+    // the same lists as indices into their chunk's union (16-bit, eight per
+    // 16-byte word of nbu, column-major like nbw); uslots[c * UNION_CAP + u]
+    // is the slot of union entry u of chunk c, un[c] the union's size (-1:
+    // the chunk uses the delta-coded lists)
+    uint4* nbu;
+    uint32_t* uslots;
+    int* un;
     uint2* cells;       // dense cell table indexed by cell Morton code: [first, end)
 };
 

@@ -399,7 +399,35 @@ __device__ __forceinline__ T at_u32(const T* __restrict__ base, int j) {
 template <int CODED, int G, class Load, class Use>
 __device__ __forceinline__ void for_each_listed(const Work& W, int i, size_t st, int cnt, Load load, Use use) {
     using V = decltype(load(0));
-    if (CODED) {
+    if (CODED == 2) {
+        // union indices: eight 16-bit entries per row, each the neighbour's
+        // place in the chunk's union (load(u) reads it from LDS)
+        const uint4* rows = W.nbu + i;
+        int k0 = 0;
+        for (; k0 + 8 <= cnt; k0 += 8) {
+            const uint4 r = rows[(size_t)(k0 >> 3) * st];
+            const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+            int u[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) u[q] = (int)((w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu);
+#pragma unroll
+            for (int h = 0; h < 8; h += G) {
+                V v[G];
+#pragma unroll
+                for (int q = 0; q < G; ++q) v[q] = load(u[h + q]);
+#pragma unroll
+                for (int q = 0; q < G; ++q) use(u[h + q], 0u, v[q]);
+            }
+        }
+        if (k0 < cnt) {
+            const uint4 r = rows[(size_t)(k0 >> 3) * st];
+            const uint32_t w[4] = {r.x, r.y, r.z, r.w};
+            for (int q = 0; k0 + q < cnt; ++q) {
+                const int u = (int)((w[q >> 1] >> (16 * (q & 1))) & 0xFFFFu);
+                use(u, 0u, load(u));
+            }
+        }
+    } else if (CODED) {
         ListDecoder dec(W, i, st);
         int k0 = 0;
         // full rows: the next row's load is issued before this row's gathers
@@ -445,6 +473,113 @@ __device__ __forceinline__ void for_each_listed(const Work& W, int i, size_t st,
             use((int)(e >> NB_SLOT_SHIFT), e, load((int)(e >> NB_SLOT_SHIFT)));
         }
     }
+}
+
+// Union of the neighbour lists of each chunk of UNION_CHUNK consecutive slots
+// (one block per chunk), for the correction loop: the distinct slots go into
+// an LDS hash table (linear probing, at most UNION_TAB probes), are sorted
+// (bitonic, in LDS) and numbered in slot order -- so that the lanes of a wave,
+// consecutive slots, read nearby LDS words of the union and few bank
+// conflicts -- and every list is rewritten as those numbers (nbu), in list
+// order.  A chunk whose union exceeds UNION_CAP keeps un = -1 and its passes
+// read the delta-coded lists.  Reads the packed lists of the step (nbr); the
+// sums keep their order, so results do not change.
+__global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
+    __shared__ uint32_t tab[UNION_TAB];
+    __shared__ uint32_t occ[UNION_TAB / 32], pre[UNION_TAB / 32];
+    __shared__ uint32_t srt[UNION_CAP];
+    __shared__ uint16_t rank_at[UNION_TAB];
+    __shared__ int s_bad, s_total;
+    constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+    const int c = blockIdx.x, tid = threadIdx.x;
+    const int i = c * UNION_CHUNK + tid;
+    const size_t st = (size_t)A.stride;
+    const int cnt = i < A.n ? min(W.ncount[i], A.P.max_neighbors) : 0;
+    const uint32_t* L = W.nbr + i;
+    for (int k = tid; k < UNION_TAB; k += UNION_CHUNK) tab[k] = EMPTY;
+    if (tid == 0) s_bad = 0;
+    __syncthreads();
+    auto home = [](uint32_t slot) { return (slot * 2654435761u) >> (32 - 11); };  // UNION_TAB = 2^11
+    for (int k = 0; k < cnt; ++k) {
+        const uint32_t slot = L[(size_t)k * st] >> NB_SLOT_SHIFT;
+        uint32_t h = home(slot);
+        bool done = false;
+        for (int probe = 0; probe < UNION_TAB && !done; ++probe) {
+            const uint32_t old = atomicCAS(&tab[h], EMPTY, slot);
+            done = old == EMPTY || old == slot;
+            h = (h + 1) & (UNION_TAB - 1);
+        }
+        if (!done) s_bad = 1;  // table full: more than UNION_TAB distinct slots
+    }
+    __syncthreads();
+    // rank of the occupied table positions: one word of the occupancy map per
+    // lane of the first wave, then a prefix over the 64 words
+    if (tid < UNION_TAB / 32) {
+        uint32_t w = 0;
+        for (int b = 0; b < 32; ++b) w |= (tab[tid * 32 + b] != EMPTY ? 1u : 0u) << b;
+        occ[tid] = w;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const uint32_t pc = tid < UNION_TAB / 32 ? (uint32_t)__popc(occ[tid]) : 0u;
+        uint32_t incl = pc;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+            if (tid >= o) incl += y;
+        }
+        if (tid < UNION_TAB / 32) pre[tid] = incl - pc;
+        if (tid == 63) s_total = (int)incl;
+    }
+    __syncthreads();
+    const int U = s_total;
+    if (s_bad || U > UNION_CAP) {
+        if (tid == 0) W.un[c] = -1;
+        return;
+    }
+    // compact, then sort ascending (padding sorts last)
+    for (int k = tid; k < UNION_CAP; k += UNION_CHUNK) srt[k] = EMPTY;
+    __syncthreads();
+    for (int k = tid; k < UNION_TAB; k += UNION_CHUNK)
+        if (tab[k] != EMPTY) srt[pre[k >> 5] + __popc(occ[k >> 5] & ((1u << (k & 31)) - 1u))] = tab[k];
+    __syncthreads();
+    for (int size = 2; size <= UNION_CAP; size <<= 1)
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int q = tid; q < UNION_CAP / 2; q += UNION_CHUNK) {
+                const int a = 2 * q - (q & (stride - 1)), b = a + stride;  // a has bit `stride` clear
+                const bool up = (a & size) == 0;
+                const uint32_t x = srt[a], y = srt[b];
+                if ((x > y) == up) {
+                    srt[a] = y;
+                    srt[b] = x;
+                }
+            }
+            __syncthreads();
+        }
+    // number of each slot = its place in the sorted union
+    for (int q = tid; q < U; q += UNION_CHUNK) {
+        const uint32_t slot = srt[q];
+        uint32_t h = home(slot);
+        for (int probe = 0; probe < UNION_TAB && tab[h] != slot; ++probe) h = (h + 1) & (UNION_TAB - 1);
+        rank_at[h] = (uint16_t)q;
+        W.uslots[(size_t)c * UNION_CAP + q] = slot;
+    }
+    __syncthreads();
+    // the list as union numbers, eight per row
+    uint4* rows = W.nbu + i;
+    uint32_t wd[4] = {0u, 0u, 0u, 0u};
+    for (int k = 0; k < cnt; ++k) {
+        const uint32_t slot = L[(size_t)k * st] >> NB_SLOT_SHIFT;
+        uint32_t h = home(slot);
+        for (int probe = 0; probe < UNION_TAB && tab[h] != slot; ++probe) h = (h + 1) & (UNION_TAB - 1);
+        const uint32_t u = rank_at[h];
+        const int q = k & 7;
+        wd[q >> 1] |= u << (16 * (q & 1));
+        if (q == 7 || k + 1 == cnt) {
+            rows[(size_t)(k >> 3) * st] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+            wd[0] = wd[1] = wd[2] = wd[3] = 0u;
+        }
+    }
+    if (tid == 0) W.un[c] = U;
 }
 
 // GetNeighbors (src/cpu/UnifiedPhysics.cpp:2242-2295) for every liquid slot,
@@ -964,6 +1099,19 @@ __device__ inline float wave_max(float v) {
     return v;
 }
 
+// Loads the union of chunk c (records rec_src[slot] of its slots) into LDS
+// and returns its size, or -1 if the chunk reads the delta-coded lists.  All
+// threads of the block call it (it synchronises); blockDim.x == UNION_CHUNK.
+__device__ inline int stage_union(const StepArgs& A, const Work& W, const float4* __restrict__ rec_src,
+                                  float4* urec, int c) {
+    const int un = c * UNION_CHUNK < A.n ? W.un[c] : -1;
+    __syncthreads();  // the previous chunk's readers are done
+    const uint32_t* us = W.uslots + (size_t)c * UNION_CAP;
+    for (int u = threadIdx.x; u < un; u += blockDim.x) urec[u] = at_u32(rec_src, (int)us[u]);
+    __syncthreads();
+    return un;
+}
+
 // ComputePredictedDensityAndPressure (:5456-5491) + the max reduction of the
 // predicted density per scene: maxima fold into a few padded slots per scene
 // (atomics spread over cache lines).  Persistent grid-stride kernel; iteration
@@ -972,10 +1120,12 @@ __device__ inline float wave_max(float v) {
 template <int MODE, int CODED, int G>
 __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                    Ctl* ctl) {
+    static_assert(!(MODE == UPF_BOUNDARY_AKINCI && CODED == 2), "the Akinci density reads boundary volumes by slot");
     if (t >= __atomic_load_n(&ctl->want_any, __ATOMIC_RELAXED)) return;
     __shared__ float lw[LUT_N];
     __shared__ float2 th[LUT_N];
     __shared__ float wmax[BLOCK / 64];  // blockDim.x <= BLOCK
+    __shared__ float4 urec[CODED == 2 ? UNION_CAP : 1];  // the chunk's union: predicted positions
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
         lw[k] = k < 300 ? luts[k] : 0.0f;  // bin 300 (dist >= h) weighs 0, as in lut_at
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
@@ -988,6 +1138,8 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     const int n_round = (A.n + gstride() - 1) / gstride();
     for (int r = 0; r < n_round; ++r) {
         const int i = r * gstride() + xblk(A.xcd) * blockDim.x + threadIdx.x;
+        int un = -1;
+        if constexpr (CODED == 2) un = stage_union(A, W, W.xs, urec, (i - (int)threadIdx.x) / UNION_CHUNK);
         float mine = 0.0f;
         int sc = -1;
         if (i < A.n) {
@@ -1003,26 +1155,31 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 const size_t st = (size_t)A.stride;
                 // G neighbours per round: their loads are issued together,
                 // the sum still runs in list order
-                for_each_listed<CODED, G>(
-                    W, i, st, cnt, [&](int j) { return at_u32(W.xs, j); },
-                    [&](int j, uint32_t, const float4& xj) __attribute__((always_inline)) {
-                        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                        const float d2 = dx * dx + dy * dy + dz * dz;
-                        if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
-                            if (d2 < T) {
-                                const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
-                                float mj = m;
-                                if (ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
-                                rho = rho + w * mj;
-                            }
-                        } else {
-                            // without a branch: a pair at d2 >= T adds w = +0, and
-                            // rho + (+0) * m is rho exactly (rho > 0)
-                            const float wt = lw[bin_of_d2(d2, A.lut_factor, th)];
-                            const float w = d2 < T ? wt : 0.0f;
-                            rho = rho + w * m;
+                auto use = [&](int j, uint32_t, const float4& xj) __attribute__((always_inline)) {
+                    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
+                        if (d2 < T) {
+                            const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
+                            float mj = m;
+                            if (ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
+                            rho = rho + w * mj;
                         }
-                    });
+                    } else {
+                        // without a branch: a pair at d2 >= T adds w = +0, and
+                        // rho + (+0) * m is rho exactly (rho > 0)
+                        const float wt = lw[bin_of_d2(d2, A.lut_factor, th)];
+                        const float w = d2 < T ? wt : 0.0f;
+                        rho = rho + w * m;
+                    }
+                };
+                auto from_global = [&](int j) { return at_u32(W.xs, j); };
+                if constexpr (CODED == 2) {
+                    if (un >= 0) for_each_listed<2, G>(W, i, st, cnt, [&](int u) { return urec[u]; }, use);
+                    else for_each_listed<1, G>(W, i, st, cnt, from_global, use);
+                } else {
+                    for_each_listed<CODED, G>(W, i, st, cnt, from_global, use);
+                }
                 const float err = fmax_ref(rho - P.rest_density, 0.0f);
                 const float delta = A.scene_of ? A.doms[sc].delta : P.density_error_factor;
                 const float p = W.p[i] + fmax_ref(err * delta, 0.0f);
@@ -1170,13 +1327,19 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     }
     __shared__ float lpg[LUT_N];
     __shared__ float2 th[LUT_N];
+    __shared__ float4 urec[CODED == 2 ? UNION_CAP : 1];  // the chunk's union: (position, p / rho0^2)
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
         lpg[k] = luts[LUT_N + k];
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
     }
     __syncthreads();
     const upf_params& P = A.P;
-    for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
+    const int n_round = (A.n + gstride() - 1) / gstride();
+    for (int r = 0; r < n_round; ++r) {
+    const int i = r * gstride() + xblk(A.xcd) * blockDim.x + threadIdx.x;
+    int un = -1;
+    if constexpr (CODED == 2) un = stage_union(A, W, W.xpd, urec, (i - (int)threadIdx.x) / UNION_CHUNK);
+    if (i >= A.n) continue;
     const float4 xi = S.pos[i];
     if (ptype(xi) != UPF_LIQUID) continue;
     const int sc = scene_at(A, i);
@@ -1194,28 +1357,34 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     // recomputed by the same expression for the coded lists)
     // (one 16-byte record per neighbour: current position and p / rho0^2,
     // or for an Akinci boundary neighbour its mass rho0 psi, negated)
-    for_each_listed<CODED, G>(
-        W, i, st, cnt,
-        [&](int j) { const float4 r = at_u32(W.xpd, j); return PosPd{r, r.w}; },
-        [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
-            const float4& xj = q.x;
-            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-            const uint32_t bin = CODED ? bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th) : (e & NB_BIN_MASK);
-            const float g = lpg[bin];
-            // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
-            // precomputed per particle by the density pass (same division)
-            float pd = q.pd, mj = m;
-            if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
-                if (__builtin_signbit(q.pd)) {  // boundary: p_j = 0, mass rho0 psi_j
-                    mj = -q.pd;
-                    pd = 0.0f;
-                }
+    auto use = [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
+        const float4& xj = q.x;
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const uint32_t bin = CODED ? bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th) : (e & NB_BIN_MASK);
+        const float g = lpg[bin];
+        // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
+        // precomputed per particle by the density pass (same division)
+        float pd = q.pd, mj = m;
+        if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
+            if (__builtin_signbit(q.pd)) {  // boundary: p_j = 0, mass rho0 psi_j
+                mj = -q.pd;
+                pd = 0.0f;
             }
-            const float grad = pi_d + pd;
-            fx = fx - (((dx * g) * grad) * m) * mj;
-            fy = fy - (((dy * g) * grad) * m) * mj;
-            fz = fz - (((dz * g) * grad) * m) * mj;
-        });
+        }
+        const float grad = pi_d + pd;
+        fx = fx - (((dx * g) * grad) * m) * mj;
+        fy = fy - (((dy * g) * grad) * m) * mj;
+        fz = fz - (((dz * g) * grad) * m) * mj;
+    };
+    auto from_global = [&](int j) { const float4 r = at_u32(W.xpd, j); return PosPd{r, r.w}; };
+    if constexpr (CODED == 2) {
+        if (un >= 0)
+            for_each_listed<2, G>(W, i, st, cnt, [&](int u) { const float4 r = urec[u]; return PosPd{r, r.w}; }, use);
+        else
+            for_each_listed<1, G>(W, i, st, cnt, from_global, use);
+    } else {
+        for_each_listed<CODED, G>(W, i, st, cnt, from_global, use);
+    }
     const float4 Fp = make_float4(fx, fy, fz, 0.0f);
     W.fp[i] = Fp;
     W.xs[i] = predict(xi, S.vel[i], W.force[i], Fp, A, dom_at(A, i), ctl);
@@ -1852,11 +2021,11 @@ using namespace upf;
     } while (0)
 
 enum ProfClass { PC_SORT = 0, PC_GATHER, PC_CELLS, PC_PSI, PC_NBR_BUILD, PC_NBR_EXT, PC_DENSITY, PC_FORCE,
-                 PC_INTEGRATE, PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_COUNT };
+                 PC_INTEGRATE, PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_UNION, PC_COUNT };
 static const char* kProfNames[PC_COUNT] = {"radix_sort", "gather", "cell_table", "boundary_volume",
                                           "neighbour_lists", "external_force_predict", "predicted_density",
                                           "pressure_force_predict", "integrate", "wcsph_density", "wcsph_force",
-                                          "rigid_bodies"};
+                                          "rigid_bodies", "neighbour_union"};
 
 struct Prof {
     bool on = false;
@@ -1928,6 +2097,7 @@ struct upf_sim {
     bool rigid = false;
     int n_bodies = 0;
     int last_method = UPF_PCISPH_STATIC;
+    bool union_ready = false;  // this step's union lists are built (static PCISPH)
     hipStream_t st = nullptr;
     Slots S[2];
     Work W;
@@ -2129,6 +2299,17 @@ static void enqueue_iteration(upf_sim* s, const StepArgs& A, int t) {
         else                                                                                               \
             hipLaunchKernelGGL(KERNEL, g, b, 0, s->st, t, A, S, s->W, s->luts, s->ctl);                    \
     } while (0)
+    // union lists (the step built them): both passes read their neighbours'
+    // records from LDS; the Akinci density needs boundary volumes by slot and
+    // keeps the delta-coded lists
+    constexpr int DU = MODE == UPF_BOUNDARY_AKINCI ? 1 : 2;
+    if (ll == 2 && s->union_ready) {
+        if (lw == 8) UPF_PASS((k_density<MODE, DU, 8>), PC_DENSITY);
+        else UPF_PASS((k_density<MODE, DU, 4>), PC_DENSITY);
+        if (lw == 8) UPF_PASS((k_force<MODE, 2, 8>), PC_FORCE);
+        else UPF_PASS((k_force<MODE, 2, 4>), PC_FORCE);
+        return;
+    }
     if (ll && lw == 8) UPF_PASS((k_density<MODE, 1, 8>), PC_DENSITY);
     else if (ll) UPF_PASS((k_density<MODE, 1, 4>), PC_DENSITY);
     else if (lw == 8) UPF_PASS((k_density<MODE, 0, 8>), PC_DENSITY);
@@ -2245,6 +2426,13 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
     else
         hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
+    s->union_ready = s->tune.loop_list == 2;
+    if (s->union_ready) {
+        s->prof.begin(PC_UNION, s->st);
+        hipLaunchKernelGGL(k_union, dim3((unsigned)((s->n + UNION_CHUNK - 1) / UNION_CHUNK)), dim3(UNION_CHUNK), 0,
+                           s->st, A, s->W);
+        s->prof.end(s->st);
+    }
     if (device_loop) {
         if (akinci) pcisph_loop_device<UPF_BOUNDARY_AKINCI>(s, A);
         else pcisph_loop_device<UPF_BOUNDARY_CPU_REF>(s, A);
@@ -2265,6 +2453,7 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
 static int step_pcisph_rigid(upf_sim* s) {
     int rc = sort_and_bin(s);
     if (rc) return rc;
+    s->union_ready = false;  // the coupling loop reads the delta-coded lists
     const StepArgs A = s->args();
     Slots& S = s->S[s->cur];
     const dim3 g = grid_for(s->n);
@@ -2406,6 +2595,10 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.nbw, (size_t)capacity * ((p->max_neighbors + 7) / 8)));
     HIPCHK(s->alloc(&W.nbf, capacity));
     HIPCHK(s->alloc(&W.nbe, (size_t)capacity * NB_ESC));
+    HIPCHK(s->alloc(&W.nbu, (size_t)capacity * ((p->max_neighbors + 7) / 8)));
+    const size_t nchunks = ((size_t)capacity + UNION_CHUNK - 1) / UNION_CHUNK;
+    HIPCHK(s->alloc(&W.uslots, nchunks * UNION_CAP));
+    HIPCHK(s->alloc(&W.un, nchunks));
     HIPCHK(s->alloc(&W.cells, ncells));
     HIPCHK(s->alloc(&s->luts, LUT_TOTAL));
     HIPCHK(s->alloc(&s->ctl, 1));
@@ -2613,7 +2806,8 @@ int upf_set_tuning(upf_sim* s, const upf_tuning* t) {
     if (t->loop_block != 64 && t->loop_block != 128 && t->loop_block != 256) return UPF_ERR_ARG;
     if (t->grid_cap < 1 || (t->nbr_rounds != 2 && t->nbr_rounds != 4 && t->nbr_rounds != 8) ||
         (t->nbr_stage != 0 && t->nbr_stage != 32 && t->nbr_stage != 64) || t->speculate < 1 || t->speculate > 64 ||
-        (t->loop_list != 0 && t->loop_list != 1) || (t->loop_width != 4 && t->loop_width != 8) ||
+        (t->loop_list < 0 || t->loop_list > 2) || (t->loop_list == 2 && t->loop_block != UNION_CHUNK) ||
+        (t->loop_width != 4 && t->loop_width != 8) ||
         (t->nbr_tile != 0 && t->nbr_tile != 256 && t->nbr_tile != 512) || t->spec_first < 0 || t->spec_first > 64 ||
         (t->xcd_remap != 0 && t->xcd_remap != 1))
         return UPF_ERR_ARG;
