@@ -115,3 +115,23 @@ def test_single_scene_graph_equals_plain(gpu, mode):
     assert np.array_equal(st["vel"][:, :3], g["vel"][:, :3])
     assert list(K[:, 0]) == list(Kp)
     assert np.array_equal(r[:, 0], rp)
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_with_union_lists_equals_plain(gpu, use_graph):
+    """The batch with the correction loop on per-chunk unions (loop_list 2:
+    chunks of 256 slots that cross scene boundaries) equals the plain
+    per-scene runs, graph-replayed or not."""
+    from unifiedparticleframeworkcuda_amd import autotune
+    nsteps = 5
+    scenes = _scenes(seed=13)
+    ref = _plain(scenes, nsteps)
+    b = upf.Batch.from_scenes(scenes)
+    b.set_tuning(dict(autotune.DEFAULT, loop_list=2))
+    b.step(nsteps, use_graph=use_graph)
+    st = b.state()
+    b.close()
+    for k, (pos, vel) in enumerate(zip(_split(scenes, st["pos"]), _split(scenes, st["vel"]))):
+        g = ref[k][0]
+        assert np.array_equal(pos[:, :3], g["pos"][:, :3]), "scene %d positions" % k
+        assert np.array_equal(vel[:, :3], g["vel"][:, :3]), "scene %d velocities" % k
