@@ -8,7 +8,7 @@
   ideal lattice, conservation laws of the step).
 The reference ships no tests, fixtures or golden vectors for this path, so
 these are the pins; tests/golden/ holds vectors written from oracle/_ref for
-the GPU box, where the reference tree is absent.
+machines where the reference tree is absent.
 """
 import ctypes as C
 import json
@@ -357,7 +357,7 @@ def test_fp32_oracle_tracks_fp64_twin():
 # --------------------------------------------------- golden vectors
 def test_golden_vectors_from_reference_units():
     """Vectors written by tests/golden/make_golden.py from oracle/_ref; they
-    pin the oracle where the reference tree is absent (the GPU box)."""
+    pin the oracle where the reference tree is absent."""
     path = os.path.join(GOLDEN, "reference_units.json")
     if not os.path.exists(path):
         pytest.skip("golden vectors not generated")

@@ -176,7 +176,7 @@ constexpr int MF_FACES = 256;  // faces per LDS stage (20 KB)
 // order).  For each face in file order: Polygon::DoesLineIntersect of the
 // segment from o to the point toggles the point; otherwise
 // Polygon::DoesPointTouch keeps it and ends the scan.  keep[] gets the
-// verdict, counts[] the kept points of each block.
+// decision, counts[] the kept points of each block.
 __global__ __launch_bounds__(MF_BLOCK) void k_mesh_test(const Face* __restrict__ faces, int nf,
                                                         const float* __restrict__ X, const float* __restrict__ Y,
                                                         const float* __restrict__ Z, int nx, int ny, int nz, float3 o,
