@@ -455,8 +455,22 @@ def cpu_baseline(args, budget_s):
         if el >= budget_s or steps >= args.cpu_max_steps:
             break
     threads = ol.oracle(True).orc_num_threads()
-    return dict(value=nf * steps / el, unit="particle-updates/s", cores=int(threads), kind="port",
-                sample="first %d step(s) of the same 1M dam break from its initial state, %.1f s" % (steps, el))
+    out = dict(value=nf * steps / el, unit="particle-updates/s", cores=int(threads), kind="port",
+               sample="first %d step(s) of the same 1M dam break from its initial state, %.1f s" % (steps, el))
+    try:
+        with open("/proc/cpuinfo") as f:
+            out["cpu_model"] = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    if args.cpu_single_steps > 0:
+        # the same first step(s) on one thread (the oracle's serial build)
+        o1 = ol.OracleSim(p, pos, None, typ, omp=False)
+        t0 = time.perf_counter()
+        o1.step(args.cpu_single_steps, upf.UPF_PCISPH_STATIC)
+        el1 = time.perf_counter() - t0
+        out["single_thread"] = {"value": nf * args.cpu_single_steps / el1, "cores": 1,
+                                "sample": "first %d step(s), %.1f s" % (args.cpu_single_steps, el1)}
+    return out
 
 
 def main():
@@ -467,6 +481,7 @@ def main():
     ap.add_argument("--lattice", type=int, nargs=3, default=[128, 64, 128])
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-max-steps", type=int, default=20)
+    ap.add_argument("--cpu-single-steps", type=int, default=1, help="steps of the single-thread CPU sample (0: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=10)
