@@ -504,7 +504,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the workload is one scene on one MI355X "
-              "(multi-GPU sharding is a design note, not part of this build)", file=sys.stderr)
+              "(how the step would shard over several GPUs is a design note only)", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
