@@ -28,3 +28,14 @@ def test_step_bytes_sums_the_per_pass_model():
         passes = [20 * n, 12 * n, 76 * n, 4 * n + 8 * c, 40 * n + 36 * nf, 96 * nf]
         per_iter = [80 * nf, 16 * n + 12 * nf, 20 * n + 16 * nf]
         assert bench.step_bytes(n, nf, c, K) == sum(passes) + K * sum(per_iter)
+
+
+def test_more_than_one_gpu_is_refused_with_a_message():
+    """The workload is one scene on one GPU: a request for several GPUs ends
+    with exit status 2 and says why, before any device is touched."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
