@@ -189,6 +189,31 @@ def test_pcisph_config0_steps_bit_exact(gpu):
     assert np.array_equal(keys, st["keys"]) and np.array_equal(ids, st["ids"])
 
 
+def test_pcisph_config0_100_steps(gpu):
+    """Config 0 over 100 steps: the per-step iteration counts and maximum
+    densities, the state every 25 steps, and from it the kinetic energy and
+    the mean and maximum density errors, identical to the oracle."""
+    p, pos, typ, nf = scenes.dam_break(32)
+    f = typ == upf.UPF_LIQUID
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    o = ol.OracleSim(p, pos, None, typ)
+    rho0 = float(p.rest_density)
+    for _ in range(4):
+        sim.step(25, upf.UPF_PCISPH_STATIC)
+        o.step(25, upf.UPF_PCISPH_STATIC)
+        g, r = _compare(sim, o, f)
+        ke = [0.5 * float(p.initial_mass) * np.sum(s["vel"][f, :3].astype(np.float64) ** 2) for s in (g, r)]
+        err = [np.abs(s["rho"][f].astype(np.float64) - rho0) / rho0 for s in (g, r)]
+        assert ke[0] == ke[1] and ke[0] > 0.0
+        assert err[0].mean() == err[1].mean() and err[0].max() == err[1].max()
+    kg, rg = sim.stats()
+    ko, ro = o.stats()
+    assert len(kg) == 100
+    assert np.array_equal(kg, ko)
+    assert np.array_equal(rg, ro)
+
+
 @pytest.mark.parametrize("box_force", [0, 1])
 def test_wcsph_steps(gpu, box_force):
     p, pos, typ, nf = scenes.dam_break(16)
