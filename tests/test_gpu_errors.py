@@ -77,3 +77,21 @@ def test_isolated_wall_particle_allowed_with_akinci_boundaries(gpu):
     assert g["psi"][-1] == r["psi"][-1] and g["psi"][-1] > 0.0
     f = typ2 == upf.UPF_LIQUID
     assert np.array_equal(g["pos"][f, :3], r["pos"][f, :3])
+
+
+def test_upload_after_an_error_continues(gpu):
+    """A failing step leaves the handle usable: an upload clears the error
+    state, and the next steps match the oracle bit for bit."""
+    p, pos, typ, nf = scenes.dam_break(16)
+    pos2, typ2 = _with_lone_wall_particle(p, pos, typ)
+    sim = upf.Simulation(p, len(typ2))
+    with pytest.raises(upf.UPFError):
+        sim.upload(pos2, typ2)
+        sim.step(1, upf.UPF_PCISPH_STATIC)
+    sim.upload(pos, typ)
+    sim.step(2, upf.UPF_PCISPH_STATIC)
+    o = ol.OracleSim(p, pos, None, typ)
+    o.step(2, upf.UPF_PCISPH_STATIC)
+    g, r = sim.state(), o.state()
+    assert np.array_equal(g["pos"][:, :3], r["pos"][:, :3])
+    assert np.array_equal(g["vel"][:, :3], r["vel"][:, :3])
