@@ -18,6 +18,11 @@
 //   * uninitialised fields of the reference particle (predicted density and
 //     previous correction pressure before the first step) start at rho0 / 0.
 //
+// Planted mistakes: a build with -DORC_PLANT=k (k > 0) carries mistake k of
+// the list in tests/test_oracle_pins.py (a flipped sign, m instead of m^2,
+// explicit instead of symplectic Euler, ...), so that those tests can show that
+// their pins fail on each.  The default build (ORC_PLANT 0) carries none.
+//
 // Parity status per function (see tests/test_oracle_*.py):
 //   params / LUTs / z-table / block size / BoxQuery ranges / rigid-body
 //   integration : pinned against the reference's own translation units compiled
@@ -47,6 +52,10 @@
 #endif
 
 namespace orc {
+
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
 
 template <class T> struct V3 {
     T x, y, z;
@@ -398,13 +407,14 @@ template <class T> class Sim {
                     if (q.type == UPF_LIQUID) {
                         T kv = L.visc(dist);
                         V3<T> vij = p.vel - q.vel;
-                        p.force -= vij * restVolume * restVolume * (T)P.visc_const * kv;
+                        if (ORC_PLANT == 5) p.force += vij * restVolume * restVolume * (T)P.visc_const * kv;
+                        else p.force -= vij * restVolume * restVolume * (T)P.visc_const * kv;
                     } else if (q.type == UPF_FROZEN) {
                         p.force += boundary_pressure(dl, p.pos, q.pos, p.rho_pred, own_pressure(p), boundary_volume(p, q));
                     }
                 }
             }
-            p.force.y -= (T)P.initial_mass * (T)P.gravity;
+            p.force.y -= (ORC_PLANT == 13 ? T(1) : (T)P.initial_mass) * (T)P.gravity;
             if (P.add_boundary_force) p.force += box_force_static(p.pos);
         }
     }
@@ -505,13 +515,15 @@ template <class T> class Sim {
                         p.force -= vij * restVolume * restVolume * (T)P.visc_const * kv;
                         if (dist >= h() * T(0.25)) {  // CalculateSurfaceTensionForcePCISPHHost (:1990-1998)
                             T sc = L.st[dl];
-                            p.force -= (p.pos - q.pos) * (T)P.surface_tension_gamma * m * m * sc * (T(1) / dist);
+                            if (ORC_PLANT == 9) p.force += (p.pos - q.pos) * (T)P.surface_tension_gamma * m * m * sc * (T(1) / dist);
+                            else p.force -= (p.pos - q.pos) * (T)P.surface_tension_gamma * m * m * sc * (T(1) / dist);
                         }
                     } else if (q.type == UPF_FROZEN || q.type == UPF_RIGID) {
                         p.force += boundary_pressure(dl, p.pos, q.pos, p.rho_pred, own_pressure(p), boundary_volume(p, q));
                         if (dist >= h() * T(0.25)) {  // CalculateSurfaceCohesionForcePCISPHHost (:2000-2008)
                             T sc = L.ad[dl];
-                            p.force -= (p.pos - q.pos) * (T)P.surface_adhesion_beta * m * (T)P.rest_density * q.psi * sc * (T(1) / dist);
+                            if (ORC_PLANT == 10) p.force += (p.pos - q.pos) * (T)P.surface_adhesion_beta * m * (T)P.rest_density * q.psi * sc * (T(1) / dist);
+                            else p.force -= (p.pos - q.pos) * (T)P.surface_adhesion_beta * m * (T)P.rest_density * q.psi * sc * (T(1) / dist);
                         }
                     }
                 } else {  // rigid
@@ -525,6 +537,8 @@ template <class T> class Sim {
                             V3<T> vij = q.vel - p.vel;  // CalculateDampingForceHost (:2021-2030)
                             V3<T> fd((T)P.damping_coefficient * vij.x, (T)P.damping_coefficient * vij.y,
                                      (T)P.damping_coefficient * vij.z);
+                            if (ORC_PLANT == 11) fs = fs * T(-1);
+                            if (ORC_PLANT == 12) fd = fd * T(-1);
                             p.force += fs + fd;
                         }
                     } else if (q.type == UPF_LIQUID) {
@@ -553,7 +567,7 @@ template <class T> class Sim {
         if (p.type != UPF_LIQUID) return;
         p.vel_pred = p.vel + ((p.force + p.fp) * (T)P.dt).div((T)P.initial_mass);
         p.pos_pred = p.pos + p.vel_pred * (T)P.dt;
-        for (int a = 0; a < 3; ++a) {
+        for (int a = 0; a < 3 && ORC_PLANT != 7; ++a) {
             if (p.pos_pred[a] < (T)P.collision_min[a]) { p.pos_pred[a] = (T)P.collision_min[a]; p.vel_pred[a] *= T(0); }
             if (p.pos_pred[a] > (T)P.collision_max[a]) { p.pos_pred[a] = (T)P.collision_max[a]; p.vel_pred[a] *= T(0); }
         }
@@ -616,7 +630,7 @@ template <class T> class Sim {
         if (p.type != UPF_LIQUID) return T(0);
         const std::vector<int>& nb = nbrs[i];
         const T m = (T)P.initial_mass;
-        p.rho_pred = kernel_self * m;
+        p.rho_pred = ORC_PLANT == 14 ? T(0) : kernel_self * m;
         if (P.add_wall_weight) p.rho_pred += wall_weight(distance_to_wall(p.pos_pred), false);
         for (size_t k = 0; k < nb.size(); ++k) {
             const Particle<T>& q = ps[nb[k]];
@@ -627,7 +641,8 @@ template <class T> class Sim {
             }
         }
         p.rho_err = MAXr<T>(p.rho_pred - (T)P.rest_density, T(0));
-        p.corr_p += MAXr<T>(p.rho_err * (T)P.density_error_factor, T(0));
+        if (ORC_PLANT == 15) p.corr_p = MAXr<T>(p.rho_err * (T)P.density_error_factor, T(0));
+        else p.corr_p += MAXr<T>(p.rho_err * (T)P.density_error_factor, T(0));
         return p.rho_pred;
     }
 
@@ -645,11 +660,13 @@ template <class T> class Sim {
             const Particle<T>& q = ps[nb[k]];
             T dist = dist3<T>(p.pos, q.pos);
             if (dist < h()) {
-                T pj = q.corr_p;
+                T pj = ORC_PLANT == 3 ? pi : q.corr_p;
                 T g = L.grad(dist);
                 V3<T> kg = (p.pos - q.pos) * g;
                 T grad = pi / densSq + pj / (rho0 * rho0);
-                p.fp -= kg * grad * m * mass_of(q);
+                if (ORC_PLANT == 1) p.fp += kg * grad * m * mass_of(q);
+                else if (ORC_PLANT == 2) p.fp -= kg * grad * m;
+                else p.fp -= kg * grad * m * mass_of(q);
             }
         }
     }
@@ -686,13 +703,18 @@ template <class T> class Sim {
         if (pcisph) p.force += p.fp;
         const T invMass = (T)(1.0 / (double)P.initial_mass);
         const T dt = (T)P.dt;
-        p.vel += p.force * invMass * dt;
-        p.pos += p.vel * dt;
+        if (ORC_PLANT == 4) {  // explicit Euler
+            p.pos += p.vel * dt;
+            p.vel += p.force * invMass * dt;
+        } else {
+            p.vel += p.force * invMass * dt;
+            p.pos += p.vel * dt;
+        }
     }
 
     // BoundaryHandlingBoxPerNonRigidParticle (:4000-4044)
     void box_clamp(Particle<T>& p) {
-        const T damping = T(-0.5);
+        const T damping = ORC_PLANT == 6 ? T(0.5) : T(-0.5);
         const int order[3] = {1, 0, 2};
         for (int o = 0; o < 3; ++o) {
             int a = order[o];
@@ -738,7 +760,7 @@ template <class T> class Sim {
         const T gamma = T(7.0);
         const T b_i = (T)P.rest_density * (T)P.gas_const_wcsph / gamma;
         const T frac = p.density / (T)P.rest_density;
-        const T pg = vpow<T>(frac, gamma);
+        const T pg = vpow<T>(frac, ORC_PLANT == 8 ? T(6.0) : gamma);
         p.pressure = MAXr<T>(T(0), b_i * (pg - T(1)));
     }
 
@@ -760,9 +782,11 @@ template <class T> class Sim {
                 const V3<T> dv = p.pos - q.pos;
                 if (q.type == UPF_LIQUID) {
                     V3<T> kg = dv * L.grad(dist);
-                    T grad = p.pressure * invDensity * invDensity + q.pressure / (q.density * q.density);
+                    T grad = p.pressure * invDensity * invDensity +
+                             (ORC_PLANT == 17 ? p.pressure * invDensity * invDensity : q.pressure / (q.density * q.density));
                     V3<T> tp = kg * grad * m * m;
-                    force -= tp;
+                    if (ORC_PLANT == 16) force += tp;
+                    else force -= tp;
                     V3<T> vij = p.vel - q.vel;
                     force -= vij * pVol * nVol * (T)P.visc_const * kv;
                 }

@@ -3,6 +3,7 @@ translation units compiled into oracle/_ref/libupfref.so.
 
 Test infrastructure only: the oracle is the checker, never the thing measured.
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -66,6 +67,27 @@ def oracle(omp=False):
             raise RuntimeError("oracle not built: run __graft_entry__.build() (%s missing)" % path)
         _libs[name] = _bind(C.CDLL(path))
     return _libs[name]
+
+
+@contextlib.contextmanager
+def planted(path):
+    """Inside the block, oracle() (both builds) returns the library at `path`:
+    the oracle built with a planted mistake (-DORC_PLANT=k), for the tests that
+    show their pins fail on it."""
+    saved = dict(_libs)
+    lib = _bind(C.CDLL(path))
+    _libs["liborc.so"] = lib
+    _libs["liborc_omp.so"] = lib
+    try:
+        yield lib
+    finally:
+        _libs.clear()
+        _libs.update(saved)
+
+
+def ref_built():
+    """True when oracle/_ref/libupfref.so exists (without loading it)."""
+    return os.path.exists(os.path.join(ORACLE_DIR, "_ref", "libupfref.so"))
 
 
 def ref_lib():
