@@ -245,7 +245,8 @@ typedef struct upf_sim upf_sim;
 int upf_create(const upf_params* p, int capacity, upf_sim** out);
 /* FreeDeviceArrays (src/gpu/particles.cu). */
 int upf_destroy(upf_sim* s);
-/* Replace the parameters (dt, stiffness, loop limits ...); the domain must not change. */
+/* Replace the parameters (dt, stiffness, loop limits ...); the domain (block
+ * size and shift, grid, scales, support radius, list capacity) must not change. */
 int upf_set_params(upf_sim* s, const upf_params* p);
 
 /* CopyZIndicesKVPairHostToDevice + CopyParticleDataHostToDevice
@@ -257,7 +258,10 @@ int upf_set_params(upf_sim* s, const upf_params* p);
 int upf_upload_particles(upf_sim* s, int n, const float* pos4, const float* vel4,
                          const int* type);
 /* CopyRigidBodyDataHostToDevice (src/gpu/particles.cu:563): bodies and the
- * per-particle body id / initial relative position (upload order).        */
+ * per-particle body id / initial relative position (upload order).  Body
+ * ids must lie in [-1, n_bodies): otherwise UPF_ERR_ARG and nothing on the
+ * device changes.  Any other failure leaves the simulation without bodies
+ * (a rigid step then returns UPF_ERR_STATE) until an upload succeeds.      */
 int upf_upload_bodies(upf_sim* s, int n_bodies, const upf_body* bodies,
                       const int* body_id, const float* rel4);
 
@@ -303,6 +307,7 @@ int upf_download_neighbors(upf_sim* s, int* counts, int32_t* flat, int cap);
 int upf_keys(const upf_params* p, int n, const float* pos4, uint32_t* keys);
 int upf_radix_sort(int n, const uint32_t* keys_in, uint32_t* keys_out, int32_t* perm_out);
 
+
 /* ------------------------------------------------------ batched form */
 /* Many independent scenes stepped together (no reference counterpart; the
  * reference holds one simulation per process).  Scenes share the physics
@@ -332,7 +337,9 @@ int upf_batch_download(upf_batch* b, float* pos4, float* vel4, float* rho);
 /* iterations / max_rho as [step][scene] for up to max_steps steps; returns
  * the number of steps recorded (negative UPF_ERR_* on failure).           */
 int upf_batch_get_stats(upf_batch* b, int max_steps, int* iterations, float* max_rho);
-/* The batch's simulation handle, for the profiling and download calls.    */
+/* The batch's simulation handle, for the profiling and download calls.
+ * upf_set_params / upf_set_tuning on it apply to the graph replays too: the
+ * batch recaptures its step graph when either changed since the capture.  */
 int upf_batch_sim(upf_batch* b, upf_sim** sim);
 
 /* ------------------------------------------------- differentiable form */

@@ -122,3 +122,29 @@ def test_batch_create_validates_before_touching_a_device():
         L.upf_batch_destroy(h)
     else:
         assert rc == upf._abi.UPF_ERR_NO_DEVICE
+
+
+def test_inconsistent_domains_are_rejected_before_the_device():
+    """Cell tables hold grid_dim^3 entries indexed by key >> 3 block_shift:
+    every creation call rejects params whose block_shift, block_size and
+    grid_dim disagree (UPF_ERR_ARG, before any device call)."""
+    L = upf.lib()
+    base = upf.make_params(0.01, 0.001)
+    upf.dam_break(base, 8, 8, 8, (0.16, 0.08, 0.08))
+    assert base.block_size == 1 << base.block_shift and base.grid_dim == 1024 // base.block_size
+
+    def variants():
+        for field, value in (("block_shift", base.block_shift - 1), ("block_shift", base.block_shift + 1),
+                             ("grid_dim", base.grid_dim * 2), ("grid_dim", base.grid_dim // 2),
+                             ("block_size", base.block_size * 3 // 2), ("block_shift", 31)):
+            q = base.copy()
+            setattr(q, field, value)
+            yield field, q
+
+    h = C.c_void_p()
+    for field, q in variants():
+        assert L.upf_create(C.byref(q), 1000, C.byref(h)) == 2, field  # UPF_ERR_ARG
+        arr = (Params * 2)(base, q)
+        cnt = (C.c_int * 2)(100, 100)
+        assert L.upf_batch_create(2, arr, cnt, 4, C.byref(h)) == 2, field  # UPF_ERR_ARG
+        assert L.upf_adjoint_create(C.byref(q), 1000, 4, 2, 2, C.byref(h)) == 2, field  # UPF_ERR_ARG

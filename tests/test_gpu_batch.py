@@ -3,6 +3,8 @@ scene, exactly what a plain simulation of that scene gives (same kernels, same
 per-scene neighbour order, per-scene loop control), with and without the
 whole step replayed as a hipGraph; and the plain path equals the oracle.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -135,3 +137,27 @@ def test_batch_with_union_lists_equals_plain(gpu, use_graph):
         g = ref[k][0]
         assert np.array_equal(pos[:, :3], g["pos"][:, :3]), "scene %d positions" % k
         assert np.array_equal(vel[:, :3], g["vel"][:, :3]), "scene %d velocities" % k
+
+
+def test_batch_graph_follows_set_params(gpu):
+    """upf_set_params on the batch's simulation handle after the step graph
+    was captured (a shorter time step, fewer loop iterations): the graph is
+    recaptured, so graph replays give exactly what the host-loop batch gives."""
+    scenes = _scenes(n_scenes=3, seed=9)
+    out = []
+    for use_graph in (True, False):
+        b = upf.Batch.from_scenes(scenes)
+        b.step(2, use_graph=use_graph)  # the graph is captured here
+        q = scenes[0][0].copy()
+        q.dt = 0.0008
+        q.max_loops = 7
+        upf.check("upf_set_params", upf.lib().upf_set_params(b.sim_handle(), C.byref(q)))
+        b.step(3, use_graph=use_graph)
+        st = b.state()
+        K, r = b.stats()
+        out.append((st, K, r))
+        b.close()
+    (a, Ka, ra), (g, Kg, rg) = out
+    assert np.array_equal(a["pos"], g["pos"]) and np.array_equal(a["vel"], g["vel"])
+    assert np.array_equal(Ka, Kg) and np.array_equal(ra, rg)
+    assert Ka[2:].max() <= 7

@@ -48,6 +48,7 @@
 
 #include "upf.h"
 #include "upf_device.h"
+#include "upf_internal.h"
 #include "upf_kernels.h"
 #include "upf_sort.h"
 
@@ -617,7 +618,7 @@ int upf_adjoint_destroy(upf_adjoint* a) {
 
 int upf_adjoint_create(const upf_params* p, int n, int steps, int iterations, int checkpoint_every, upf_adjoint** out) {
     if (!p || !out || n <= 0 || steps <= 0 || iterations <= 0 || checkpoint_every <= 0) return UPF_ERR_ARG;
-    if (p->max_neighbors <= 0 || p->max_neighbors > 4096 || p->grid_dim == 0 || p->grid_dim > 512) return UPF_ERR_ARG;
+    if (upf::check_params(p) != UPF_OK) return UPF_ERR_ARG;
     if (p->add_wall_weight) return UPF_ERR_ARG;  // the differentiable step has no wall weight
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPF_ERR_NO_DEVICE;

@@ -105,6 +105,28 @@ float sqrt_threshold(float h) {
     return t;
 }
 
+int check_params(const upf_params* p) {
+    if (!p) return UPF_ERR_ARG;
+    // the z-index grid: block_size = 2^block_shift cells of the 1024-wide
+    // grid per axis (ZIndex::SetBlockSize, src/cpu/zIndex.cpp:27-40), so a
+    // cell code key >> 3 block_shift indexes a grid_dim^3 table
+    const uint32_t b = p->block_size;
+    if (p->block_shift < 1 || p->block_shift > 9 || b != (1u << p->block_shift)) return UPF_ERR_ARG;
+    if (p->grid_dim != 1024u / b) return UPF_ERR_ARG;
+    // the list builds take candidates from the 3x3x3 cells around a particle's
+    // cell: the query box [x - h, x + h] must not reach a fourth cell, so a
+    // cell is wider than the scaled support radius by more than rounding
+    for (int a = 0; a < 3; ++a)
+        if (!((float)b > p->scales[a] * p->support_radius * 1.0001f)) return UPF_ERR_ARG;
+    if (p->vbox_min[0] != 0.0f || p->vbox_min[1] != 0.0f || p->vbox_min[2] != 0.0f) return UPF_ERR_ARG;
+    if (p->max_neighbors <= 0 || p->max_neighbors > 4096) return UPF_ERR_ARG;
+    if (p->add_wall_weight != 0 && p->add_wall_weight != 1) return UPF_ERR_ARG;
+    if (p->add_wall_weight)
+        for (int a = 0; a < 3; ++a)
+            if (!(p->container_min[a] < p->container_max[a])) return UPF_ERR_ARG;
+    return UPF_OK;
+}
+
 }  // namespace upf
 
 using namespace upf;

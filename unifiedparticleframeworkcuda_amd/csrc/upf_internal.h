@@ -18,6 +18,11 @@ float sqrt_threshold(float h);
 uint64_t splitmix64(uint64_t& s);
 double uniform01(uint64_t& s);
 int choose_domain(upf_params* p, const float box[3], float pad);
+// UPF_OK when p describes a domain every device path can index: block_size a
+// power of two with block_shift = log2(block_size) and grid_dim = 1024 /
+// block_size (cell tables hold grid_dim^3 entries indexed by key >> 3 shift),
+// cells wider than the scaled support radius, list capacity in range.
+int check_params(const upf_params* p);
 // Text file of `which` (UPF_TEXT_*) from host arrays; vel4, p and rho are
 // read for UPF_TEXT_RESTART only.
 int write_text(const char* path, int which, int n, const float* pos4, const int* type, const float* vel4,

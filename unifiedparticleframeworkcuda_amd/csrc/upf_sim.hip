@@ -95,6 +95,8 @@ struct StepArgs {
     int n_scenes;
     int slots_per_scene;
     int code_lists;      // the correction loop reads delta-coded lists (the external-force pass writes them)
+    uint32_t st_base;    // geometry of the kernel step tables (step_geom of support_radius_sq_limit)
+    float st_low;
     int xcd;             // remap blocks to XCDs by contiguous ranges (grids are multiples of 8)
 };
 
@@ -295,18 +297,93 @@ __device__ __forceinline__ uint32_t bin_of_d2(float d2, float factor, const floa
     return b0 - (d2 < t.x ? 1u : 0u) + (d2 >= t.y ? 1u : 0u);
 }
 
+// Kernel values as step functions of the squared distance.  A pass that
+// needs V[bin(d2)] (V one of the tables, bin as above) reads it from a table
+// of cells of the float d2 instead: ST_OCT octaves of d2 below and around
+// the support radius squared, 256 cells per octave (cell = the top 8
+// mantissa bits), each cell {t, v}: v = the value at the cell's first float,
+// t = the first float of the cell at which the value changes (+inf if none).
+// A cell is narrower than any bin (a bin spans a relative distance of at
+// least 1/300, a cell 1/512), so it holds at most one change; the host
+// builds the cells from the exact thresholds and upf_create checks every
+// float d2 on the device.  The value is v of the cell, or v of the next cell
+// once d2 >= t -- one LDS read of two entries, a compare and a select, no
+// square root.  Squared distances below the first cell (pairs closer than
+// about h / 6, rare) take the square-root path; those above the last cell
+// clamp to it (its value there is that of every farther pair).
+constexpr int ST_OCT = 6;
+constexpr int ST_CELLS = 256 * ST_OCT;
+constexpr int ST_STRIDE = 2 * (ST_CELLS + 1) + 2;  // floats per table (+ a sentinel cell, 16-byte aligned)
+constexpr int ST_OFF = (LUT_TOTAL + 3) / 4 * 4;     // float offset of the step tables in the LUT buffer
+enum { ST_WM = 0, ST_W = 1, ST_G = 2, ST_VL = 3, ST_COUNT = 4 };
+constexpr int LUT_ALL = ST_OFF + ST_COUNT * ST_STRIDE;
+
+// geometry of the cells: the top octave holds T = support_radius_sq_limit
+struct StepGeom {
+    uint32_t base;  // (float bits >> 15) of the first cell
+    float low;      // first float of the first cell
+};
+__host__ __device__ inline StepGeom step_geom(float T) {
+    uint32_t tb;
+#ifdef __HIP_DEVICE_COMPILE__
+    tb = __float_as_uint(T);
+#else
+    std::memcpy(&tb, &T, 4);
+#endif
+    const uint32_t e0 = (tb >> 23) - (ST_OCT - 1);
+    StepGeom g;
+    g.base = e0 << 8;
+    const uint32_t lb = e0 << 23;
+#ifdef __HIP_DEVICE_COMPILE__
+    g.low = __uint_as_float(lb);
+#else
+    std::memcpy(&g.low, &lb, 4);
+#endif
+    return g;
+}
+
+// block-wide copy of step table k into LDS (no barrier)
+__device__ inline void load_step_table(float2* dst, const float* __restrict__ luts, int k) {
+    const float2* src = reinterpret_cast<const float2*>(luts + ST_OFF + k * ST_STRIDE);
+    for (int c = threadIdx.x; c <= ST_CELLS; c += blockDim.x) dst[c] = src[c];
+}
+
+// the fast part: valid for d2 >= low (else the caller takes the root path)
+__device__ __forceinline__ float step_fast(float d2, const float2* __restrict__ tab, uint32_t base) {
+    const uint32_t c = min((__float_as_uint(d2) >> 15) - base, (uint32_t)(ST_CELLS - 1));
+    const float2 a = tab[c], b = tab[c + 1];
+    return d2 >= a.x ? b.y : a.y;
+}
+
 // Counts the floats d2 in [0, limit) whose bin_of_d2 differs from
-// lut_bin(sqrt_rn(d2)).
-__global__ void k_check_bins(float factor, const float* __restrict__ luts, uint32_t limit_bits,
+// lut_bin(sqrt_rn(d2)), and those for which a step table differs from its
+// definition: ST_WM = m W[bin] below T else 0, ST_W = W[bin] below T else 0,
+// ST_G = G[bin], ST_VL = VL[bin].
+__global__ void k_check_bins(float factor, const float* __restrict__ luts, uint32_t limit_bits, float T, float m,
                              unsigned long long* bad) {
     __shared__ float2 th[LUT_N];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x)
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
     __syncthreads();
+    const StepGeom g = step_geom(T);
+    const float2* tabs = reinterpret_cast<const float2*>(luts + ST_OFF);
+    const float* W = luts;
+    const float* G = luts + LUT_N;
+    const float* VL = luts + 2 * LUT_N;
     unsigned long long n_bad = 0;
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < limit_bits; u += gridDim.x * blockDim.x) {
         const float d2 = __uint_as_float(u);
-        n_bad += bin_of_d2(d2, factor, th) != lut_bin(sqrtf(d2), factor) ? 1ull : 0ull;
+        const uint32_t b = lut_bin(sqrtf(d2), factor);
+        n_bad += bin_of_d2(d2, factor, th) != b ? 1ull : 0ull;
+        if (d2 < g.low) continue;
+        const float wb = b >= 300 ? 0.0f : W[b];
+        const float ref[ST_COUNT] = {d2 < T ? wb * m : 0.0f, d2 < T ? wb : 0.0f, b >= 300 ? 0.0f : G[b],
+                                     b >= 300 ? 0.0f : VL[b]};
+#pragma unroll
+        for (int k = 0; k < ST_COUNT; ++k) {
+            const float v = step_fast(d2, tabs + (size_t)k * (ST_STRIDE / 2), g.base);
+            n_bad += __float_as_uint(v) != __float_as_uint(ref[k]) ? 1ull : 0ull;
+        }
     }
     if (n_bad) atomicAdd(bad, n_bad);
 }
@@ -991,11 +1068,13 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
                                                       Ctl* ctl) {
     __shared__ float lvl[LUT_N], lpg[LUT_N];
     __shared__ float2 th[LUT_N];
+    __shared__ float2 svl[ST_CELLS + 1];
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
         lvl[t] = luts[2 * LUT_N + t];
         lpg[t] = luts[LUT_N + t];
         th[t] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[t];
     }
+    load_step_table(svl, luts, ST_VL);
     __syncthreads();
     const upf_params& P = A.P;
     for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
@@ -1025,12 +1104,15 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     auto pair = [&](uint32_t e, const float4& xj, const float4& vj) __attribute__((always_inline)) {
         const int tj = ptype(xj);
         if (tj == UPF_LIQUID) {
-            uint32_t bin = e & NB_BIN_MASK;
-            if (bin_free) {  // the list build's expression on the same positions
+            float kv;
+            if (bin_free) {  // the list build's distance on the same positions
                 const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                bin = bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th);
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                kv = step_fast(d2, svl, A.st_base);
+                if (d2 < A.st_low) kv = lvl[bin_of_d2(d2, A.lut_factor, th)];
+            } else {
+                kv = lvl[e & NB_BIN_MASK];
             }
-            const float kv = lvl[bin];
             const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
             fx = fx - vx * rv * rv * visc * kv;
             fy = fy - vy * rv * rv * visc * kv;
@@ -1126,10 +1208,13 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     __shared__ float2 th[LUT_N];
     __shared__ float wmax[BLOCK / 64];  // blockDim.x <= BLOCK
     __shared__ float4 urec[CODED == 2 ? UNION_CAP : 1];  // the chunk's union: predicted positions
+    // W (Akinci: times the neighbour's mass below) or m W, 0 from T on
+    __shared__ float2 sw[ST_CELLS + 1];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
         lw[k] = k < 300 ? luts[k] : 0.0f;  // bin 300 (dist >= h) weighs 0, as in lut_at
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
     }
+    load_step_table(sw, luts, MODE == UPF_BOUNDARY_AKINCI ? ST_W : ST_WM);
     __syncthreads();
     const upf_params& P = A.P;
     const int lane = threadIdx.x & 63;
@@ -1158,19 +1243,18 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 auto use = [&](int j, uint32_t, const float4& xj) __attribute__((always_inline)) {
                     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                     const float d2 = dx * dx + dy * dy + dz * dz;
+                    // without a branch: a pair at d2 >= T adds w = +0, and
+                    // rho + (+0) * m_j is rho exactly (rho > 0, m_j >= 0)
                     if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
-                        if (d2 < T) {
-                            const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
-                            float mj = m;
-                            if (ptype(xj) != UPF_LIQUID) mj = P.rest_density * S.psi[j];
-                            rho = rho + w * mj;
-                        }
+                        float w = step_fast(d2, sw, A.st_base);
+                        if (d2 < A.st_low) w = lw[bin_of_d2(d2, A.lut_factor, th)];
+                        float mj = m;
+                        if (ptype(xj) != UPF_LIQUID && d2 < T) mj = P.rest_density * S.psi[j];
+                        rho = rho + w * mj;
                     } else {
-                        // without a branch: a pair at d2 >= T adds w = +0, and
-                        // rho + (+0) * m is rho exactly (rho > 0)
-                        const float wt = lw[bin_of_d2(d2, A.lut_factor, th)];
-                        const float w = d2 < T ? wt : 0.0f;
-                        rho = rho + w * m;
+                        float wm = step_fast(d2, sw, A.st_base);  // the table holds W * m
+                        if (d2 < A.st_low) wm = lw[bin_of_d2(d2, A.lut_factor, th)] * m;
+                        rho = rho + wm;
                     }
                 };
                 auto from_global = [&](int j) { return at_u32(W.xs, j); };
@@ -1328,10 +1412,12 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     __shared__ float lpg[LUT_N];
     __shared__ float2 th[LUT_N];
     __shared__ float4 urec[CODED == 2 ? UNION_CAP : 1];  // the chunk's union: (position, p / rho0^2)
+    __shared__ float2 sg[CODED ? ST_CELLS + 1 : 1];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
         lpg[k] = luts[LUT_N + k];
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
     }
+    if (CODED) load_step_table(sg, luts, ST_G);
     __syncthreads();
     const upf_params& P = A.P;
     const int n_round = (A.n + gstride() - 1) / gstride();
@@ -1360,8 +1446,14 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     auto use = [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
         const float4& xj = q.x;
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const uint32_t bin = CODED ? bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th) : (e & NB_BIN_MASK);
-        const float g = lpg[bin];
+        float g;
+        if (CODED) {
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            g = step_fast(d2, sg, A.st_base);
+            if (d2 < A.st_low) g = lpg[bin_of_d2(d2, A.lut_factor, th)];
+        } else {
+            g = lpg[e & NB_BIN_MASK];
+        }
         // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
         // precomputed per particle by the density pass (same division)
         float pd = q.pd, mj = m;
@@ -2168,6 +2260,9 @@ struct upf_sim {
         A.slots_per_scene = sps;
         A.code_lists = tune.loop_list;
         A.xcd = tune.xcd_remap;
+        const StepGeom sg = step_geom(P.support_radius_sq_limit);
+        A.st_base = sg.base;
+        A.st_low = sg.low;
         return A;
     }
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -2184,24 +2279,6 @@ static inline dim3 grid_persist(const upf_sim* s, int n) {
     const int b = s->tune.loop_block;
     const int g = std::max(1, std::min((n + b - 1) / b, s->tune.grid_cap));
     return dim3((unsigned)((g + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
-}
-
-static int check_params(const upf_params* p) {
-    if (!p) return UPF_ERR_ARG;
-    if (p->max_neighbors > 4096) return UPF_ERR_ARG;
-    if (p->block_size < 2 || p->grid_dim == 0 || p->grid_dim > 512) return UPF_ERR_ARG;
-    // the list builds take candidates from the 3x3x3 cells around a particle's
-    // cell: the query box [x - h, x + h] must not reach a fourth cell, so a
-    // cell is wider than the scaled support radius by more than rounding
-    for (int a = 0; a < 3; ++a)
-        if (!((float)p->block_size > p->scales[a] * p->support_radius * 1.0001f)) return UPF_ERR_ARG;
-    if (p->vbox_min[0] != 0.0f || p->vbox_min[1] != 0.0f || p->vbox_min[2] != 0.0f) return UPF_ERR_ARG;
-    if (p->max_neighbors <= 0 || p->max_neighbors > 4096) return UPF_ERR_ARG;
-    if (p->add_wall_weight != 0 && p->add_wall_weight != 1) return UPF_ERR_ARG;
-    if (p->add_wall_weight)
-        for (int a = 0; a < 3; ++a)
-            if (!(p->container_min[a] < p->container_max[a])) return UPF_ERR_ARG;
-    return UPF_OK;
 }
 
 static int alloc_slots(upf_sim* s, Slots& S, int cap) {
@@ -2568,6 +2645,51 @@ static void build_bin_thresholds(float factor, float* out) {
     }
 }
 
+// The step table of V (see step_fast): cut -> the value is 0 for d2 >= T;
+// scale multiplies every value (as the pass would, in float).  thr2 = the
+// threshold pairs of build_bin_thresholds.  False if a cell would hold two
+// changes of value (the cells are too wide for the bins), or if the value
+// at the end of the last cell is not 0 (farther pairs clamp to it).
+static bool build_step_table(const float* V, bool cut, float T, float scale, const float* thr2, float* out) {
+    const StepGeom g = step_geom(T);
+    auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
+    auto fl = [](uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; };
+    // the bin of a float: the largest k with thr[k] <= d2 (thr2[2k] = thr[k])
+    auto value = [&](uint32_t u) {
+        const float d2 = fl(u);
+        int b = 0;
+        while (b < 300 && !(d2 < thr2[2 * (b + 1)])) ++b;
+        float v = b >= 300 ? 0.0f : V[b];
+        if (cut && !(d2 < T)) v = 0.0f;
+        return v * scale;
+    };
+    // the floats at which the value may change: the thresholds and T
+    std::vector<uint32_t> ev;
+    for (int k = 1; k <= 300; ++k) ev.push_back(bits(thr2[2 * k]));
+    if (cut) ev.push_back(bits(T));
+    std::sort(ev.begin(), ev.end());
+    for (int c = 0; c <= ST_CELLS; ++c) {
+        const uint32_t lo = (g.base + (uint32_t)c) << 15, hi = lo + (1u << 15);
+        const float v0 = value(lo);
+        float t = std::numeric_limits<float>::infinity();
+        int changes = 0;
+        float cur = v0;
+        for (uint32_t e : ev) {
+            if (e <= lo || e >= hi) continue;
+            const float v = value(e);
+            if (bits(v) != bits(cur)) {
+                if (++changes == 1) t = fl(e);
+                cur = v;
+            }
+        }
+        if (changes > 1) return false;
+        out[2 * c] = c == ST_CELLS ? std::numeric_limits<float>::infinity() : t;
+        out[2 * c + 1] = v0;
+    }
+    // pairs beyond the last cell read it: its value at its end must be theirs (0)
+    return bits(value(((g.base + ST_CELLS) << 15) - 1)) == 0u && bits(out[2 * ST_CELLS + 1]) == 0u;
+}
+
 static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     const upf_params* p = &s->P;
     s->cap = capacity;
@@ -2600,7 +2722,7 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.uslots, nchunks * UNION_CAP));
     HIPCHK(s->alloc(&W.un, nchunks));
     HIPCHK(s->alloc(&W.cells, ncells));
-    HIPCHK(s->alloc(&s->luts, LUT_TOTAL));
+    HIPCHK(s->alloc(&s->luts, LUT_ALL));
     HIPCHK(s->alloc(&s->ctl, 1));
     HIPCHK(s->alloc(&s->sctl, n_scenes));
     void* tmp = nullptr;
@@ -2612,21 +2734,39 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(hipHostMalloc((void**)&s->mirror, sizeof(CtlMirror), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(s->mirror, 0, sizeof(CtlMirror));
     HIPCHK(hipHostGetDevicePointer((void**)&s->mirror_dev, s->mirror, 0));
-    std::vector<float> L(LUT_TOTAL, 0.0f);
+    std::vector<float> L(LUT_ALL, 0.0f);
     build_luts(p->support_radius, L.data());
     const float factor = (float)(1.0 / (double)p->support_radius);  // StepArgs::lut_factor
     build_bin_thresholds(factor, L.data() + LUT_TH_OFF);
+    {
+        const float T = p->support_radius_sq_limit, m = p->initial_mass;
+        const float* th2 = L.data() + LUT_TH_OFF;
+        float* st = L.data() + ST_OFF;
+        const bool ok = build_step_table(L.data(), true, T, m, th2, st + ST_WM * ST_STRIDE) &&
+                        build_step_table(L.data(), true, T, 1.0f, th2, st + ST_W * ST_STRIDE) &&
+                        build_step_table(L.data() + LUT_N, false, T, 1.0f, th2, st + ST_G * ST_STRIDE) &&
+                        build_step_table(L.data() + 2 * LUT_N, false, T, 1.0f, th2, st + ST_VL * ST_STRIDE);
+        if (!ok) {
+            std::fprintf(stderr, "upf: kernel step tables cannot be built for h = %g\n", (double)p->support_radius);
+            return UPF_ERR_ARG;
+        }
+    }
     HIPCHK(hipMemcpy(s->luts, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice));
     {
         // the sqrt-free bins equal the correctly rounded ones for every float
-        // below 2 thr[300] (above it both are 300)
+        // below 2 thr[300] (above it both are 300), and so do the step
+        // tables' values (up to four times the last cell's end: every
+        // farther float clamps to the same cell)
         unsigned long long* bad = nullptr;
         HIPCHK(s->alloc(&bad, 1));
         HIPCHK(hipMemset(bad, 0, sizeof(unsigned long long)));
-        const float top = 2.0f * L[LUT_TH_OFF + 2 * 300];
+        float top = 2.0f * L[LUT_TH_OFF + 2 * 300];
+        const StepGeom g = step_geom(p->support_radius_sq_limit);
         uint32_t top_bits;
         std::memcpy(&top_bits, &top, 4);
-        hipLaunchKernelGGL(k_check_bins, dim3(4096), dim3(256), 0, s->st, factor, s->luts, top_bits, bad);
+        top_bits = std::max(top_bits, (g.base + ST_CELLS + 512u) << 15);
+        hipLaunchKernelGGL(k_check_bins, dim3(4096), dim3(256), 0, s->st, factor, s->luts, top_bits,
+                           p->support_radius_sq_limit, p->initial_mass, bad);
         unsigned long long h_bad = 1;
         HIPCHK(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, s->st));
         HIPCHK(hipStreamSynchronize(s->st));
@@ -2660,7 +2800,7 @@ static int stats_alloc(upf_sim* s, int cap_steps) {
 int upf_create(const upf_params* p, int capacity, upf_sim** out) {
     // list entries hold the neighbour slot in 23 bits
     if (!out || capacity <= 0 || capacity >= (1 << (32 - NB_SLOT_SHIFT))) return UPF_ERR_ARG;
-    int rc = check_params(p);
+    int rc = upf::check_params(p);
     if (rc) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UPF_ERR_NO_DEVICE;
@@ -2691,10 +2831,10 @@ int upf_destroy(upf_sim* s) {
 
 int upf_set_params(upf_sim* s, const upf_params* p) {
     if (!s || !p) return UPF_ERR_ARG;
-    int rc = check_params(p);
+    int rc = upf::check_params(p);
     if (rc) return rc;
-    if (p->grid_dim != s->P.grid_dim || p->block_size != s->P.block_size || p->max_neighbors != s->P.max_neighbors ||
-        p->support_radius != s->P.support_radius)
+    if (p->grid_dim != s->P.grid_dim || p->block_size != s->P.block_size || p->block_shift != s->P.block_shift ||
+        p->max_neighbors != s->P.max_neighbors || p->support_radius != s->P.support_radius)
         return UPF_ERR_ARG;
     for (int a = 0; a < 3; ++a)
         if (p->scales[a] != s->P.scales[a]) return UPF_ERR_ARG;
@@ -2795,6 +2935,14 @@ int upf_step(upf_sim* s, int nsteps, int method) {
         if (rc) return rc;
         const int e = err_code(s->h_ctl->err);
         if (e) return e;
+    }
+    if (nsteps > 0 && (method == UPF_PCISPH_STATIC || method == UPF_PCISPH_RIGID)) {
+        // the loop's read-back comes before integration: errors that the last
+        // step's integration raised (a position leaving the z-index grid)
+        // are read once more here, so that this call reports them
+        const int rc = read_ctl(s);
+        if (rc) return rc;
+        return err_code(s->h_ctl->err);
     }
     HIPCHK(hipStreamSynchronize(s->st));
     s->prof.collect();
@@ -2955,21 +3103,9 @@ int upf_upload_bodies(upf_sim* s, int n_bodies, const upf_body* bodies, const in
     if (!s || n_bodies <= 0 || n_bodies >= 0xFFFF || !bodies || !body_id || !rel4) return UPF_ERR_ARG;
     if (!s->uploaded) return UPF_ERR_STATE;
     const int n = s->n;
-    if (s->d_bodies) {
-        (void)hipFree(s->d_bodies);
-        (void)hipFree(s->d_body_start);
-        s->d_bodies = nullptr;
-    }
-    HIPCHK(hipMalloc((void**)&s->d_bodies, (size_t)n_bodies * sizeof(upf_body)));
-    HIPCHK(hipMalloc((void**)&s->d_body_start, (size_t)(n_bodies + 1) * sizeof(int)));
-    if (!s->d_body_keys) {
-        HIPCHK(s->alloc(&s->d_body_keys, s->cap));
-        HIPCHK(s->alloc(&s->d_body_keys2, s->cap));
-        HIPCHK(s->alloc(&s->d_body_members, s->cap));
-    }
-    HIPCHK(hipMemcpy(s->d_bodies, bodies, (size_t)n_bodies * sizeof(upf_body), hipMemcpyHostToDevice));
-    // per-particle body data in the current slot order: slots are still in
-    // upload order unless a step ran, so scatter through the ids
+    // per-particle body data in the current slot order (slots are still in
+    // upload order unless a step ran, so scatter through the ids), checked on
+    // the host before anything on the device changes
     std::vector<int> id(n), bid(n);
     std::vector<float> rel((size_t)4 * n), psi(n);
     Slots& S = s->S[s->cur];
@@ -2978,14 +3114,38 @@ int upf_upload_bodies(upf_sim* s, int n_bodies, const upf_body* bodies, const in
     for (int k = 0; k < n; ++k) {
         const int u = id[k];
         bid[k] = body_id[u];
-        if (bid[k] >= n_bodies) return UPF_ERR_ARG;
+        if (bid[k] < -1 || bid[k] >= n_bodies) return UPF_ERR_ARG;
         for (int a = 0; a < 4; ++a) rel[4 * k + a] = rel4[4 * u + a];
         // a rigid particle's weighted_volume_ = its mass / rest density (UnifiedParticle.cpp:24)
         if (bid[k] >= 0) psi[k] = rel4[4 * u + 3] / s->P.rest_density;
     }
-    HIPCHK(hipMemcpy(S.body, bid.data(), (size_t)n * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(S.rel, rel.data(), (size_t)n * 16, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(S.psi, psi.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    // the previous bodies are dropped first: after a failure below the
+    // simulation has none (a rigid step returns UPF_ERR_STATE), never a body
+    // count that disagrees with the device arrays
+    s->rigid = false;
+    s->n_bodies = 0;
+    if (s->d_bodies) (void)hipFree(s->d_bodies);
+    if (s->d_body_start) (void)hipFree(s->d_body_start);
+    s->d_bodies = nullptr;
+    s->d_body_start = nullptr;
+    upf_body* d_b = nullptr;
+    int* d_start = nullptr;
+    bool ok = hipMalloc((void**)&d_b, (size_t)n_bodies * sizeof(upf_body)) == hipSuccess &&
+              hipMalloc((void**)&d_start, (size_t)(n_bodies + 1) * sizeof(int)) == hipSuccess;
+    if (ok && !s->d_body_keys)
+        ok = s->alloc(&s->d_body_keys, s->cap) == hipSuccess && s->alloc(&s->d_body_keys2, s->cap) == hipSuccess &&
+             s->alloc(&s->d_body_members, s->cap) == hipSuccess;
+    ok = ok && hipMemcpy(d_b, bodies, (size_t)n_bodies * sizeof(upf_body), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(S.body, bid.data(), (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(S.rel, rel.data(), (size_t)n * 16, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(S.psi, psi.data(), (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (!ok) {
+        if (d_b) (void)hipFree(d_b);
+        if (d_start) (void)hipFree(d_start);
+        return UPF_ERR_HIP;
+    }
+    s->d_bodies = d_b;
+    s->d_body_start = d_start;
     s->n_bodies = n_bodies;
     s->rigid = true;
     return UPF_OK;
@@ -3131,6 +3291,7 @@ struct upf_batch {
     std::vector<int> scene_of;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};  // one step, by the slot buffer it starts from
     upf_tuning captured{};                          // launch shapes baked into gexec
+    upf_params captured_p{};                        // physics baked into gexec (StepArgs, loop bounds)
 };
 
 // Captures one static-boundary PCISPH step starting from slot buffer `cur`
@@ -3181,7 +3342,7 @@ int upf_batch_create(int n_scenes, const upf_params* scene_params, const int* co
     uint64_t ncells = 0;
     for (int k = 0; k < n_scenes; ++k) {
         if (counts[k] <= 0) return UPF_ERR_ARG;
-        int rc = check_params(&scene_params[k]);
+        int rc = upf::check_params(&scene_params[k]);
         if (!rc && scene_params[k].add_wall_weight) rc = UPF_ERR_ARG;  // single-scene steps only
         if (rc) return rc;
         if (!same_physics(scene_params[0], scene_params[k])) return UPF_ERR_ARG;
@@ -3264,14 +3425,19 @@ int upf_batch_step(upf_batch* b, int nsteps, int use_graph) {
         const int e = err_code(s->h_ctl->err);
         if (e) return e;
     }
-    if (b->gexec[0] && std::memcmp(&b->captured, &s->tune, sizeof(upf_tuning)) != 0) {
-        for (auto& g : b->gexec) {  // the tuning changed since the capture
+    // the graphs bake in the launch shapes and the physics (every kernel's
+    // StepArgs, the number of loop iterations): recapture when upf_set_tuning
+    // or upf_set_params (through upf_batch_sim) changed either
+    if (b->gexec[0] && (std::memcmp(&b->captured, &s->tune, sizeof(upf_tuning)) != 0 ||
+                        std::memcmp(&b->captured_p, &s->P, sizeof(upf_params)) != 0)) {
+        for (auto& g : b->gexec) {
             (void)hipGraphExecDestroy(g);
             g = nullptr;
         }
     }
     if (k < nsteps && !b->gexec[0]) {
         b->captured = s->tune;
+        b->captured_p = s->P;
         const int cur = s->cur;
         int rc = capture_step(s, 0, &b->gexec[0]);
         if (!rc) rc = capture_step(s, 1, &b->gexec[1]);
