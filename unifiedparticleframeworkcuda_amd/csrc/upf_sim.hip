@@ -672,16 +672,31 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
 // which keeps the LDS free for more waves).
-template <int ROUNDS, int STAGE, int BIN>
+//
+// CODE: the lists leave delta-coded (the layout of ListEncoder: first slot
+// in nbf, then 16-bit increments, eight per 16-byte word of nbw, column-
+// major; an increment >= 2^16 is 0 plus an entry of nbe).  Each passing
+// lane takes the slot of the next lower passing lane of its ballot (or the
+// particle's last entry) as its predecessor.  The first CODE_STAGE
+// increments of every list are staged in LDS (one column per slot, 16-bit)
+// and leave as whole 16-byte words of 64 consecutive columns, the rest (rare)
+// go to nbw directly.
+constexpr int CODE_STAGE = 64;
+constexpr int CODE_COL = CODE_STAGE + 2;  // 16-bit entries per LDS column (odd dword stride)
+
+template <int ROUNDS, int STAGE, int BIN, int CODE = 0>
 __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
-                                                  uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
+                                                  uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl,
+                                                  Work W) {
+    static_assert(!(CODE && (STAGE || BIN)), "the delta-coded output stages its own lists and carries no bins");
     constexpr int NB_CHUNK = 64 * ROUNDS;
     // packed cell coordinates: 10-bit fields whose top bit (coordinates are
     // below 512) serves as the borrow guard of a packed compare
     constexpr uint32_t GUARD = (1u << 9) | (1u << 19) | (1u << 29);
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
     __shared__ uint32_t stage[(STAGE > 0 ? STAGE : 1) * 65];
+    __shared__ uint16_t cst[CODE ? 64 * CODE_COL : 2];
     const upf_params& P = A.P;
     const float lut_factor = A.lut_factor;
     const int n = A.n;
@@ -722,6 +737,9 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
     }
     const uint64_t liquid = __ballot(my_liquid);
     int my_cnt = 0;  // this lane's slot: list length so far
+    uint32_t my_prev = 0u;  // CODE: its last entry's slot
+    int my_esc = 0;         // CODE: its escapes so far
+    const size_t st = (size_t)stride;
     int s = base;
     while (s < wend) {
         const int sl = s - base;
@@ -810,6 +828,8 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                 const uint32_t blo = (uint32_t)__builtin_amdgcn_readlane((int)my_blo, pl);
                 const uint32_t bhi_g = (uint32_t)__builtin_amdgcn_readlane((int)my_bhi, pl);
                 int cnt = __builtin_amdgcn_readlane(my_cnt, pl);
+                uint32_t prev = CODE ? (uint32_t)__builtin_amdgcn_readlane((int)my_prev, pl) : 0u;
+                int esc = CODE ? __builtin_amdgcn_readlane(my_esc, pl) : 0;
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
                     if (r >= rounds) break;
@@ -821,6 +841,35 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     const bool pass = inbox & (d2 < T) & (cj[r] != p);
                     const uint64_t m = __ballot(pass);
                     if (m == 0ull) continue;
+                    if constexpr (CODE) {
+                        // predecessor: the next lower passing lane's candidate
+                        const uint64_t lower = m & lt;
+                        const int src = lower ? 63 - __builtin_clzll(lower) : lane;
+                        const uint32_t pj = (uint32_t)__shfl(cj[r], src, 64);
+                        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        const int k = cnt + (int)below;
+                        const uint32_t d = (uint32_t)cj[r] - (lower ? pj : prev);
+                        const bool big = pass && k > 0 && d > 65535u;
+                        const uint64_t em = __ballot(big);
+                        if (pass) {
+                            if (k == 0) W.nbf[p] = (uint32_t)cj[r];
+                            if (big) {
+                                const int e = esc + (int)__popcll(em & lt);
+                                if (e < NB_ESC) W.nbe[(size_t)e * st + p] = d;
+                                else set_err(ctl, UPF_ERR_NBR_OVERFLOW);
+                            }
+                            const uint16_t wv = (k == 0 || big) ? (uint16_t)0 : (uint16_t)d;
+                            if (k < CODE_STAGE)
+                                cst[pl * CODE_COL + k] = wv;
+                            else if (k < cap)
+                                reinterpret_cast<uint16_t*>(W.nbw + (size_t)(k >> 3) * st + p)[k & 7] = wv;
+                        }
+                        esc += (int)__popcll(em);
+                        prev = (uint32_t)__builtin_amdgcn_readlane(cj[r], 63 - __builtin_clzll(m));
+                        cnt += __popcll(m);
+                        continue;
+                    }
                     if (pass) {
                         const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -839,11 +888,25 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     }
                     cnt += __popcll(m);
                 }
-                if (lane == pl) my_cnt = cnt;
+                if (lane == pl) {
+                    my_cnt = cnt;
+                    if (CODE) { my_prev = prev; my_esc = esc; }
+                }
             }
         }
         __syncthreads();  // r_* are rewritten for the next cell
         s = pe;
+    }
+    if constexpr (CODE) {
+        // the staged increments leave as whole rows: row q of the wave's 64
+        // columns is 1 KB of consecutive memory
+        __syncthreads();
+        const int mine = mine_valid ? min(min(my_cnt, cap), CODE_STAGE) : 0;
+        int rows = (mine + 7) >> 3;
+        for (int o = 32; o > 0; o >>= 1) rows = max(rows, __shfl_xor(rows, o, 64));
+        const uint32_t* col = reinterpret_cast<const uint32_t*>(cst + lane * CODE_COL);
+        for (int q = 0; q < rows; ++q)
+            if (8 * q < mine) W.nbw[(size_t)q * st + me] = make_uint4(col[4 * q], col[4 * q + 1], col[4 * q + 2], col[4 * q + 3]);
     }
     if (mine_valid) {
         ncount[me] = my_cnt < cap ? my_cnt : cap;
@@ -1063,7 +1126,13 @@ __device__ __forceinline__ float pressure_record_w(int type, float psi, float rh
     return 0.0f;
 }
 
-template <int MODE>
+struct XV {
+    float4 x, v;
+};
+
+// SRC 0: the packed lists of the step (re-written delta-coded for the loop
+// when A.code_lists); SRC 1: the delta-coded lists the build wrote.
+template <int MODE, int SRC>
 __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                       Ctl* ctl) {
     __shared__ float lvl[LUT_N], lpg[LUT_N];
@@ -1099,9 +1168,9 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
     const uint32_t* L = W.nbr + i;
-    const bool bin_free = A.code_lists != 0;  // the lists of this step carry no bins
+    const bool bin_free = SRC == 1 || A.code_lists != 0;  // the lists of this step carry no bins
     const size_t st = (size_t)A.stride;
-    auto pair = [&](uint32_t e, const float4& xj, const float4& vj) __attribute__((always_inline)) {
+    auto pair = [&](int j, uint32_t e, const float4& xj, const float4& vj) __attribute__((always_inline)) {
         const int tj = ptype(xj);
         if (tj == UPF_LIQUID) {
             float kv;
@@ -1131,7 +1200,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
                 uint32_t dl = (uint32_t)(dist * 300.0f / h);
                 dl = dl > 300u ? 300u : dl;
                 g = lpg[dl];
-                wv = S.psi[(int)(e >> NB_SLOT_SHIFT)];
+                wv = S.psi[j];
             }
             const float s1 = (((dx * g) * pg) * P.initial_mass) * P.rest_density;
             const float s2 = (((dy * g) * pg) * P.initial_mass) * P.rest_density;
@@ -1141,6 +1210,10 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
             fz = fz + (s3 * wv) * -1.0f;
         }
     };
+    if constexpr (SRC == 1) {
+        for_each_listed<1, 4>(W, i, st, cnt, [&](int j) { return XV{S.pos[j], S.vel[j]}; },
+                              [&](int j, uint32_t, const XV& q) { pair(j, 0u, q.x, q.v); });
+    } else {
     ListEncoder enc(W, i, st);
     const bool code = A.code_lists != 0;
     int k = 0;
@@ -1157,15 +1230,16 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
             vj[u] = S.vel[j];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pair(e[u], xj[u], vj[u]);
+        for (int u = 0; u < 4; ++u) pair((int)(e[u] >> NB_SLOT_SHIFT), e[u], xj[u], vj[u]);
     }
     for (; k < cnt; ++k) {
         const uint32_t e = L[k * st];
         const int j = (int)(e >> NB_SLOT_SHIFT);
         if (code) enc.put(k, (uint32_t)j, ctl);
-        pair(e, S.pos[j], S.vel[j]);
+        pair(j, e, S.pos[j], S.vel[j]);
     }
     if (code) enc.finish();
+    }
     fy = fy - P.initial_mass * P.gravity;
     const Dom d = dom_at(A, i);
     if (P.add_boundary_force) box_force_static(P, d, xi, fx, fy, fz);
@@ -2215,6 +2289,7 @@ struct upf_sim {
     int steps_done = 0;
     upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4, 1};
     bool static_lists = false;  // the lists being built serve the static PCISPH step only
+    bool lists_coded = false;   // this step's build wrote the delta-coded lists (no packed lists)
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
     uint32_t ncells = 0;
@@ -2443,17 +2518,27 @@ static void build_lists(upf_sim* s, const StepArgs& A) {
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_NBR_BUILD, s->st);
     const dim3 g((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
-    auto launch = [&](auto kern) {
+    auto launch_tile = [&](auto kern) {
         hipLaunchKernelGGL(kern, g, dim3(64), 0, s->st, A, S.pos, s->W.keys_sorted, s->W.cells, s->W.nbr, s->W.ncount,
                            s->ctl);
+    };
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, g, dim3(64), 0, s->st, A, S.pos, s->W.keys_sorted, s->W.cells, s->W.nbr, s->W.ncount,
+                           s->ctl, s->W);
     };
     const int r = s->tune.nbr_rounds;
     // the correction loop of the delta-coded lists never reads the bins
     const bool bins = !(A.code_lists && s->static_lists);
-    if (s->tune.nbr_tile == 256) {
-        launch(k_nbr_build_tile<256>);
+    // static PCISPH with delta-coded loop lists: the build writes them directly
+    s->lists_coded = s->static_lists && A.code_lists == 1 && s->tune.nbr_tile == 0 && s->tune.nbr_stage == 0;
+    if (s->lists_coded) {
+        if (r == 2) launch(k_nbr_build<2, 0, 0, 1>);
+        else if (r == 8) launch(k_nbr_build<8, 0, 0, 1>);
+        else launch(k_nbr_build<4, 0, 0, 1>);
+    } else if (s->tune.nbr_tile == 256) {
+        launch_tile(k_nbr_build_tile<256>);
     } else if (s->tune.nbr_tile == 512) {
-        launch(k_nbr_build_tile<512>);
+        launch_tile(k_nbr_build_tile<512>);
     } else if (s->tune.nbr_stage == 0) {
         if (!bins) {
             if (r == 2) launch(k_nbr_build<2, 0, 0>);
@@ -2498,10 +2583,16 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
     build_lists(s, A);
     s->static_lists = false;
     s->prof.begin(PC_NBR_EXT, s->st);
-    if (akinci)
-        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
-    else
-        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
+    if (s->lists_coded) {
+        if (akinci)
+            hipLaunchKernelGGL((k_ext_static<UPF_BOUNDARY_AKINCI, 1>), gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
+        else
+            hipLaunchKernelGGL((k_ext_static<UPF_BOUNDARY_CPU_REF, 1>), gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
+    } else if (akinci) {
+        hipLaunchKernelGGL((k_ext_static<UPF_BOUNDARY_AKINCI, 0>), gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
+    } else {
+        hipLaunchKernelGGL((k_ext_static<UPF_BOUNDARY_CPU_REF, 0>), gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
+    }
     s->prof.end(s->st);
     s->union_ready = s->tune.loop_list == 2;
     if (s->union_ready) {
