@@ -307,6 +307,18 @@ int upf_download_neighbors(upf_sim* s, int* counts, int32_t* flat, int cap);
 int upf_keys(const upf_params* p, int n, const float* pos4, uint32_t* keys);
 int upf_radix_sort(int n, const uint32_t* keys_in, uint32_t* keys_out, int32_t* perm_out);
 
+/* No counterpart in the reference: upf_radix_sort for callers that cannot
+ * allocate device memory (code under stream capture, serving loops,
+ * harnesses that forbid hipMalloc).  keys_in, keys_out and perm_out are
+ * device pointers to n elements each (no aliasing); stream is a hipStream_t
+ * (NULL: the default stream); the call enqueues the sort and returns without
+ * synchronising.  The sort's working memory is a fixed array of the library
+ * (about 9 MB on every device it runs on, held until the process ends),
+ * sized for n <= UPF_NOALLOC_SORT_MAX: larger n gives UPF_ERR_ARG.  The call
+ * never allocates; calls that overlap on one device share the array and
+ * race, so a caller serialises them (one stream, or events).               */
+#define UPF_NOALLOC_SORT_MAX (1 << 20)
+int upf_radix_sort_noalloc(int n, const uint32_t* keys_in, uint32_t* keys_out, int32_t* perm_out, void* stream);
 
 /* ------------------------------------------------------ batched form */
 /* Many independent scenes stepped together (no reference counterpart; the
