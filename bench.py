@@ -2,10 +2,13 @@
 
 Workload (the metric's configuration): 128 x 64 x 128 = 1,048,576 fluid
 particles (spacing 0.01 m, jitter +-0.001 spacing, seed 0) in the corner of a
-2.56 x 0.64 x 1.28 m box with one layer of wall particles (~0.18M), PCISPH
-with min 3 / max 50 iterations and a 1 % density-error threshold, Morton sort
-every step, dt = 1 ms, FP32.  One particle-update = one fluid particle
-advanced one full step including all K iterations.
+2.56 x 0.64 x 1.28 m box with one layer of wall particles at the spacing
+(~84k), PCISPH with min 3 / max 50 iterations and a 1 % density-error
+threshold, Morton sort every step, dt = 1 ms, FP32.  One particle-update =
+one fluid particle advanced one full step including all K iterations.  A
+second leg (`half_spacing_walls`) runs the same fluid in the wall layout the
+configuration names -- walls sampled at half the spacing (~335k boundary
+particles) -- with Akinci boundary volumes.
 
 Prints ONE JSON line.  `roofline` is for the kernel class with the largest
 device time (HIP events recorded by the library on its own stream around every
@@ -78,17 +81,58 @@ def measured_traffic(cls):
     return None
 
 
-def run_gpu(args):
+# The dam-break workloads: wall sampling (relative to the spacing) and the
+# boundary-pressure mode.
+WORKLOADS = {
+    # one wall layer at the full spacing (~84k) with the CPU path's boundary
+    # pressure (whose Akinci term is zero): the walls act through the density
+    "pcisph_dam_break_1m": dict(wall_rel=1.0, mode="cpu_ref"),
+    # the configuration's wall layout: walls sampled at half spacing (~335k)
+    # with Akinci boundary volumes (the reference GPU path's boundary pressure;
+    # with the CPU path's fluid mass such walls would weigh four layers)
+    "pcisph_dam_break_1m_half_spacing_walls": dict(wall_rel=0.5, mode="akinci"),
+}
+
+
+def build_scene(name, lattice):
+    """(params, pos4, type, n_fluid) of a dam-break workload."""
     import unifiedparticleframeworkcuda_amd as upf
+    nx, ny, nz = lattice
+    w = WORKLOADS[name]
+    p = upf.make_params(0.01, 0.001)
+    pos, typ, nf = upf.dam_break(p, nx, ny, nz, (0.02 * nx, 0.01 * ny, 0.01 * nz), wall_rel=w["wall_rel"])
+    upf.density_error_factor(p)
+    p.boundary_pressure_mode = upf.UPF_BOUNDARY_AKINCI if w["mode"] == "akinci" else upf.UPF_BOUNDARY_CPU_REF
+    return p, pos, typ, nf
+
+
+def loop_summary(K, rmax, max_loops=50):
+    """Iteration counts and density errors of a run's steps: the loop stops
+    once the largest predicted density is within 1 % of rho0 (after at least
+    3 iterations) or after max_loops; steps that reach max_loops end above."""
+    K = np.asarray(K)
+    err = np.maximum(0.0, 0.1 * np.asarray(rmax, np.float64) - 100.0)
+    conv = K < max_loops
+    return {"iterations": {"mean": float(np.mean(K)), "max": int(np.max(K)),
+                           "hist": {str(int(k)): int(c) for k, c in zip(*np.unique(K, return_counts=True))}},
+            "steps_at_max_loops": int(np.sum(~conv)),
+            "max_density_error_pct": float(err.max()),
+            "max_density_error_pct_converged_steps": float(err[conv].max()) if conv.any() else None}
+
+
+def run_gpu(args, workload=None, steps=None, profile=True):
+    import unifiedparticleframeworkcuda_amd as upf
+    workload = workload or args.workload
+    steps = steps or args.steps
     nx, ny, nz = args.lattice
-    p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz)
+    p, pos, typ, nf = build_scene(workload, args.lattice)
     n = len(typ)
     sim = upf.Simulation(p, n)
     tuning = None
     if not args.no_tune:
         from unifiedparticleframeworkcuda_amd import autotune
         trials = []
-        tuning, tune_ms = autotune.tune_simulation(sim, pos, typ, "pcisph_dam_break_%dx%dx%d" % (nx, ny, nz),
+        tuning, tune_ms = autotune.tune_simulation(sim, pos, typ, "%s_%dx%dx%d" % (workload, nx, ny, nz),
                                                    use_cache=not args.retune, log=trials)
         if args.show_trials:
             for t, sec in trials:
@@ -105,14 +149,17 @@ def run_gpu(args):
     sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
-    sim.step(args.steps, upf.UPF_PCISPH_STATIC)
+    sim.step(steps, upf.UPF_PCISPH_STATIC)
     hip.hipDeviceSynchronize()
     dt = time.perf_counter() - t0
     K, rmax = sim.stats()
-    Kt = K[-args.steps:]
-    rt = rmax[-args.steps:]
+    Kt = K[-steps:]
+    rt = rmax[-steps:]
     keys, ids, cells = sim.sorted_state()
     ncells = len(cells)
+    if not profile:
+        sim.close()
+        return dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, tuning=tuning)
     # the same steps again (same inputs, same iteration counts) with HIP events
     # recorded around every launch: per-kernel-class device time.  Here the
     # host enqueues no iteration ahead of the loop decision, so that every
@@ -123,29 +170,29 @@ def run_gpu(args):
     sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
     L.upf_profile_enable(sim.h, 1)
     L.upf_profile_reset(sim.h)
-    sim.step(args.steps, upf.UPF_PCISPH_STATIC)
+    sim.step(steps, upf.UPF_PCISPH_STATIC)
     hip.hipDeviceSynchronize()
     Kp, _ = sim.stats()
     prof = upf.profile(sim.h)
     L.upf_profile_enable(sim.h, 0)
-    assert np.array_equal(Kp[-args.steps:], Kt), "the profiled replay took other iteration counts"
+    assert np.array_equal(Kp[-steps:], Kt), "the profiled replay took other iteration counts"
     sim.set_tuning(timed_tuning)
     sim.close()
     out = dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof, tuning=tuning)
     if not args.no_graph:
         # the same scene as a batch of one: every step one hipGraph replay
-        b = upf.Batch([p], [n], max_record_steps=args.warmup + args.steps + 8)
+        b = upf.Batch([p], [n], max_record_steps=args.warmup + steps + 8)
         if tuning:
             b.set_tuning(tuning)
         b.upload(pos, typ)
         b.step(args.warmup, use_graph=True)
         hip.hipDeviceSynchronize()
         t0 = time.perf_counter()
-        b.step(args.steps, use_graph=True)
+        b.step(steps, use_graph=True)
         hip.hipDeviceSynchronize()
         out["graph_seconds"] = time.perf_counter() - t0
         Kg, _ = b.stats()
-        out["graph_K"] = Kg[-args.steps:, 0]
+        out["graph_K"] = Kg[-steps:, 0]
         b.close()
     return out
 
@@ -443,8 +490,7 @@ def cpu_baseline(args, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     import unifiedparticleframeworkcuda_amd as upf
-    nx, ny, nz = args.lattice
-    p, pos, typ, nf = upf.standard_dam_break(nx, ny, nz)
+    p, pos, typ, nf = build_scene(args.workload, args.lattice)
     o = ol.OracleSim(p, pos, None, typ, omp=True)
     steps = 0
     t0 = time.perf_counter()
@@ -456,7 +502,7 @@ def cpu_baseline(args, budget_s):
             break
     threads = ol.oracle(True).orc_num_threads()
     out = dict(value=nf * steps / el, unit="particle-updates/s", cores=int(threads), kind="port",
-               sample="first %d step(s) of the same 1M dam break from its initial state, %.1f s" % (steps, el))
+               sample="first %d step(s) of the same 1M dam break (%s) from its initial state, %.1f s" % (steps, args.workload, el))
     try:
         with open("/proc/cpuinfo") as f:
             out["cpu_model"] = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
@@ -479,6 +525,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--lattice", type=int, nargs=3, default=[128, 64, 128])
+    ap.add_argument("--workload", default="pcisph_dam_break_1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-second-layout", action="store_true",
+                    help="skip the leg with the other wall layout")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--cpu-max-steps", type=int, default=20)
     ap.add_argument("--cpu-single-steps", type=int, default=1, help="steps of the single-thread CPU sample (0: none)")
@@ -540,12 +589,11 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "pcisph_dam_break_1m", "fluid": nf, "boundary": n - nf,
+        "config": {"workload": args.workload, "fluid": nf, "boundary": n - nf,
                    "lattice": list(args.lattice), "spacing_m": 0.01, "dt_s": 0.001,
+                   "wall_sampling_rel": WORKLOADS[args.workload]["wall_rel"],
+                   "boundary_mode": WORKLOADS[args.workload]["mode"],
                    "min_max_iterations": [3, 50], "density_error_threshold_pct": 1.0, "path": "plain"},
-        "iterations": {"mean": float(np.mean(g["K"])), "max": int(np.max(g["K"])),
-                       "hist": {str(int(k)): int(c) for k, c in zip(*np.unique(g["K"], return_counts=True))}},
-        "max_density_error_pct": float(max(0.0, 0.1 * float(np.max(g["rmax"])) - 100.0)),
         "step_roofline": {"achieved": total_bytes / sec / 1e9, "unit": "GB/s", "frac": total_bytes / sec / HBM_PEAK,
                           "bytes_per_update": total_bytes / (nf * args.steps)},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in g["prof"].items()},
@@ -556,6 +604,14 @@ def main():
                           "same_iterations": bool(np.array_equal(g["graph_K"], g["K"]))}),
         "roofline": roof,
     }
+    out.update(loop_summary(g["K"], g["rmax"]))
+    if not args.no_second_layout:
+        other = [w for w in sorted(WORKLOADS) if w != args.workload][0]
+        g2 = run_gpu(args, workload=other, steps=min(args.steps, 20), profile=False)
+        out["half_spacing_walls" if other.endswith("half_spacing_walls") else "walls_at_spacing"] = dict(
+            workload=other, fluid=g2["nf"], boundary=g2["n"] - g2["nf"], steps=len(g2["K"]),
+            ms_per_step=g2["seconds"] / len(g2["K"]) * 1e3, updates_per_s=g2["nf"] * len(g2["K"]) / g2["seconds"],
+            **loop_summary(g2["K"], g2["rmax"]))
     if not args.no_batch:
         out["batched"] = run_batch(args)
     if not args.no_adjoint:
