@@ -28,8 +28,11 @@
 //   integration : pinned against the reference's own translation units compiled
 //   into oracle/_ref (when /root/reference is present) and closed forms;
 //   neighbour search : brute force; sort : std::stable_sort and invariants;
-//   density error factor : closed-form lattice sum; PCISPH / WCSPH step :
-//   conservation laws, hydrostatic rest, FP64 twin of the same restatement;
+//   density error factor : closed-form lattice sum; PCISPH / WCSPH step and
+//   the coupling pair forces : few-particle closed forms in float64 from the
+//   published formulas (tests/test_oracle_pins.py, which also shows that each
+//   planted mistake fails them), conservation laws, a settling hydrostatic
+//   column, FP64 twin of the same restatement;
 //   OBJ reader / lattice fill and the particle text files (UnifiedIO
 //   writers and restart reader): the reference's own translation units in
 //   oracle/_ref and golden results written by them (tests/golden/).

@@ -95,8 +95,6 @@ struct StepArgs {
     int n_scenes;
     int slots_per_scene;
     int code_lists;      // the correction loop reads delta-coded lists (the external-force pass writes them)
-    uint32_t st_base;    // geometry of the kernel step tables (step_geom of support_radius_sq_limit)
-    float st_low;
     int xcd;             // remap blocks to XCDs by contiguous ranges (grids are multiples of 8)
 };
 
@@ -297,93 +295,18 @@ __device__ __forceinline__ uint32_t bin_of_d2(float d2, float factor, const floa
     return b0 - (d2 < t.x ? 1u : 0u) + (d2 >= t.y ? 1u : 0u);
 }
 
-// Kernel values as step functions of the squared distance.  A pass that
-// needs V[bin(d2)] (V one of the tables, bin as above) reads it from a table
-// of cells of the float d2 instead: ST_OCT octaves of d2 below and around
-// the support radius squared, 256 cells per octave (cell = the top 8
-// mantissa bits), each cell {t, v}: v = the value at the cell's first float,
-// t = the first float of the cell at which the value changes (+inf if none).
-// A cell is narrower than any bin (a bin spans a relative distance of at
-// least 1/300, a cell 1/512), so it holds at most one change; the host
-// builds the cells from the exact thresholds and upf_create checks every
-// float d2 on the device.  The value is v of the cell, or v of the next cell
-// once d2 >= t -- one LDS read of two entries, a compare and a select, no
-// square root.  Squared distances below the first cell (pairs closer than
-// about h / 6, rare) take the square-root path; those above the last cell
-// clamp to it (its value there is that of every farther pair).
-constexpr int ST_OCT = 6;
-constexpr int ST_CELLS = 256 * ST_OCT;
-constexpr int ST_STRIDE = 2 * (ST_CELLS + 1) + 2;  // floats per table (+ a sentinel cell, 16-byte aligned)
-constexpr int ST_OFF = (LUT_TOTAL + 3) / 4 * 4;     // float offset of the step tables in the LUT buffer
-enum { ST_WM = 0, ST_W = 1, ST_G = 2, ST_VL = 3, ST_COUNT = 4 };
-constexpr int LUT_ALL = ST_OFF + ST_COUNT * ST_STRIDE;
-
-// geometry of the cells: the top octave holds T = support_radius_sq_limit
-struct StepGeom {
-    uint32_t base;  // (float bits >> 15) of the first cell
-    float low;      // first float of the first cell
-};
-__host__ __device__ inline StepGeom step_geom(float T) {
-    uint32_t tb;
-#ifdef __HIP_DEVICE_COMPILE__
-    tb = __float_as_uint(T);
-#else
-    std::memcpy(&tb, &T, 4);
-#endif
-    const uint32_t e0 = (tb >> 23) - (ST_OCT - 1);
-    StepGeom g;
-    g.base = e0 << 8;
-    const uint32_t lb = e0 << 23;
-#ifdef __HIP_DEVICE_COMPILE__
-    g.low = __uint_as_float(lb);
-#else
-    std::memcpy(&g.low, &lb, 4);
-#endif
-    return g;
-}
-
-// block-wide copy of step table k into LDS (no barrier)
-__device__ inline void load_step_table(float2* dst, const float* __restrict__ luts, int k) {
-    const float2* src = reinterpret_cast<const float2*>(luts + ST_OFF + k * ST_STRIDE);
-    for (int c = threadIdx.x; c <= ST_CELLS; c += blockDim.x) dst[c] = src[c];
-}
-
-// the fast part: valid for d2 >= low (else the caller takes the root path)
-__device__ __forceinline__ float step_fast(float d2, const float2* __restrict__ tab, uint32_t base) {
-    const uint32_t c = min((__float_as_uint(d2) >> 15) - base, (uint32_t)(ST_CELLS - 1));
-    const float2 a = tab[c], b = tab[c + 1];
-    return d2 >= a.x ? b.y : a.y;
-}
-
 // Counts the floats d2 in [0, limit) whose bin_of_d2 differs from
-// lut_bin(sqrt_rn(d2)), and those for which a step table differs from its
-// definition: ST_WM = m W[bin] below T else 0, ST_W = W[bin] below T else 0,
-// ST_G = G[bin], ST_VL = VL[bin].
-__global__ void k_check_bins(float factor, const float* __restrict__ luts, uint32_t limit_bits, float T, float m,
+// lut_bin(sqrt_rn(d2)).
+__global__ void k_check_bins(float factor, const float* __restrict__ luts, uint32_t limit_bits,
                              unsigned long long* bad) {
     __shared__ float2 th[LUT_N];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x)
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
     __syncthreads();
-    const StepGeom g = step_geom(T);
-    const float2* tabs = reinterpret_cast<const float2*>(luts + ST_OFF);
-    const float* W = luts;
-    const float* G = luts + LUT_N;
-    const float* VL = luts + 2 * LUT_N;
     unsigned long long n_bad = 0;
     for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < limit_bits; u += gridDim.x * blockDim.x) {
         const float d2 = __uint_as_float(u);
-        const uint32_t b = lut_bin(sqrtf(d2), factor);
-        n_bad += bin_of_d2(d2, factor, th) != b ? 1ull : 0ull;
-        if (d2 < g.low) continue;
-        const float wb = b >= 300 ? 0.0f : W[b];
-        const float ref[ST_COUNT] = {d2 < T ? wb * m : 0.0f, d2 < T ? wb : 0.0f, b >= 300 ? 0.0f : G[b],
-                                     b >= 300 ? 0.0f : VL[b]};
-#pragma unroll
-        for (int k = 0; k < ST_COUNT; ++k) {
-            const float v = step_fast(d2, tabs + (size_t)k * (ST_STRIDE / 2), g.base);
-            n_bad += __float_as_uint(v) != __float_as_uint(ref[k]) ? 1ull : 0ull;
-        }
+        n_bad += bin_of_d2(d2, factor, th) != lut_bin(sqrtf(d2), factor) ? 1ull : 0ull;
     }
     if (n_bad) atomicAdd(bad, n_bad);
 }
@@ -672,31 +595,16 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
 // which keeps the LDS free for more waves).
-//
-// CODE: the lists leave delta-coded (the layout of ListEncoder: first slot
-// in nbf, then 16-bit increments, eight per 16-byte word of nbw, column-
-// major; an increment >= 2^16 is 0 plus an entry of nbe).  Each passing
-// lane takes the slot of the next lower passing lane of its ballot (or the
-// particle's last entry) as its predecessor.  The first CODE_STAGE
-// increments of every list are staged in LDS (one column per slot, 16-bit)
-// and leave as whole 16-byte words of 64 consecutive columns, the rest (rare)
-// go to nbw directly.
-constexpr int CODE_STAGE = 64;
-constexpr int CODE_COL = CODE_STAGE + 2;  // 16-bit entries per LDS column (odd dword stride)
-
-template <int ROUNDS, int STAGE, int BIN, int CODE = 0>
+template <int ROUNDS, int STAGE, int BIN>
 __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
-                                                  uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl,
-                                                  Work W) {
-    static_assert(!(CODE && (STAGE || BIN)), "the delta-coded output stages its own lists and carries no bins");
+                                                  uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
     constexpr int NB_CHUNK = 64 * ROUNDS;
     // packed cell coordinates: 10-bit fields whose top bit (coordinates are
     // below 512) serves as the borrow guard of a packed compare
     constexpr uint32_t GUARD = (1u << 9) | (1u << 19) | (1u << 29);
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
     __shared__ uint32_t stage[(STAGE > 0 ? STAGE : 1) * 65];
-    __shared__ uint16_t cst[CODE ? 64 * CODE_COL : 2];
     const upf_params& P = A.P;
     const float lut_factor = A.lut_factor;
     const int n = A.n;
@@ -737,9 +645,6 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
     }
     const uint64_t liquid = __ballot(my_liquid);
     int my_cnt = 0;  // this lane's slot: list length so far
-    uint32_t my_prev = 0u;  // CODE: its last entry's slot
-    int my_esc = 0;         // CODE: its escapes so far
-    const size_t st = (size_t)stride;
     int s = base;
     while (s < wend) {
         const int sl = s - base;
@@ -828,8 +733,6 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                 const uint32_t blo = (uint32_t)__builtin_amdgcn_readlane((int)my_blo, pl);
                 const uint32_t bhi_g = (uint32_t)__builtin_amdgcn_readlane((int)my_bhi, pl);
                 int cnt = __builtin_amdgcn_readlane(my_cnt, pl);
-                uint32_t prev = CODE ? (uint32_t)__builtin_amdgcn_readlane((int)my_prev, pl) : 0u;
-                int esc = CODE ? __builtin_amdgcn_readlane(my_esc, pl) : 0;
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
                     if (r >= rounds) break;
@@ -841,35 +744,6 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     const bool pass = inbox & (d2 < T) & (cj[r] != p);
                     const uint64_t m = __ballot(pass);
                     if (m == 0ull) continue;
-                    if constexpr (CODE) {
-                        // predecessor: the next lower passing lane's candidate
-                        const uint64_t lower = m & lt;
-                        const int src = lower ? 63 - __builtin_clzll(lower) : lane;
-                        const uint32_t pj = (uint32_t)__shfl(cj[r], src, 64);
-                        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                        const int k = cnt + (int)below;
-                        const uint32_t d = (uint32_t)cj[r] - (lower ? pj : prev);
-                        const bool big = pass && k > 0 && d > 65535u;
-                        const uint64_t em = __ballot(big);
-                        if (pass) {
-                            if (k == 0) W.nbf[p] = (uint32_t)cj[r];
-                            if (big) {
-                                const int e = esc + (int)__popcll(em & lt);
-                                if (e < NB_ESC) W.nbe[(size_t)e * st + p] = d;
-                                else set_err(ctl, UPF_ERR_NBR_OVERFLOW);
-                            }
-                            const uint16_t wv = (k == 0 || big) ? (uint16_t)0 : (uint16_t)d;
-                            if (k < CODE_STAGE)
-                                cst[pl * CODE_COL + k] = wv;
-                            else if (k < cap)
-                                reinterpret_cast<uint16_t*>(W.nbw + (size_t)(k >> 3) * st + p)[k & 7] = wv;
-                        }
-                        esc += (int)__popcll(em);
-                        prev = (uint32_t)__builtin_amdgcn_readlane(cj[r], 63 - __builtin_clzll(m));
-                        cnt += __popcll(m);
-                        continue;
-                    }
                     if (pass) {
                         const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -888,25 +762,11 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                     }
                     cnt += __popcll(m);
                 }
-                if (lane == pl) {
-                    my_cnt = cnt;
-                    if (CODE) { my_prev = prev; my_esc = esc; }
-                }
+                if (lane == pl) my_cnt = cnt;
             }
         }
         __syncthreads();  // r_* are rewritten for the next cell
         s = pe;
-    }
-    if constexpr (CODE) {
-        // the staged increments leave as whole rows: row q of the wave's 64
-        // columns is 1 KB of consecutive memory
-        __syncthreads();
-        const int mine = mine_valid ? min(min(my_cnt, cap), CODE_STAGE) : 0;
-        int rows = (mine + 7) >> 3;
-        for (int o = 32; o > 0; o >>= 1) rows = max(rows, __shfl_xor(rows, o, 64));
-        const uint32_t* col = reinterpret_cast<const uint32_t*>(cst + lane * CODE_COL);
-        for (int q = 0; q < rows; ++q)
-            if (8 * q < mine) W.nbw[(size_t)q * st + me] = make_uint4(col[4 * q], col[4 * q + 1], col[4 * q + 2], col[4 * q + 3]);
     }
     if (mine_valid) {
         ncount[me] = my_cnt < cap ? my_cnt : cap;
@@ -1126,24 +986,26 @@ __device__ __forceinline__ float pressure_record_w(int type, float psi, float rh
     return 0.0f;
 }
 
-struct XV {
-    float4 x, v;
-};
+// The predicted-position record of a non-liquid particle: with Akinci
+// boundaries its w carries the particle's mass in the density sums, negated,
+// -(rho0 psi) (not the type bits of a liquid particle), so that the density
+// pass reads it with the position instead of gathering psi.
+template <int MODE>
+__device__ __forceinline__ float4 boundary_xs(float4 x, float psi, float rho0) {
+    if (MODE == UPF_BOUNDARY_AKINCI) x.w = -(rho0 * psi);
+    return x;
+}
 
-// SRC 0: the packed lists of the step (re-written delta-coded for the loop
-// when A.code_lists); SRC 1: the delta-coded lists the build wrote.
-template <int MODE, int SRC>
+template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                       Ctl* ctl) {
     __shared__ float lvl[LUT_N], lpg[LUT_N];
     __shared__ float2 th[LUT_N];
-    __shared__ float2 svl[ST_CELLS + 1];
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
         lvl[t] = luts[2 * LUT_N + t];
         lpg[t] = luts[LUT_N + t];
         th[t] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[t];
     }
-    load_step_table(svl, luts, ST_VL);
     __syncthreads();
     const upf_params& P = A.P;
     for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
@@ -1154,7 +1016,8 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     W.rho[i] = S.rho_last[i];
     if (ti != UPF_LIQUID) {
-        W.xs[i] = (ti == UPF_RIGID && !P.rigid_predicted_current) ? S.x0[i] : xi;
+        W.xs[i] = boundary_xs<MODE>((ti == UPF_RIGID && !P.rigid_predicted_current) ? S.x0[i] : xi, S.psi[i],
+                                    P.rest_density);
         W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         continue;
     }
@@ -1168,20 +1031,17 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
     const uint32_t* L = W.nbr + i;
-    const bool bin_free = SRC == 1 || A.code_lists != 0;  // the lists of this step carry no bins
+    const bool bin_free = A.code_lists != 0;  // the lists of this step carry no bins
     const size_t st = (size_t)A.stride;
-    auto pair = [&](int j, uint32_t e, const float4& xj, const float4& vj) __attribute__((always_inline)) {
+    auto pair = [&](uint32_t e, const float4& xj, const float4& vj) __attribute__((always_inline)) {
         const int tj = ptype(xj);
         if (tj == UPF_LIQUID) {
-            float kv;
-            if (bin_free) {  // the list build's distance on the same positions
+            uint32_t bin = e & NB_BIN_MASK;
+            if (bin_free) {  // the list build's expression on the same positions
                 const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                kv = step_fast(d2, svl, A.st_base);
-                if (d2 < A.st_low) kv = lvl[bin_of_d2(d2, A.lut_factor, th)];
-            } else {
-                kv = lvl[e & NB_BIN_MASK];
+                bin = bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th);
             }
+            const float kv = lvl[bin];
             const float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
             fx = fx - vx * rv * rv * visc * kv;
             fy = fy - vy * rv * rv * visc * kv;
@@ -1200,7 +1060,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
                 uint32_t dl = (uint32_t)(dist * 300.0f / h);
                 dl = dl > 300u ? 300u : dl;
                 g = lpg[dl];
-                wv = S.psi[j];
+                wv = S.psi[(int)(e >> NB_SLOT_SHIFT)];
             }
             const float s1 = (((dx * g) * pg) * P.initial_mass) * P.rest_density;
             const float s2 = (((dy * g) * pg) * P.initial_mass) * P.rest_density;
@@ -1210,10 +1070,6 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
             fz = fz + (s3 * wv) * -1.0f;
         }
     };
-    if constexpr (SRC == 1) {
-        for_each_listed<1, 4>(W, i, st, cnt, [&](int j) { return XV{S.pos[j], S.vel[j]}; },
-                              [&](int j, uint32_t, const XV& q) { pair(j, 0u, q.x, q.v); });
-    } else {
     ListEncoder enc(W, i, st);
     const bool code = A.code_lists != 0;
     int k = 0;
@@ -1230,16 +1086,15 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
             vj[u] = S.vel[j];
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) pair((int)(e[u] >> NB_SLOT_SHIFT), e[u], xj[u], vj[u]);
+        for (int u = 0; u < 4; ++u) pair(e[u], xj[u], vj[u]);
     }
     for (; k < cnt; ++k) {
         const uint32_t e = L[k * st];
         const int j = (int)(e >> NB_SLOT_SHIFT);
         if (code) enc.put(k, (uint32_t)j, ctl);
-        pair(j, e, S.pos[j], S.vel[j]);
+        pair(e, S.pos[j], S.vel[j]);
     }
     if (code) enc.finish();
-    }
     fy = fy - P.initial_mass * P.gravity;
     const Dom d = dom_at(A, i);
     if (P.add_boundary_force) box_force_static(P, d, xi, fx, fy, fz);
@@ -1282,13 +1137,10 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     __shared__ float2 th[LUT_N];
     __shared__ float wmax[BLOCK / 64];  // blockDim.x <= BLOCK
     __shared__ float4 urec[CODED == 2 ? UNION_CAP : 1];  // the chunk's union: predicted positions
-    // W (Akinci: times the neighbour's mass below) or m W, 0 from T on
-    __shared__ float2 sw[ST_CELLS + 1];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
         lw[k] = k < 300 ? luts[k] : 0.0f;  // bin 300 (dist >= h) weighs 0, as in lut_at
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
     }
-    load_step_table(sw, luts, MODE == UPF_BOUNDARY_AKINCI ? ST_W : ST_WM);
     __syncthreads();
     const upf_params& P = A.P;
     const int lane = threadIdx.x & 63;
@@ -1317,18 +1169,19 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 auto use = [&](int j, uint32_t, const float4& xj) __attribute__((always_inline)) {
                     const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
                     const float d2 = dx * dx + dy * dy + dz * dz;
-                    // without a branch: a pair at d2 >= T adds w = +0, and
-                    // rho + (+0) * m_j is rho exactly (rho > 0, m_j >= 0)
                     if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
-                        float w = step_fast(d2, sw, A.st_base);
-                        if (d2 < A.st_low) w = lw[bin_of_d2(d2, A.lut_factor, th)];
-                        float mj = m;
-                        if (ptype(xj) != UPF_LIQUID && d2 < T) mj = P.rest_density * S.psi[j];
-                        rho = rho + w * mj;
+                        if (d2 < T) {
+                            const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
+                            // a boundary neighbour's record carries -(rho0 psi)
+                            const float mj = ptype(xj) == UPF_LIQUID ? m : -xj.w;
+                            rho = rho + w * mj;
+                        }
                     } else {
-                        float wm = step_fast(d2, sw, A.st_base);  // the table holds W * m
-                        if (d2 < A.st_low) wm = lw[bin_of_d2(d2, A.lut_factor, th)] * m;
-                        rho = rho + wm;
+                        // without a branch: a pair at d2 >= T adds w = +0, and
+                        // rho + (+0) * m is rho exactly (rho > 0)
+                        const float wt = lw[bin_of_d2(d2, A.lut_factor, th)];
+                        const float w = d2 < T ? wt : 0.0f;
+                        rho = rho + w * m;
                     }
                 };
                 auto from_global = [&](int j) { return at_u32(W.xs, j); };
@@ -1486,12 +1339,10 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     __shared__ float lpg[LUT_N];
     __shared__ float2 th[LUT_N];
     __shared__ float4 urec[CODED == 2 ? UNION_CAP : 1];  // the chunk's union: (position, p / rho0^2)
-    __shared__ float2 sg[CODED ? ST_CELLS + 1 : 1];
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
         lpg[k] = luts[LUT_N + k];
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
     }
-    if (CODED) load_step_table(sg, luts, ST_G);
     __syncthreads();
     const upf_params& P = A.P;
     const int n_round = (A.n + gstride() - 1) / gstride();
@@ -1520,14 +1371,8 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     auto use = [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
         const float4& xj = q.x;
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        float g;
-        if (CODED) {
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            g = step_fast(d2, sg, A.st_base);
-            if (d2 < A.st_low) g = lpg[bin_of_d2(d2, A.lut_factor, th)];
-        } else {
-            g = lpg[e & NB_BIN_MASK];
-        }
+        const uint32_t bin = CODED ? bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th) : (e & NB_BIN_MASK);
+        const float g = lpg[bin];
         // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
         // precomputed per particle by the density pass (same division)
         float pd = q.pd, mj = m;
@@ -1799,7 +1644,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
     W.fp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     W.rho[i] = S.rho_last[i];
     if (ti != UPF_LIQUID && ti != UPF_RIGID) {
-        W.xs[i] = xi;
+        W.xs[i] = boundary_xs<MODE>(xi, S.psi[i], P.rest_density);
         W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
@@ -1942,7 +1787,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
         if (hit) { b[0] -= cd * vi.x; b[1] -= cd * vi.y; b[2] -= cd * vi.z; }
         fx = fx + b[0]; fy = fy + b[1]; fz = fz + b[2];
         W.force[i] = make_float4(fx, fy, fz, 0.0f);
-        W.xs[i] = P.rigid_predicted_current ? xi : S.x0[i];
+        W.xs[i] = boundary_xs<MODE>(P.rigid_predicted_current ? xi : S.x0[i], S.psi[i], P.rest_density);
     }
 }
 
@@ -2289,7 +2134,6 @@ struct upf_sim {
     int steps_done = 0;
     upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4, 1};
     bool static_lists = false;  // the lists being built serve the static PCISPH step only
-    bool lists_coded = false;   // this step's build wrote the delta-coded lists (no packed lists)
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
     uint32_t ncells = 0;
@@ -2335,9 +2179,6 @@ struct upf_sim {
         A.slots_per_scene = sps;
         A.code_lists = tune.loop_list;
         A.xcd = tune.xcd_remap;
-        const StepGeom sg = step_geom(P.support_radius_sq_limit);
-        A.st_base = sg.base;
-        A.st_low = sg.low;
         return A;
     }
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -2518,27 +2359,17 @@ static void build_lists(upf_sim* s, const StepArgs& A) {
     Slots& S = s->S[s->cur];
     s->prof.begin(PC_NBR_BUILD, s->st);
     const dim3 g((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
-    auto launch_tile = [&](auto kern) {
-        hipLaunchKernelGGL(kern, g, dim3(64), 0, s->st, A, S.pos, s->W.keys_sorted, s->W.cells, s->W.nbr, s->W.ncount,
-                           s->ctl);
-    };
     auto launch = [&](auto kern) {
         hipLaunchKernelGGL(kern, g, dim3(64), 0, s->st, A, S.pos, s->W.keys_sorted, s->W.cells, s->W.nbr, s->W.ncount,
-                           s->ctl, s->W);
+                           s->ctl);
     };
     const int r = s->tune.nbr_rounds;
     // the correction loop of the delta-coded lists never reads the bins
     const bool bins = !(A.code_lists && s->static_lists);
-    // static PCISPH with delta-coded loop lists: the build writes them directly
-    s->lists_coded = s->static_lists && A.code_lists == 1 && s->tune.nbr_tile == 0 && s->tune.nbr_stage == 0;
-    if (s->lists_coded) {
-        if (r == 2) launch(k_nbr_build<2, 0, 0, 1>);
-        else if (r == 8) launch(k_nbr_build<8, 0, 0, 1>);
-        else launch(k_nbr_build<4, 0, 0, 1>);
-    } else if (s->tune.nbr_tile == 256) {
-        launch_tile(k_nbr_build_tile<256>);
+    if (s->tune.nbr_tile == 256) {
+        launch(k_nbr_build_tile<256>);
     } else if (s->tune.nbr_tile == 512) {
-        launch_tile(k_nbr_build_tile<512>);
+        launch(k_nbr_build_tile<512>);
     } else if (s->tune.nbr_stage == 0) {
         if (!bins) {
             if (r == 2) launch(k_nbr_build<2, 0, 0>);
@@ -2583,16 +2414,10 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
     build_lists(s, A);
     s->static_lists = false;
     s->prof.begin(PC_NBR_EXT, s->st);
-    if (s->lists_coded) {
-        if (akinci)
-            hipLaunchKernelGGL((k_ext_static<UPF_BOUNDARY_AKINCI, 1>), gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
-        else
-            hipLaunchKernelGGL((k_ext_static<UPF_BOUNDARY_CPU_REF, 1>), gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
-    } else if (akinci) {
-        hipLaunchKernelGGL((k_ext_static<UPF_BOUNDARY_AKINCI, 0>), gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
-    } else {
-        hipLaunchKernelGGL((k_ext_static<UPF_BOUNDARY_CPU_REF, 0>), gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
-    }
+    if (akinci)
+        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
+    else
+        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
     s->union_ready = s->tune.loop_list == 2;
     if (s->union_ready) {
@@ -2736,51 +2561,6 @@ static void build_bin_thresholds(float factor, float* out) {
     }
 }
 
-// The step table of V (see step_fast): cut -> the value is 0 for d2 >= T;
-// scale multiplies every value (as the pass would, in float).  thr2 = the
-// threshold pairs of build_bin_thresholds.  False if a cell would hold two
-// changes of value (the cells are too wide for the bins), or if the value
-// at the end of the last cell is not 0 (farther pairs clamp to it).
-static bool build_step_table(const float* V, bool cut, float T, float scale, const float* thr2, float* out) {
-    const StepGeom g = step_geom(T);
-    auto bits = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; };
-    auto fl = [](uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; };
-    // the bin of a float: the largest k with thr[k] <= d2 (thr2[2k] = thr[k])
-    auto value = [&](uint32_t u) {
-        const float d2 = fl(u);
-        int b = 0;
-        while (b < 300 && !(d2 < thr2[2 * (b + 1)])) ++b;
-        float v = b >= 300 ? 0.0f : V[b];
-        if (cut && !(d2 < T)) v = 0.0f;
-        return v * scale;
-    };
-    // the floats at which the value may change: the thresholds and T
-    std::vector<uint32_t> ev;
-    for (int k = 1; k <= 300; ++k) ev.push_back(bits(thr2[2 * k]));
-    if (cut) ev.push_back(bits(T));
-    std::sort(ev.begin(), ev.end());
-    for (int c = 0; c <= ST_CELLS; ++c) {
-        const uint32_t lo = (g.base + (uint32_t)c) << 15, hi = lo + (1u << 15);
-        const float v0 = value(lo);
-        float t = std::numeric_limits<float>::infinity();
-        int changes = 0;
-        float cur = v0;
-        for (uint32_t e : ev) {
-            if (e <= lo || e >= hi) continue;
-            const float v = value(e);
-            if (bits(v) != bits(cur)) {
-                if (++changes == 1) t = fl(e);
-                cur = v;
-            }
-        }
-        if (changes > 1) return false;
-        out[2 * c] = c == ST_CELLS ? std::numeric_limits<float>::infinity() : t;
-        out[2 * c + 1] = v0;
-    }
-    // pairs beyond the last cell read it: its value at its end must be theirs (0)
-    return bits(value(((g.base + ST_CELLS) << 15) - 1)) == 0u && bits(out[2 * ST_CELLS + 1]) == 0u;
-}
-
 static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     const upf_params* p = &s->P;
     s->cap = capacity;
@@ -2813,7 +2593,7 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.uslots, nchunks * UNION_CAP));
     HIPCHK(s->alloc(&W.un, nchunks));
     HIPCHK(s->alloc(&W.cells, ncells));
-    HIPCHK(s->alloc(&s->luts, LUT_ALL));
+    HIPCHK(s->alloc(&s->luts, LUT_TOTAL));
     HIPCHK(s->alloc(&s->ctl, 1));
     HIPCHK(s->alloc(&s->sctl, n_scenes));
     void* tmp = nullptr;
@@ -2825,39 +2605,21 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(hipHostMalloc((void**)&s->mirror, sizeof(CtlMirror), hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(s->mirror, 0, sizeof(CtlMirror));
     HIPCHK(hipHostGetDevicePointer((void**)&s->mirror_dev, s->mirror, 0));
-    std::vector<float> L(LUT_ALL, 0.0f);
+    std::vector<float> L(LUT_TOTAL, 0.0f);
     build_luts(p->support_radius, L.data());
     const float factor = (float)(1.0 / (double)p->support_radius);  // StepArgs::lut_factor
     build_bin_thresholds(factor, L.data() + LUT_TH_OFF);
-    {
-        const float T = p->support_radius_sq_limit, m = p->initial_mass;
-        const float* th2 = L.data() + LUT_TH_OFF;
-        float* st = L.data() + ST_OFF;
-        const bool ok = build_step_table(L.data(), true, T, m, th2, st + ST_WM * ST_STRIDE) &&
-                        build_step_table(L.data(), true, T, 1.0f, th2, st + ST_W * ST_STRIDE) &&
-                        build_step_table(L.data() + LUT_N, false, T, 1.0f, th2, st + ST_G * ST_STRIDE) &&
-                        build_step_table(L.data() + 2 * LUT_N, false, T, 1.0f, th2, st + ST_VL * ST_STRIDE);
-        if (!ok) {
-            std::fprintf(stderr, "upf: kernel step tables cannot be built for h = %g\n", (double)p->support_radius);
-            return UPF_ERR_ARG;
-        }
-    }
     HIPCHK(hipMemcpy(s->luts, L.data(), L.size() * sizeof(float), hipMemcpyHostToDevice));
     {
         // the sqrt-free bins equal the correctly rounded ones for every float
-        // below 2 thr[300] (above it both are 300), and so do the step
-        // tables' values (up to four times the last cell's end: every
-        // farther float clamps to the same cell)
+        // below 2 thr[300] (above it both are 300)
         unsigned long long* bad = nullptr;
         HIPCHK(s->alloc(&bad, 1));
         HIPCHK(hipMemset(bad, 0, sizeof(unsigned long long)));
-        float top = 2.0f * L[LUT_TH_OFF + 2 * 300];
-        const StepGeom g = step_geom(p->support_radius_sq_limit);
+        const float top = 2.0f * L[LUT_TH_OFF + 2 * 300];
         uint32_t top_bits;
         std::memcpy(&top_bits, &top, 4);
-        top_bits = std::max(top_bits, (g.base + ST_CELLS + 512u) << 15);
-        hipLaunchKernelGGL(k_check_bins, dim3(4096), dim3(256), 0, s->st, factor, s->luts, top_bits,
-                           p->support_radius_sq_limit, p->initial_mass, bad);
+        hipLaunchKernelGGL(k_check_bins, dim3(4096), dim3(256), 0, s->st, factor, s->luts, top_bits, bad);
         unsigned long long h_bad = 1;
         HIPCHK(hipMemcpyAsync(&h_bad, bad, sizeof(h_bad), hipMemcpyDeviceToHost, s->st));
         HIPCHK(hipStreamSynchronize(s->st));
