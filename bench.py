@@ -129,7 +129,10 @@ def run_gpu(args, workload=None, steps=None, profile=True):
     n = len(typ)
     sim = upf.Simulation(p, n)
     tuning = None
-    if not args.no_tune:
+    if args.tuning:
+        tuning = dict(sim.tuning().as_dict(), **json.loads(args.tuning))
+        sim.set_tuning(tuning)
+    elif not args.no_tune:
         from unifiedparticleframeworkcuda_amd import autotune
         trials = []
         tuning, tune_ms = autotune.tune_simulation(sim, pos, typ, "%s_%dx%dx%d" % (workload, nx, ny, nz),
@@ -245,6 +248,12 @@ def run_batch(args):
     b.step(K, use_graph=False)
     hip.hipDeviceSynchronize()
     th = time.perf_counter() - t0
+    b.step(2, use_graph=2)  # capture of the split graphs + one step
+    hip.hipDeviceSynchronize()
+    t0 = time.perf_counter()
+    b.step(K, use_graph=2)
+    hip.hipDeviceSynchronize()
+    ts = time.perf_counter() - t0
     Ks, rs = b.stats()
     b.close()
     sims = []
@@ -261,9 +270,11 @@ def run_batch(args):
     tp = time.perf_counter() - t0
     for s in sims:
         s.close()
-    out.update(graph_updates_per_s=nf * K / tg, host_loop_updates_per_s=nf * K / th, plain_updates_per_s=nf * K / tp,
-               graph_ms_per_step=tg / K * 1e3, plain_ms_per_step=tp / K * 1e3,
-               speedup_graph_vs_plain=tp / tg, iterations_mean=float(np.mean(Ks[2:2 + K])),
+    out.update(graph_updates_per_s=nf * K / tg, graph_split_updates_per_s=nf * K / ts,
+               host_loop_updates_per_s=nf * K / th, plain_updates_per_s=nf * K / tp,
+               graph_ms_per_step=tg / K * 1e3, graph_split_ms_per_step=ts / K * 1e3, plain_ms_per_step=tp / K * 1e3,
+               speedup_graph_vs_plain=tp / tg, speedup_graph_split_vs_plain=tp / ts,
+               iterations_mean=float(np.mean(Ks[2:2 + K])),
                iterations_max=int(np.max(Ks[2:2 + K])),
                max_density_error_pct=float(max(0.0, 0.1 * float(np.max(rs[2:2 + K])) - 100.0)))
     return out
@@ -543,6 +554,8 @@ def main():
     ap.add_argument("--no-state-io", action="store_true")
     ap.add_argument("--no-wall-weight", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch shapes, no autotuning")
+    ap.add_argument("--tuning", default=None,
+                    help="launch shapes as JSON (fields of upf_tuning; the rest default), instead of autotuning")
     ap.add_argument("--retune", action="store_true", help="ignore the tuning cache")
     ap.add_argument("--show-trials", action="store_true", help="print every tuning trial to stderr")
     ap.add_argument("--no-speculate", action="store_true",

@@ -337,12 +337,17 @@ int upf_batch_create(int n_scenes, const upf_params* scene_params,
 int upf_batch_destroy(upf_batch* b);
 /* Particles of all scenes concatenated scene by scene. */
 int upf_batch_upload(upf_batch* b, const float* pos4, const float* vel4, const int* type);
-/* nsteps PCISPH static-boundary steps of every scene.  use_graph != 0: each
+/* nsteps PCISPH static-boundary steps of every scene.  use_graph == 1: each
  * step replays one hipGraph holding the whole step, the correction loop
  * included (max_loops iterations, the convergence test on the device;
  * iterations past every scene's decision exit at once), with no host
- * round trip; errors are checked once at the end.  use_graph == 0: the host
- * reads the loop decision back as upf_step does.                          */
+ * round trip; errors are checked once at the end.  use_graph == 2: split
+ * graphs -- the step's head with the first 4 iterations, then graphs of 4
+ * further iterations while the device's decision (read back after each, the
+ * previous step's count going ahead unread) asks for more, then the tail;
+ * no idle iterations beyond the last chunk.  use_graph == 0: no graphs, the
+ * host reads the loop decision back as upf_step does.  All three give the
+ * same results bit for bit.                                               */
 int upf_batch_step(upf_batch* b, int nsteps, int use_graph);
 /* Concatenated upload order, like upf_batch_upload. */
 int upf_batch_download(upf_batch* b, float* pos4, float* vel4, float* rho);

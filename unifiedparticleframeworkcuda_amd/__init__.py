@@ -505,7 +505,11 @@ class Batch:
         check("upf_batch_upload", lib().upf_batch_upload(self.h, ptr(pos4), ptr(vel), ptr(types, C.c_int)))
 
     def step(self, nsteps=1, use_graph=True):
-        check("upf_batch_step", lib().upf_batch_step(self.h, int(nsteps), 1 if use_graph else 0))
+        """use_graph: True / 1 every step one graph replay (the whole correction
+        loop on the device), 2 the split graphs (head, loop chunks while the
+        device's decision asks for more, tail), False / 0 no graphs."""
+        mode = (1 if use_graph else 0) if isinstance(use_graph, bool) else int(use_graph)
+        check("upf_batch_step", lib().upf_batch_step(self.h, int(nsteps), mode))
 
     def state(self):
         n = self.n

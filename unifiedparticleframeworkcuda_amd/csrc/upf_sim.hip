@@ -1315,11 +1315,6 @@ __global__ void k_publish(const Ctl* __restrict__ ctl, const SceneCtl* __restric
 // ComputeCorrectivePressureForce (:5493-5522), then the prediction of the
 // next iteration for the same particle; the loop test of the iteration runs
 // first (the force pass does not depend on it).
-struct PosPd {
-    float4 x;
-    float pd;
-};
-
 struct PosVel {
     float4 x, v;
 };
@@ -1368,17 +1363,16 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     // recomputed by the same expression for the coded lists)
     // (one 16-byte record per neighbour: current position and p / rho0^2,
     // or for an Akinci boundary neighbour its mass rho0 psi, negated)
-    auto use = [&](int j, uint32_t e, const PosPd& q) __attribute__((always_inline)) {
-        const float4& xj = q.x;
+    auto use = [&](int j, uint32_t e, const float4& xj) __attribute__((always_inline)) {
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const uint32_t bin = CODED ? bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th) : (e & NB_BIN_MASK);
         const float g = lpg[bin];
         // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
         // precomputed per particle by the density pass (same division)
-        float pd = q.pd, mj = m;
+        float pd = xj.w, mj = m;
         if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
-            if (__builtin_signbit(q.pd)) {  // boundary: p_j = 0, mass rho0 psi_j
-                mj = -q.pd;
+            if (__builtin_signbit(xj.w)) {  // boundary: p_j = 0, mass rho0 psi_j
+                mj = -xj.w;
                 pd = 0.0f;
             }
         }
@@ -1387,10 +1381,10 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
         fy = fy - (((dy * g) * grad) * m) * mj;
         fz = fz - (((dz * g) * grad) * m) * mj;
     };
-    auto from_global = [&](int j) { const float4 r = at_u32(W.xpd, j); return PosPd{r, r.w}; };
+    auto from_global = [&](int j) { return at_u32(W.xpd, j); };
     if constexpr (CODED == 2) {
         if (un >= 0)
-            for_each_listed<2, G>(W, i, st, cnt, [&](int u) { const float4 r = urec[u]; return PosPd{r, r.w}; }, use);
+            for_each_listed<2, G>(W, i, st, cnt, [&](int u) { return urec[u]; }, use);
         else
             for_each_listed<1, G>(W, i, st, cnt, from_global, use);
     } else {
@@ -2297,19 +2291,19 @@ static void enqueue_iteration(upf_sim* s, const StepArgs& A, int t) {
     // keeps the delta-coded lists
     constexpr int DU = MODE == UPF_BOUNDARY_AKINCI ? 1 : 2;
     if (ll == 2 && s->union_ready) {
-        if (lw == 8) UPF_PASS((k_density<MODE, DU, 8>), PC_DENSITY);
+        if (lw >= 8) UPF_PASS((k_density<MODE, DU, 8>), PC_DENSITY);
         else UPF_PASS((k_density<MODE, DU, 4>), PC_DENSITY);
-        if (lw == 8) UPF_PASS((k_force<MODE, 2, 8>), PC_FORCE);
+        if (lw >= 8) UPF_PASS((k_force<MODE, 2, 8>), PC_FORCE);
         else UPF_PASS((k_force<MODE, 2, 4>), PC_FORCE);
         return;
     }
-    if (ll && lw == 8) UPF_PASS((k_density<MODE, 1, 8>), PC_DENSITY);
+    if (ll && lw >= 8) UPF_PASS((k_density<MODE, 1, 8>), PC_DENSITY);
     else if (ll) UPF_PASS((k_density<MODE, 1, 4>), PC_DENSITY);
-    else if (lw == 8) UPF_PASS((k_density<MODE, 0, 8>), PC_DENSITY);
+    else if (lw >= 8) UPF_PASS((k_density<MODE, 0, 8>), PC_DENSITY);
     else UPF_PASS((k_density<MODE, 0, 4>), PC_DENSITY);
-    if (ll && lw == 8) UPF_PASS((k_force<MODE, 1, 8>), PC_FORCE);
+    if (ll && lw >= 8) UPF_PASS((k_force<MODE, 1, 8>), PC_FORCE);
     else if (ll) UPF_PASS((k_force<MODE, 1, 4>), PC_FORCE);
-    else if (lw == 8) UPF_PASS((k_force<MODE, 0, 8>), PC_FORCE);
+    else if (lw >= 8) UPF_PASS((k_force<MODE, 0, 8>), PC_FORCE);
     else UPF_PASS((k_force<MODE, 0, 4>), PC_FORCE);
 #undef UPF_PASS
 }
@@ -2392,11 +2386,12 @@ static void build_lists(upf_sim* s, const StepArgs& A) {
     s->prof.end(s->st);
 }
 
-// One PCISPH step with static boundaries (CpuParticlePhysicsStaticBoundariesPCISPH,
-// src/cpu/UnifiedPhysics.cpp:4220-4238).  device_loop: the loop decision stays
-// on the device and every iteration up to max_loops is enqueued (graph capture);
-// otherwise the host reads the decision back.
-static int step_pcisph_static(upf_sim* s, bool device_loop) {
+// The parts of a static-boundary step: the head (sort, cell table, boundary
+// volumes on the first step, lists, external forces and first prediction)
+// and the tail (integration, per-scene record).  The correction loop runs
+// between them; its iterations only touch the slot buffer the head sorted
+// into, so a batch captures head, loop chunks and tail as separate graphs.
+static int step_static_head(upf_sim* s) {
     int rc = sort_and_bin(s);
     if (rc) return rc;
     const StepArgs A = s->args();
@@ -2426,6 +2421,41 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
                            s->st, A, s->W);
         s->prof.end(s->st);
     }
+    HIPCHK(hipGetLastError());
+    return UPF_OK;
+}
+
+// correction iterations [t0, t1) of the step whose head ran (no loop init)
+static void step_static_iterations(upf_sim* s, int t0, int t1) {
+    const StepArgs A = s->args();
+    const bool akinci = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI;
+    for (int t = t0; t < t1; ++t) {
+        if (akinci) enqueue_iteration<UPF_BOUNDARY_AKINCI>(s, A, t);
+        else enqueue_iteration<UPF_BOUNDARY_CPU_REF>(s, A, t);
+    }
+}
+
+static int step_static_tail(upf_sim* s) {
+    const StepArgs A = s->args();
+    Slots& S = s->S[s->cur];
+    s->prof.begin(PC_INTEGRATE, s->st);
+    hipLaunchKernelGGL(k_integrate, grid_persist(s, s->n), block_loop(s), 0, s->st, A, S, s->W, 1, 0,
+                       s->rigid ? 1 : 0, s->ctl);
+    s->prof.end(s->st);
+    if (s->d_kstat)
+        hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, s->st, A, s->ctl, s->d_kstat, s->d_rstat, s->stat_cap);
+    HIPCHK(hipGetLastError());
+    return UPF_OK;
+}
+
+// CpuParticlePhysicsStaticBoundariesPCISPH (src/cpu/UnifiedPhysics.cpp:4220-4238).
+// device_loop: the loop decision stays on the device and every iteration up
+// to max_loops is enqueued; otherwise the host reads the decision back.
+static int step_pcisph_static(upf_sim* s, bool device_loop) {
+    int rc = step_static_head(s);
+    if (rc) return rc;
+    const StepArgs A = s->args();
+    const bool akinci = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI;
     if (device_loop) {
         if (akinci) pcisph_loop_device<UPF_BOUNDARY_AKINCI>(s, A);
         else pcisph_loop_device<UPF_BOUNDARY_CPU_REF>(s, A);
@@ -2433,13 +2463,7 @@ static int step_pcisph_static(upf_sim* s, bool device_loop) {
         rc = akinci ? pcisph_loop<UPF_BOUNDARY_AKINCI>(s, A) : pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
         if (rc) return rc;
     }
-    s->prof.begin(PC_INTEGRATE, s->st);
-    hipLaunchKernelGGL(k_integrate, gp, bl, 0, s->st, A, S, s->W, 1, 0, s->rigid ? 1 : 0, s->ctl);
-    s->prof.end(s->st);
-    if (s->d_kstat)
-        hipLaunchKernelGGL(k_record, dim3(1), dim3(256), 0, s->st, A, s->ctl, s->d_kstat, s->d_rstat, s->stat_cap);
-    HIPCHK(hipGetLastError());
-    return UPF_OK;
+    return step_static_tail(s);
 }
 
 // CpuParticlePhysicsVersatileCouplingPCISPH (src/cpu/UnifiedPhysics.cpp:4991-5024)
@@ -3143,9 +3167,29 @@ struct upf_batch {
     std::vector<int> counts;
     std::vector<int> scene_of;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};  // one step, by the slot buffer it starts from
-    upf_tuning captured{};                          // launch shapes baked into gexec
-    upf_params captured_p{};                        // physics baked into gexec (StepArgs, loop bounds)
+    // split form, by the slot buffer the step starts from: the head with the
+    // first LOOP_CHUNK iterations, LOOP_CHUNK further iterations each, the tail
+    hipGraphExec_t head[2] = {nullptr, nullptr};
+    std::vector<hipGraphExec_t> chunk[2];
+    hipGraphExec_t tail[2] = {nullptr, nullptr};
+    int last_k = 0;                                 // split form: iterations of the previous step
+    upf_tuning captured{};                          // launch shapes baked into the graphs
+    upf_params captured_p{};                        // physics baked into the graphs (StepArgs, loop bounds)
+
+    void drop_graphs() {
+        for (auto& g : gexec)
+            if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+        for (int c = 0; c < 2; ++c) {
+            if (head[c]) { (void)hipGraphExecDestroy(head[c]); head[c] = nullptr; }
+            if (tail[c]) { (void)hipGraphExecDestroy(tail[c]); tail[c] = nullptr; }
+            for (auto& g : chunk[c])
+                if (g) (void)hipGraphExecDestroy(g);
+            chunk[c].clear();
+        }
+    }
 };
+
+constexpr int LOOP_CHUNK = 4;  // correction iterations per graph of the split form
 
 // Captures one static-boundary PCISPH step starting from slot buffer `cur`
 // (the whole correction loop with the decision on the device).
@@ -3164,6 +3208,50 @@ static int capture_step(upf_sim* s, int cur, hipGraphExec_t* out) {
     s->prof.on = prof;
     if (!rc && hipGraphInstantiate(out, g, nullptr, nullptr, 0) != hipSuccess) rc = UPF_ERR_HIP;
     if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+template <class F>
+static int capture_graph(upf_sim* s, hipGraphExec_t* out, F enqueue) {
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal) != hipSuccess) return UPF_ERR_HIP;
+    int rc = enqueue();
+    const hipError_t e = hipStreamEndCapture(s->st, &g);
+    if (!rc && e != hipSuccess) rc = UPF_ERR_HIP;
+    if (!rc && hipGraphInstantiate(out, g, nullptr, nullptr, 0) != hipSuccess) rc = UPF_ERR_HIP;
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+// The split form's graphs for both starting slot buffers.
+static int capture_split(upf_batch* b) {
+    upf_sim* s = b->s;
+    const bool prof = s->prof.on;
+    s->prof.on = false;
+    const int cur0 = s->cur;
+    const int maxl = std::max(s->P.max_loops, 1);
+    int rc = UPF_OK;
+    for (int c = 0; c < 2 && !rc; ++c) {
+        s->cur = c;
+        rc = capture_graph(s, &b->head[c], [&] {
+            const int r = step_static_head(s);  // sorts into buffer 1 - c
+            if (r) return r;
+            hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(256), 0, s->st, s->args(), s->ctl);
+            step_static_iterations(s, 0, std::min(LOOP_CHUNK, maxl));
+            return (int)UPF_OK;
+        });
+        for (int t0 = LOOP_CHUNK; !rc && t0 < maxl; t0 += LOOP_CHUNK) {
+            hipGraphExec_t ge = nullptr;
+            rc = capture_graph(s, &ge, [&] {
+                step_static_iterations(s, t0, std::min(t0 + LOOP_CHUNK, maxl));
+                return (int)UPF_OK;
+            });
+            if (ge) b->chunk[c].push_back(ge);
+        }
+        if (!rc) rc = capture_graph(s, &b->tail[c], [&] { return step_static_tail(s); });
+    }
+    s->cur = cur0;
+    s->prof.on = prof;
     return rc;
 }
 
@@ -3251,8 +3339,7 @@ int upf_batch_create(int n_scenes, const upf_params* scene_params, const int* co
 
 int upf_batch_destroy(upf_batch* b) {
     if (!b) return UPF_ERR_ARG;
-    for (auto& g : b->gexec)
-        if (g) (void)hipGraphExecDestroy(g);
+    b->drop_graphs();
     if (b->s) upf_destroy(b->s);
     delete b;
     return UPF_OK;
@@ -3281,26 +3368,64 @@ int upf_batch_step(upf_batch* b, int nsteps, int use_graph) {
     // the graphs bake in the launch shapes and the physics (every kernel's
     // StepArgs, the number of loop iterations): recapture when upf_set_tuning
     // or upf_set_params (through upf_batch_sim) changed either
-    if (b->gexec[0] && (std::memcmp(&b->captured, &s->tune, sizeof(upf_tuning)) != 0 ||
-                        std::memcmp(&b->captured_p, &s->P, sizeof(upf_params)) != 0)) {
-        for (auto& g : b->gexec) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
+    if ((b->gexec[0] || b->head[0]) && (std::memcmp(&b->captured, &s->tune, sizeof(upf_tuning)) != 0 ||
+                                        std::memcmp(&b->captured_p, &s->P, sizeof(upf_params)) != 0))
+        b->drop_graphs();
+    if (use_graph == 2) {
+        // split form: head graph, then chunk graphs while the device's loop
+        // decision (read back) asks for more iterations, then the tail graph
+        if (k < nsteps && !b->head[0]) {
+            b->captured = s->tune;
+            b->captured_p = s->P;
+            const int rc = capture_split(b);
+            if (rc) {
+                b->drop_graphs();
+                return rc;
+            }
         }
-    }
-    if (k < nsteps && !b->gexec[0]) {
-        b->captured = s->tune;
-        b->captured_p = s->P;
-        const int cur = s->cur;
-        int rc = capture_step(s, 0, &b->gexec[0]);
-        if (!rc) rc = capture_step(s, 1, &b->gexec[1]);
-        s->cur = cur;
-        if (rc) return rc;
-    }
-    for (; k < nsteps; ++k) {
-        HIPCHK(hipGraphLaunch(b->gexec[s->cur], s->st));
-        s->cur = 1 - s->cur;
-        ++s->steps_done;
+        const int maxl = std::max(s->P.max_loops, 1);
+        for (; k < nsteps; ++k) {
+            const int c = s->cur;
+            HIPCHK(hipGraphLaunch(b->head[c], s->st));
+            int t = std::min(LOOP_CHUNK, maxl);
+            size_t q = 0;
+            // iterations the previous step needed go ahead of the first read-back,
+            // at most spec_first past the head's, as in the host loop
+            const int ahead = std::min(std::min(b->last_k, t + s->tune.spec_first), maxl);
+            while (t < ahead && q < b->chunk[c].size()) {
+                HIPCHK(hipGraphLaunch(b->chunk[c][q++], s->st));
+                t = std::min(t + LOOP_CHUNK, maxl);
+            }
+            for (;;) {
+                const int rc = read_ctl(s);
+                if (rc) return rc;
+                if (s->h_ctl->want_any <= t || q >= b->chunk[c].size() || (s->h_ctl->err & (1u << UPF_ERR_NAN)))
+                    break;
+                HIPCHK(hipGraphLaunch(b->chunk[c][q++], s->st));
+                t = std::min(t + LOOP_CHUNK, maxl);
+            }
+            b->last_k = s->h_ctl->want_any;
+            HIPCHK(hipGraphLaunch(b->tail[c], s->st));
+            s->cur = 1 - c;
+            ++s->steps_done;
+            const int e = err_code(s->h_ctl->err);
+            if (e) return e;
+        }
+    } else {
+        if (k < nsteps && !b->gexec[0]) {
+            b->captured = s->tune;
+            b->captured_p = s->P;
+            const int cur = s->cur;
+            int rc = capture_step(s, 0, &b->gexec[0]);
+            if (!rc) rc = capture_step(s, 1, &b->gexec[1]);
+            s->cur = cur;
+            if (rc) return rc;
+        }
+        for (; k < nsteps; ++k) {
+            HIPCHK(hipGraphLaunch(b->gexec[s->cur], s->st));
+            s->cur = 1 - s->cur;
+            ++s->steps_done;
+        }
     }
     int rc = read_ctl(s);
     if (rc) return rc;
