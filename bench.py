@@ -219,9 +219,12 @@ def copy_calibration(nbytes=1 << 28):
 def run_batch(args):
     """256 uneven dam breaks (fluid counts log-uniform in [2048, 65536], block
     aspect ratios in [0.5, 2], own box and walls each): the batched form
-    (one launch per stage for all scenes, each step one hipGraph replay)
-    against the plain path (one simulation per scene, upf_step per scene),
-    same scenes, same number of steps."""
+    (one launch per stage for all scenes) with every step one whole-loop
+    hipGraph replay, with split graphs (head, chunks of iterations, tail;
+    the loop decision read back between them) and with the host loop, each
+    from the same initial state over the same steps (same iteration counts,
+    asserted), against the plain path (one simulation per scene, upf_step
+    per scene), same scenes, same steps."""
     import unifiedparticleframeworkcuda_amd as upf
     hip = C.CDLL("libamdhip64.so")
     scenes = upf.batch_dam_breaks(args.batch_scenes, seed=0)
@@ -229,33 +232,31 @@ def run_batch(args):
     n = sum(len(sc[2]) for sc in scenes)
     K = args.batch_steps
     out = dict(scenes=len(scenes), fluid=nf, particles=n, steps=K)
-    b = upf.Batch.from_scenes(scenes, max_record_steps=2 * K + 64)
+    allpos = np.concatenate([sc[1] for sc in scenes])
+    alltyp = np.concatenate([sc[2] for sc in scenes])
+    b = upf.Batch.from_scenes(scenes, max_record_steps=K + 64)
     if not args.no_tune:
         from unifiedparticleframeworkcuda_amd import autotune
-        allpos = np.concatenate([sc[1] for sc in scenes])
-        alltyp = np.concatenate([sc[2] for sc in scenes])
         bt, _ = autotune.tune_batch(b, allpos, alltyp, "batch_%d_scenes_seed0" % len(scenes),
                                     use_cache=not args.retune)
         out["tuning"] = bt
+    t = {}
+    Ks = rs = None
+    for name, mode in (("graph", 1), ("graph_split", 2), ("host_loop", 0)):
         b.upload(allpos, alltyp)
-    b.step(2, use_graph=True)  # eager first step (boundary volumes), graph capture + one replay
-    hip.hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    b.step(K, use_graph=True)
-    hip.hipDeviceSynchronize()
-    tg = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    b.step(K, use_graph=False)
-    hip.hipDeviceSynchronize()
-    th = time.perf_counter() - t0
-    b.step(2, use_graph=2)  # capture of the split graphs + one step
-    hip.hipDeviceSynchronize()
-    t0 = time.perf_counter()
-    b.step(K, use_graph=2)
-    hip.hipDeviceSynchronize()
-    ts = time.perf_counter() - t0
-    Ks, rs = b.stats()
+        b.step(2, use_graph=mode)  # eager first step (boundary volumes), graph capture + one replay
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        b.step(K, use_graph=mode)
+        hip.hipDeviceSynchronize()
+        t[name] = time.perf_counter() - t0
+        Km, rm = b.stats()
+        if Ks is None:
+            Ks, rs = Km, rm
+        elif not (np.array_equal(Km, Ks) and np.array_equal(rm, rs)):
+            raise RuntimeError("batched modes disagree on the loop statistics")
     b.close()
+    tg, ts, th = t["graph"], t["graph_split"], t["host_loop"]
     sims = []
     for p, pos, typ, _ in scenes:
         s = upf.Simulation(p, len(typ))
@@ -270,9 +271,17 @@ def run_batch(args):
     tp = time.perf_counter() - t0
     for s in sims:
         s.close()
+    # the byte model summed over scenes, each at its own iteration count
+    KK = np.asarray(Ks[2:2 + K])
+    model = sum(step_bytes(len(sc[2]), sc[3], int(sc[0].grid_dim) ** 3, int(KK[:, j].sum()) / K) * K
+                for j, sc in enumerate(scenes))
+    best = min(tg, ts, th)
+    out["byte_model"] = {"bytes_per_step": model / K, "achieved": model / best / 1e9, "unit": "GB/s",
+                         "frac": model / best / HBM_PEAK, "mode": min(t, key=t.get)}
     out.update(graph_updates_per_s=nf * K / tg, graph_split_updates_per_s=nf * K / ts,
                host_loop_updates_per_s=nf * K / th, plain_updates_per_s=nf * K / tp,
-               graph_ms_per_step=tg / K * 1e3, graph_split_ms_per_step=ts / K * 1e3, plain_ms_per_step=tp / K * 1e3,
+               graph_ms_per_step=tg / K * 1e3, graph_split_ms_per_step=ts / K * 1e3,
+               host_loop_ms_per_step=th / K * 1e3, plain_ms_per_step=tp / K * 1e3,
                speedup_graph_vs_plain=tp / tg, speedup_graph_split_vs_plain=tp / ts,
                iterations_mean=float(np.mean(Ks[2:2 + K])),
                iterations_max=int(np.max(Ks[2:2 + K])),

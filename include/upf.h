@@ -520,6 +520,9 @@ typedef struct upf_tuning {
                          at most this many past the minimum (0 .. 64)                  */
     int xcd_remap;    /* 1: each XCD (own L2) takes one contiguous eighth of the blocks
                          of the list build and the grid-stride passes; 0: round robin   */
+    int nbr_skip;     /* list build: 1 skips, per particle, a round of 64 candidates none
+                         of whose cells lies in its query box (pays where cells are much
+                         wider than h); 0 tests every round                            */
 } upf_tuning;
 int upf_set_tuning(upf_sim* s, const upf_tuning* t);
 int upf_get_tuning(const upf_sim* s, upf_tuning* t);

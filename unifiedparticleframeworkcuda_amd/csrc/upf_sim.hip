@@ -102,6 +102,19 @@ __device__ inline int scene_at(const StepArgs& A, int i) { return A.scene_of ? A
 __device__ inline Dom dom_at(const StepArgs& A, int i) { return A.scene_of ? A.doms[A.scene_of[i]] : A.dom0; }
 __device__ inline int gstride() { return gridDim.x * blockDim.x; }
 
+// Batched form, iteration t: true when the block's slots [i0, i0 + blockDim.x)
+// all belong to one scene whose loop stopped before t (a scene's slots are
+// contiguous), so that late iterations pass over converged scenes without
+// reading their slots.  A scene's count of iterations only grows, and the
+// loop test of iteration t leaves it above t if it was, so a read racing with
+// that test decides the same way.
+__device__ inline bool block_idle(const StepArgs& A, int i0, int t) {
+    if (!A.scene_of || i0 >= A.n) return false;
+    const int s0 = A.scene_of[i0];
+    const int s1 = A.scene_of[min(i0 + (int)blockDim.x, A.n) - 1];
+    return s0 == s1 && t >= A.sctl[s0].want;
+}
+
 // Workgroups go to the 8 XCDs round-robin, each XCD with its own L2.  With the
 // grid a multiple of 8, remap gives every XCD one contiguous eighth of the
 // block range, so that blocks of neighbouring slots -- which gather the same
@@ -587,7 +600,9 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // lists the occupied cells of the 3x3x3 stencil in Morton order (so that the
 // candidates come in ascending slot order), then tests the candidates 64 at
 // a time against each particle of the cell: the candidate's cell must lie in
-// the particle's query box (the reference's BoxQuery) and its distance must be
+// the particle's query box (the reference's BoxQuery; decided once per cell
+// and particle for the occupied stencil cells; SKIP: a round of 64 candidates
+// none of whose cells lies in the box is passed over) and its distance must be
 // below h.  Ballots keep each list in ascending order (BIN: entries carry
 // the pair's table bin, else only the slot).  ROUNDS x 64
 // candidates are held in registers per chunk.  The lists of the wave's 64
@@ -595,14 +610,12 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
 // which keeps the LDS free for more waves).
-template <int ROUNDS, int STAGE, int BIN>
+template <int ROUNDS, int STAGE, int BIN, int SKIP = 0>
 __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                                   uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
     constexpr int NB_CHUNK = 64 * ROUNDS;
-    // packed cell coordinates: 10-bit fields whose top bit (coordinates are
-    // below 512) serves as the borrow guard of a packed compare
-    constexpr uint32_t GUARD = (1u << 9) | (1u << 19) | (1u << 29);
+    // cell coordinates and query boxes are packed in 10-bit fields (x, y, z)
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
     __shared__ uint32_t stage[(STAGE > 0 ? STAGE : 1) * 65];
     const upf_params& P = A.P;
@@ -625,7 +638,7 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
     const uint32_t my_key = mine_valid ? keys[me] : 0xFFFFFFFFu;
     const int my_scene = mine_valid ? scene_at(A, me) : -1;
     const bool my_liquid = mine_valid && ptype(my_x) == UPF_LIQUID;
-    uint32_t my_blo = 0u, my_bhi = GUARD;
+    uint32_t my_blo = 0u, my_bhi = 0u;
     if (my_liquid) {
         // BoxQuery of the particle in cell coordinates, clipped to the grid
         const Dom dm = A.scene_of ? A.doms[my_scene] : A.dom0;
@@ -660,6 +673,10 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
         // the cell's liquid slots (bits sl .. pe - base - 1)
         const uint64_t upto = (pe - base) >= 64 ? ~0ull : ((1ull << (pe - base)) - 1ull);
         const uint64_t cell_liquid = liquid & upto & ~((1ull << sl) - 1ull);
+        if (cell_liquid == 0ull) {  // no list to build in this cell (walls, rigid particles)
+            s = pe;
+            continue;
+        }
         const uint32_t cc[3] = {compact10(c), compact10(c >> 1), compact10(c >> 2)};
         // stencil cells in Morton order: lane e names enumeration index e
         BoxCells bc;
@@ -701,24 +718,54 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
         if (lane == 0) r_pref[0] = 0;
         __syncthreads();
         const int M = (int)r_pref[nr];
-        for (int c0 = 0; c0 < M && cell_liquid; c0 += NB_CHUNK) {
+        // Which occupied stencil cells (bit k: rank k) lie in the query box of
+        // this lane's particle -- the box test of every candidate of the cell,
+        // done once per cell: lane k < nr holds rank k's cell; per axis, the
+        // ranks one below, at and one above the cell's own coordinate, and the
+        // lane's box (blo .. bhi per field) keeps those of each class it meets.
+        uint32_t in_box;
+        {
+            const uint32_t qk = lane < nr ? r_cell[lane] : 0u;
+            in_box = (1u << nr) - 1u;  // nr <= 27
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const uint32_t qa = (qk >> (10 * a)) & 0x3FFu;
+                const uint32_t below = (uint32_t)__ballot(lane < nr && qa < cc[a]);
+                const uint32_t above = (uint32_t)__ballot(lane < nr && qa > cc[a]);
+                const uint32_t at = (uint32_t)__ballot(lane < nr && qa == cc[a]);
+                const uint32_t l = (my_blo >> (10 * a)) & 0x3FFu, u = (my_bhi >> (10 * a)) & 0x3FFu;
+                // stencil coordinates are cc - 1, cc, cc + 1 (cc - 1 only when cc > 0)
+                if (!(cc[a] > 0 && l <= cc[a] - 1 && cc[a] - 1 <= u)) in_box &= ~below;
+                if (!(l <= cc[a] && cc[a] <= u)) in_box &= ~at;
+                if (!(l <= cc[a] + 1 && cc[a] + 1 <= u)) in_box &= ~above;
+            }
+        }
+        for (int c0 = 0; c0 < M; c0 += NB_CHUNK) {
             // this lane's candidates of the chunk; a lane past the end holds
-            // an infinitely distant candidate, which fails the distance test
+            // an infinitely distant candidate in no cell, which fails both tests
             float4 cx[ROUNDS];
             int cj[ROUNDS];
-            uint32_t ccell[ROUNDS];
+            uint32_t cbit[ROUNDS];  // 1 << rank of the candidate's cell
+            uint32_t rbits[ROUNDS]; // SKIP: ranks met by the round's candidates (uniform)
             int k = 0;  // range of the lane's candidate; rises with the round
 #pragma unroll
             for (int r = 0; r < ROUNDS; ++r) {
                 const int ci = c0 + r * 64 + lane;
                 cj[r] = -1;
-                ccell[r] = 0u;
+                cbit[r] = 0u;
                 cx[r] = make_float4(__int_as_float(0x7f800000), 0.f, 0.f, 0.f);
                 if (ci < M) {
                     while (k + 1 < nr && (int)r_pref[k + 1] <= ci) ++k;
                     cj[r] = (int)r_start[k] + (ci - (int)r_pref[k]);
-                    ccell[r] = r_cell[k];
+                    cbit[r] = 1u << k;
                     cx[r] = pos[cj[r]];
+                }
+                if constexpr (SKIP != 0) {
+                    // the round's ranks run from its first lane's to its last lane's
+                    const int last = min(63, M - 1 - (c0 + r * 64));
+                    const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)cbit[r], 0);
+                    const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)cbit[r], last < 0 ? 0 : last);
+                    rbits[r] = last < 0 ? 0u : (b1 << 1) - b0;
                 }
             }
             const int rounds = min(ROUNDS, (M - c0 + 63) >> 6);
@@ -730,15 +777,14 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                 const float px = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_x.x), pl));
                 const float py = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_x.y), pl));
                 const float pz = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(my_x.z), pl));
-                const uint32_t blo = (uint32_t)__builtin_amdgcn_readlane((int)my_blo, pl);
-                const uint32_t bhi_g = (uint32_t)__builtin_amdgcn_readlane((int)my_bhi, pl);
+                const uint32_t boxm = (uint32_t)__builtin_amdgcn_readlane((int)in_box, pl);
                 int cnt = __builtin_amdgcn_readlane(my_cnt, pl);
 #pragma unroll
                 for (int r = 0; r < ROUNDS; ++r) {
                     if (r >= rounds) break;
-                    // blo <= cell <= bhi in every field: no field borrows
-                    const uint32_t q = ccell[r];
-                    const bool inbox = (((q | GUARD) - blo) & (bhi_g - q) & GUARD) == GUARD;
+                    if constexpr (SKIP != 0)
+                        if ((boxm & rbits[r]) == 0u) continue;  // no candidate of the round in the box
+                    const bool inbox = (boxm & cbit[r]) != 0u;
                     const float dx = px - cx[r].x, dy = py - cx[r].y, dz = pz - cx[r].z;
                     const float d2 = dx * dx + dy * dy + dz * dz;
                     const bool pass = inbox & (d2 < T) & (cj[r] != p);
@@ -1149,14 +1195,18 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     const int n_round = (A.n + gstride() - 1) / gstride();
     for (int r = 0; r < n_round; ++r) {
         const int i = r * gstride() + xblk(A.xcd) * blockDim.x + threadIdx.x;
+        if (block_idle(A, i - (int)threadIdx.x, t)) continue;
         int un = -1;
         if constexpr (CODED == 2) un = stage_union(A, W, W.xs, urec, (i - (int)threadIdx.x) / UNION_CHUNK);
         float mine = 0.0f;
         int sc = -1;
         if (i < A.n) {
-            const float4 xi = W.xs[i];
             sc = scene_at(A, i);
-            if (ptype(xi) == UPF_LIQUID && t < A.sctl[sc].want) {
+            // batched: a converged scene's slot stops before its record is
+            // read (one scene runs iteration t: the kernel's entry test decided it)
+            const bool runs = !A.scene_of || t < A.sctl[sc].want;
+            const float4 xi = runs ? W.xs[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (runs && ptype(xi) == UPF_LIQUID) {
                 const float m = P.initial_mass;
                 const float T = P.support_radius_sq_limit;
                 float rho = P.kernel_self * m;
@@ -1343,13 +1393,15 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     const int n_round = (A.n + gstride() - 1) / gstride();
     for (int r = 0; r < n_round; ++r) {
     const int i = r * gstride() + xblk(A.xcd) * blockDim.x + threadIdx.x;
+    if (block_idle(A, i - (int)threadIdx.x, t)) continue;
     int un = -1;
     if constexpr (CODED == 2) un = stage_union(A, W, W.xpd, urec, (i - (int)threadIdx.x) / UNION_CHUNK);
     if (i >= A.n) continue;
+    // batched: a converged scene's slot stops before its record is read (one
+    // scene runs iteration t: the kernel's entry test decided it)
+    if (A.scene_of && t >= A.sctl[A.scene_of[i]].want) continue;
     const float4 xi = S.pos[i];
     if (ptype(xi) != UPF_LIQUID) continue;
-    const int sc = scene_at(A, i);
-    if (t >= A.sctl[sc].want) continue;
     const float m = P.initial_mass;
     const float T = P.support_radius_sq_limit;
     const float rho0 = P.rest_density;
@@ -2126,7 +2178,7 @@ struct upf_sim {
     float* d_rstat = nullptr;
     int stat_cap = 0;
     int steps_done = 0;
-    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4, 1};
+    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4, 1, 0};
     bool static_lists = false;  // the lists being built serve the static PCISPH step only
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
@@ -2365,7 +2417,11 @@ static void build_lists(upf_sim* s, const StepArgs& A) {
     } else if (s->tune.nbr_tile == 512) {
         launch(k_nbr_build_tile<512>);
     } else if (s->tune.nbr_stage == 0) {
-        if (!bins) {
+        if (!bins && s->tune.nbr_skip) {
+            if (r == 2) launch(k_nbr_build<2, 0, 0, 1>);
+            else if (r == 8) launch(k_nbr_build<8, 0, 0, 1>);
+            else launch(k_nbr_build<4, 0, 0, 1>);
+        } else if (!bins) {
             if (r == 2) launch(k_nbr_build<2, 0, 0>);
             else if (r == 8) launch(k_nbr_build<8, 0, 0>);
             else launch(k_nbr_build<4, 0, 0>);
@@ -2834,7 +2890,7 @@ int upf_set_tuning(upf_sim* s, const upf_tuning* t) {
         (t->loop_list < 0 || t->loop_list > 2) || (t->loop_list == 2 && t->loop_block != UNION_CHUNK) ||
         (t->loop_width != 4 && t->loop_width != 8) ||
         (t->nbr_tile != 0 && t->nbr_tile != 256 && t->nbr_tile != 512) || t->spec_first < 0 || t->spec_first > 64 ||
-        (t->xcd_remap != 0 && t->xcd_remap != 1))
+        (t->xcd_remap != 0 && t->xcd_remap != 1) || (t->nbr_skip != 0 && t->nbr_skip != 1))
         return UPF_ERR_ARG;
     s->tune = *t;
     return UPF_OK;
