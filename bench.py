@@ -47,6 +47,7 @@ def class_bytes(name, n, nf, ncells):
         "cell_table": 4 * n + 8 * ncells,
         "neighbour_lists": 20 * n + 4 * n,      # positions + keys read, counts written
         "external_force_predict": 40 * n + 36 * nf + 80 * nf,  # + the fused first prediction
+        "neighbour_lists_external_force": 24 * n + 40 * n + 36 * nf + 80 * nf,  # both in one kernel
         "predicted_density": 16 * n + 12 * nf,
         "pressure_force_predict": 20 * n + 16 * nf + 80 * nf,  # + the fused next prediction
         "integrate": 96 * nf,
@@ -57,6 +58,7 @@ def class_bytes(name, n, nf, ncells):
 # kernel symbol prefix of each profile class (for the committed PMC summary)
 CLASS_KERNEL = {"pressure_force_predict": "upf::k_force<", "predicted_density": "upf::k_density<",
                 "neighbour_lists": "upf::k_nbr_build<", "external_force_predict": "upf::k_ext_static<",
+                "neighbour_lists_external_force": "upf::k_nbr_build_ext<",
                 "integrate": "upf::k_integrate", "gather": "upf::k_gather", "radix_sort": "upf::rs_scatter"}
 PROFILE = os.path.join(ROOT, "profiles", "pcisph_dam_break_1m_kernels.md")
 
@@ -167,21 +169,38 @@ def run_gpu(args, workload=None, steps=None, profile=True):
     # recorded around every launch: per-kernel-class device time.  Here the
     # host enqueues no iteration ahead of the loop decision, so that every
     # launch of the iteration passes does work (launch shapes unchanged).
+    # The kernels of a step run back to back, so their device times should
+    # add up to the timed run's step time; a replay whose event times fall
+    # short of it by more than 1.5 % (seen now and then) is run again, up to
+    # three times, and the replay with the largest sum is kept (the most
+    # conservative per-kernel times).
     timed_tuning = sim.tuning().as_dict()
     sim.set_tuning(dict(timed_tuning, speculate=1, spec_first=0))
-    sim.upload(pos, typ)
-    sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
-    L.upf_profile_enable(sim.h, 1)
-    L.upf_profile_reset(sim.h)
-    sim.step(steps, upf.UPF_PCISPH_STATIC)
-    hip.hipDeviceSynchronize()
-    Kp, _ = sim.stats()
-    prof = upf.profile(sim.h)
-    L.upf_profile_enable(sim.h, 0)
-    assert np.array_equal(Kp[-steps:], Kt), "the profiled replay took other iteration counts"
+    best = None
+    for attempt in range(1, 4):
+        sim.upload(pos, typ)
+        sim.step(args.warmup, upf.UPF_PCISPH_STATIC)
+        L.upf_profile_enable(sim.h, 1)
+        L.upf_profile_reset(sim.h)
+        hip.hipDeviceSynchronize()
+        t0 = time.perf_counter()
+        sim.step(steps, upf.UPF_PCISPH_STATIC)
+        hip.hipDeviceSynchronize()
+        replay_s = time.perf_counter() - t0
+        Kp, _ = sim.stats()
+        prof = upf.profile(sim.h)
+        L.upf_profile_enable(sim.h, 0)
+        assert np.array_equal(Kp[-steps:], Kt), "the profiled replay took other iteration counts"
+        total = sum(v[0] for v in prof.values()) * 1e-3
+        if best is None or total > best[0]:
+            best = (total, prof, replay_s, attempt)
+        if total >= 0.985 * dt:
+            break
+    _, prof, replay_s, _ = best
     sim.set_tuning(timed_tuning)
     sim.close()
-    out = dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof, tuning=tuning)
+    out = dict(p=p, n=n, nf=nf, ncells=ncells, seconds=dt, K=Kt, rmax=rt, prof=prof, tuning=tuning,
+               replay_seconds=replay_s, replay_attempts=attempt)
     if not args.no_graph:
         # the same scene as a batch of one: every step one hipGraph replay
         b = upf.Batch([p], [n], max_record_steps=args.warmup + steps + 8)
@@ -619,6 +638,8 @@ def main():
         "step_roofline": {"achieved": total_bytes / sec / 1e9, "unit": "GB/s", "frac": total_bytes / sec / HBM_PEAK,
                           "bytes_per_update": total_bytes / (nf * args.steps)},
         "kernel_ms_per_step": {k: v[0] / args.steps for k, v in g["prof"].items()},
+        "replay_ms_per_step": g["replay_seconds"] / args.steps * 1e3,
+        "replay_attempts": g["replay_attempts"],
         "tuning": g["tuning"],
         "graph_replay": (None if "graph_seconds" not in g else
                          {"ms_per_step": g["graph_seconds"] / args.steps * 1e3,

@@ -523,6 +523,9 @@ typedef struct upf_tuning {
     int nbr_skip;     /* list build: 1 skips, per particle, a round of 64 candidates none
                          of whose cells lies in its query box (pays where cells are much
                          wider than h); 0 tests every round                            */
+    int nbr_fuse;     /* 1: the list build and the external-force pass run as one kernel
+                         (cell-by-cell build without LDS staging, delta-coded loop lists);
+                         0: two kernels                                               */
 } upf_tuning;
 int upf_set_tuning(upf_sim* s, const upf_tuning* t);
 int upf_get_tuning(const upf_sim* s, upf_tuning* t);

@@ -12,7 +12,7 @@ from unifiedparticleframeworkcuda_amd._abi import Tuning
 
 def test_coordinate_search_finds_separable_optimum():
     target = dict(loop_block=128, grid_cap=4096, nbr_rounds=8, nbr_stage=32, speculate=2, loop_list=0, loop_width=8,
-                  nbr_tile=256, spec_first=8, xcd_remap=0, nbr_skip=1)
+                  nbr_tile=256, spec_first=8, xcd_remap=0, nbr_skip=1, nbr_fuse=0)
     calls = []
 
     def cost(t):
@@ -39,7 +39,7 @@ def test_cache_round_trip(tmp_path):
 def test_tuning_struct_round_trip():
     t = Tuning.from_dict(autotune.DEFAULT)
     assert t.as_dict() == autotune.DEFAULT
-    assert C.sizeof(Tuning) == 11 * 4
+    assert C.sizeof(Tuning) == 12 * 4
 
 
 def test_set_tuning_rejects_unsupported_shapes_without_device():

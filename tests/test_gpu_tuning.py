@@ -34,19 +34,19 @@ def _run(p, pos, typ, tuning, steps=5):
 
 @pytest.mark.parametrize("tuning", [
     dict(loop_block=64, grid_cap=16, nbr_rounds=2, nbr_stage=32, speculate=1, loop_list=0, loop_width=4, nbr_tile=0,
-         spec_first=0, xcd_remap=0, nbr_skip=1),
+         spec_first=0, xcd_remap=0, nbr_skip=1, nbr_fuse=1),
     dict(loop_block=128, grid_cap=2048, nbr_rounds=8, nbr_stage=32, speculate=6, loop_list=1, loop_width=8,
-         nbr_tile=256, spec_first=64, xcd_remap=1, nbr_skip=0),
+         nbr_tile=256, spec_first=64, xcd_remap=1, nbr_skip=0, nbr_fuse=1),
     dict(loop_block=256, grid_cap=3, nbr_rounds=4, nbr_stage=64, speculate=2, loop_list=0, loop_width=8, nbr_tile=512,
-         spec_first=2, xcd_remap=1, nbr_skip=1),
+         spec_first=2, xcd_remap=1, nbr_skip=1, nbr_fuse=0),
     dict(loop_block=256, grid_cap=8192, nbr_rounds=8, nbr_stage=0, speculate=4, loop_list=1, loop_width=4, nbr_tile=0,
-         spec_first=4, xcd_remap=0, nbr_skip=1),
+         spec_first=4, xcd_remap=0, nbr_skip=1, nbr_fuse=0),
     # union lists: neighbour records from LDS (chunks whose union is too large
     # for it fall back to the delta-coded lists); a small grid re-stages per chunk
     dict(loop_block=256, grid_cap=8192, nbr_rounds=4, nbr_stage=0, speculate=4, loop_list=2, loop_width=8, nbr_tile=0,
-         spec_first=4, xcd_remap=1, nbr_skip=0),
+         spec_first=4, xcd_remap=1, nbr_skip=0, nbr_fuse=1),
     dict(loop_block=256, grid_cap=3, nbr_rounds=4, nbr_stage=0, speculate=1, loop_list=2, loop_width=4, nbr_tile=0,
-         spec_first=0, xcd_remap=0, nbr_skip=0),
+         spec_first=0, xcd_remap=0, nbr_skip=0, nbr_fuse=1),
 ])
 def test_tunings_give_identical_steps(gpu, tuning):
     p, pos, typ = _compressed_scene()
@@ -68,8 +68,8 @@ def test_round_skip_on_wide_cells_equals_the_oracle(gpu):
     ratio = (1 << p.block_shift) / p.scales[0] / p.support_radius
     print("wide cells: %d particles, cell side %.2f h" % (len(typ), ratio), flush=True)
     assert ratio > 1.5
-    runs = [_run(p, pos, typ, dict(autotune.DEFAULT, nbr_rounds=r, nbr_skip=k), steps=3)
-            for r in (2, 8) for k in (0, 1)]
+    runs = [_run(p, pos, typ, dict(autotune.DEFAULT, nbr_rounds=r, nbr_skip=k, nbr_fuse=f), steps=3)
+            for r in (2, 8) for k in (0, 1) for f in (0, 1)]
     o = ol.OracleSim(p, pos, None, typ)
     o.step(3, upf.UPF_PCISPH_STATIC)
     r = o.state()
@@ -86,7 +86,8 @@ def test_invalid_tuning_rejected(gpu):
               dict(autotune.DEFAULT, nbr_stage=16), dict(autotune.DEFAULT, grid_cap=0),
               dict(autotune.DEFAULT, loop_list=3), dict(autotune.DEFAULT, loop_list=2, loop_block=128),
               dict(autotune.DEFAULT, loop_width=6),
-              dict(autotune.DEFAULT, nbr_tile=128), dict(autotune.DEFAULT, nbr_skip=2)):
+              dict(autotune.DEFAULT, nbr_tile=128), dict(autotune.DEFAULT, nbr_skip=2),
+              dict(autotune.DEFAULT, nbr_fuse=2)):
         with pytest.raises(upf.UPFError):
             sim.set_tuning(d)
     assert sim.tuning().as_dict() == autotune.DEFAULT

@@ -148,7 +148,8 @@ class Tuning(C.Structure):
     """upf_tuning: launch shapes that change speed only."""
     _fields_ = [("loop_block", C.c_int), ("grid_cap", C.c_int), ("nbr_rounds", C.c_int),
                 ("nbr_stage", C.c_int), ("speculate", C.c_int), ("loop_list", C.c_int), ("loop_width", C.c_int),
-                ("nbr_tile", C.c_int), ("spec_first", C.c_int), ("xcd_remap", C.c_int), ("nbr_skip", C.c_int)]
+                ("nbr_tile", C.c_int), ("spec_first", C.c_int), ("xcd_remap", C.c_int), ("nbr_skip", C.c_int),
+                ("nbr_fuse", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -16,11 +16,11 @@ from . import UPF_PCISPH_STATIC, UPFError
 from ._abi import Tuning
 
 DEFAULT = dict(loop_block=256, grid_cap=8192, nbr_rounds=4, nbr_stage=0, speculate=4, loop_list=1, loop_width=8,
-               nbr_tile=0, spec_first=4, xcd_remap=1, nbr_skip=0)
+               nbr_tile=0, spec_first=4, xcd_remap=1, nbr_skip=0, nbr_fuse=1)
 CANDIDATES = dict(loop_block=(64, 128, 256), grid_cap=(2048, 4096, 8192, 16384), nbr_rounds=(2, 4, 8),
                   nbr_stage=(0, 32, 64), speculate=(1, 2, 4, 8), loop_list=(0, 1, 2),
                   loop_width=(4, 8), nbr_tile=(0, 256, 512), spec_first=(0, 2, 4, 8, 64),
-                  xcd_remap=(0, 1), nbr_skip=(0, 1))
+                  xcd_remap=(0, 1), nbr_skip=(0, 1), nbr_fuse=(0, 1))
 
 
 def cache_path():

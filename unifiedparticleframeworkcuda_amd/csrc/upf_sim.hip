@@ -610,10 +610,10 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
 // which keeps the LDS free for more waves).
-template <int ROUNDS, int STAGE, int BIN, int SKIP = 0>
-__global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
-                                                  const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
-                                                  uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
+template <int ROUNDS, int STAGE, int BIN, int SKIP>
+__device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __restrict__ pos,
+                                           const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
+                                           uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
     constexpr int NB_CHUNK = 64 * ROUNDS;
     // cell coordinates and query boxes are packed in 10-bit fields (x, y, z)
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
@@ -826,6 +826,13 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
         for (int k = 0; k < rows; ++k)
             if (k < mine) nbr[(size_t)k * stride + me] = stage[k * 65 + lane];
     }
+}
+
+template <int ROUNDS, int STAGE, int BIN, int SKIP = 0>
+__global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
+                                                  const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
+                                                  uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
+    build_wave<ROUNDS, STAGE, BIN, SKIP>(A, pos, keys, cells, nbr, ncount, ctl);
 }
 
 // GetNeighbors again, tile form: one wave per 64 consecutive slots of one
@@ -1042,19 +1049,15 @@ __device__ __forceinline__ float4 boundary_xs(float4 x, float psi, float rho0) {
     return x;
 }
 
+// The external-force work of slot i (one thread), shared by the
+// external-force pass and the fused list build: per-step records of the slot,
+// and for a liquid slot the forces over its list, the first predicted
+// position and the delta-coded copy of the list.  lvl, lpg, th: the
+// viscosity and pressure-gradient tables and the bin thresholds in LDS.
 template <int MODE>
-__global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
-                                                      Ctl* ctl) {
-    __shared__ float lvl[LUT_N], lpg[LUT_N];
-    __shared__ float2 th[LUT_N];
-    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
-        lvl[t] = luts[2 * LUT_N + t];
-        lpg[t] = luts[LUT_N + t];
-        th[t] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[t];
-    }
-    __syncthreads();
+__device__ __forceinline__ void ext_static_slot(const StepArgs& A, const Slots& S, const Work& W, int i,
+                                                const float* lvl, const float* lpg, const float2* th, Ctl* ctl) {
     const upf_params& P = A.P;
-    for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride()) {
     const float4 xi = S.pos[i];
     const int ti = ptype(xi);
     W.p[i] = 0.0f;
@@ -1065,7 +1068,7 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
         W.xs[i] = boundary_xs<MODE>((ti == UPF_RIGID && !P.rigid_predicted_current) ? S.x0[i] : xi, S.psi[i],
                                     P.rest_density);
         W.force[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        continue;
+        return;
     }
     const float4 vi = S.vel[i];
     const float h = P.support_radius;
@@ -1147,7 +1150,45 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
     const float4 F = make_float4(fx, fy, fz, 0.0f);
     W.force[i] = F;
     W.xs[i] = predict(xi, vi, F, make_float4(0.f, 0.f, 0.f, 0.f), A, d, ctl);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                      Ctl* ctl) {
+    __shared__ float lvl[LUT_N], lpg[LUT_N];
+    __shared__ float2 th[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
+        lvl[t] = luts[2 * LUT_N + t];
+        lpg[t] = luts[LUT_N + t];
+        th[t] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[t];
     }
+    __syncthreads();
+    for (int i = xblk(A.xcd) * blockDim.x + threadIdx.x; i < A.n; i += gstride())
+        ext_static_slot<MODE>(A, S, W, i, lvl, lpg, th, ctl);
+}
+
+// The list build and the external-force pass in one kernel: a wave builds
+// the lists of its 64 slots (k_nbr_build, packed entries without bins), then
+// each lane runs the external-force work of its own slot over the list the
+// wave just wrote.  Nothing crosses waves between the two parts, so the
+// memory-bound force work of some waves overlaps the VALU-bound list work of
+// others, and the lists are read back while they are still in the cache.
+template <int ROUNDS, int SKIP, int MODE>
+__global__ __launch_bounds__(64) void k_nbr_build_ext(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                      Ctl* ctl) {
+    __shared__ float lvl[LUT_N], lpg[LUT_N];
+    __shared__ float2 th[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
+        lvl[t] = luts[2 * LUT_N + t];
+        lpg[t] = luts[LUT_N + t];
+        th[t] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[t];
+    }
+    build_wave<ROUNDS, 0, 0, SKIP>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl);
+    // the block is the wave: its lists (written by all its lanes) and the
+    // tables are complete and visible to every lane after the barrier
+    __syncthreads();
+    const int i = xblk(A.xcd) * 64 + (int)threadIdx.x;
+    if (i < A.n) ext_static_slot<MODE>(A, S, W, i, lvl, lpg, th, ctl);
 }
 
 // ------------------------------------------------------ PCISPH iteration
@@ -2078,11 +2119,11 @@ using namespace upf;
     } while (0)
 
 enum ProfClass { PC_SORT = 0, PC_GATHER, PC_CELLS, PC_PSI, PC_NBR_BUILD, PC_NBR_EXT, PC_DENSITY, PC_FORCE,
-                 PC_INTEGRATE, PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_UNION, PC_COUNT };
+                 PC_INTEGRATE, PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_UNION, PC_NBR_BUILD_EXT, PC_COUNT };
 static const char* kProfNames[PC_COUNT] = {"radix_sort", "gather", "cell_table", "boundary_volume",
                                           "neighbour_lists", "external_force_predict", "predicted_density",
                                           "pressure_force_predict", "integrate", "wcsph_density", "wcsph_force",
-                                          "rigid_bodies", "neighbour_union"};
+                                          "rigid_bodies", "neighbour_union", "neighbour_lists_external_force"};
 
 struct Prof {
     bool on = false;
@@ -2178,7 +2219,7 @@ struct upf_sim {
     float* d_rstat = nullptr;
     int stat_cap = 0;
     int steps_done = 0;
-    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4, 1, 0};
+    upf_tuning tune = {256, 8192, 4, 0, 4, 1, 8, 0, 4, 1, 0, 1};
     bool static_lists = false;  // the lists being built serve the static PCISPH step only
     float* luts = nullptr;  // 5 * 301
     void* sort_tmp = nullptr;
@@ -2461,15 +2502,34 @@ static int step_static_head(upf_sim* s) {
         s->prof.end(s->st);
         s->psi_ready = true;
     }
-    s->static_lists = true;
-    build_lists(s, A);
-    s->static_lists = false;
-    s->prof.begin(PC_NBR_EXT, s->st);
-    if (akinci)
-        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
-    else
-        hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
-    s->prof.end(s->st);
+    // the fused list build + external forces needs the cell-by-cell build
+    // without LDS staging and the delta-coded loop lists (no bins)
+    if (s->tune.nbr_fuse && s->tune.nbr_stage == 0 && s->tune.nbr_tile == 0 && A.code_lists) {
+        s->prof.begin(PC_NBR_BUILD_EXT, s->st);
+        const dim3 gb((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
+        auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, s->st, A, S, s->W, s->luts, s->ctl); };
+        const int r = s->tune.nbr_rounds, k = s->tune.nbr_skip;
+        if (akinci) {
+            if (r == 2) k ? launch(k_nbr_build_ext<2, 1, UPF_BOUNDARY_AKINCI>) : launch(k_nbr_build_ext<2, 0, UPF_BOUNDARY_AKINCI>);
+            else if (r == 8) k ? launch(k_nbr_build_ext<8, 1, UPF_BOUNDARY_AKINCI>) : launch(k_nbr_build_ext<8, 0, UPF_BOUNDARY_AKINCI>);
+            else k ? launch(k_nbr_build_ext<4, 1, UPF_BOUNDARY_AKINCI>) : launch(k_nbr_build_ext<4, 0, UPF_BOUNDARY_AKINCI>);
+        } else {
+            if (r == 2) k ? launch(k_nbr_build_ext<2, 1, UPF_BOUNDARY_CPU_REF>) : launch(k_nbr_build_ext<2, 0, UPF_BOUNDARY_CPU_REF>);
+            else if (r == 8) k ? launch(k_nbr_build_ext<8, 1, UPF_BOUNDARY_CPU_REF>) : launch(k_nbr_build_ext<8, 0, UPF_BOUNDARY_CPU_REF>);
+            else k ? launch(k_nbr_build_ext<4, 1, UPF_BOUNDARY_CPU_REF>) : launch(k_nbr_build_ext<4, 0, UPF_BOUNDARY_CPU_REF>);
+        }
+        s->prof.end(s->st);
+    } else {
+        s->static_lists = true;
+        build_lists(s, A);
+        s->static_lists = false;
+        s->prof.begin(PC_NBR_EXT, s->st);
+        if (akinci)
+            hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_AKINCI>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
+        else
+            hipLaunchKernelGGL(k_ext_static<UPF_BOUNDARY_CPU_REF>, gp, bl, 0, s->st, A, S, s->W, s->luts, s->ctl);
+        s->prof.end(s->st);
+    }
     s->union_ready = s->tune.loop_list == 2;
     if (s->union_ready) {
         s->prof.begin(PC_UNION, s->st);
@@ -2890,7 +2950,8 @@ int upf_set_tuning(upf_sim* s, const upf_tuning* t) {
         (t->loop_list < 0 || t->loop_list > 2) || (t->loop_list == 2 && t->loop_block != UNION_CHUNK) ||
         (t->loop_width != 4 && t->loop_width != 8) ||
         (t->nbr_tile != 0 && t->nbr_tile != 256 && t->nbr_tile != 512) || t->spec_first < 0 || t->spec_first > 64 ||
-        (t->xcd_remap != 0 && t->xcd_remap != 1) || (t->nbr_skip != 0 && t->nbr_skip != 1))
+        (t->xcd_remap != 0 && t->xcd_remap != 1) || (t->nbr_skip != 0 && t->nbr_skip != 1) ||
+        (t->nbr_fuse != 0 && t->nbr_fuse != 1))
         return UPF_ERR_ARG;
     s->tune = *t;
     return UPF_OK;
