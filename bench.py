@@ -55,9 +55,10 @@ def class_bytes(name, n, nf, ncells):
     return table.get(name)
 
 
-# kernel symbol prefix of each profile class (for the committed PMC summary)
+# kernel symbol prefix of each profile class of the default path (for the
+# committed PMC summary; the separate list build and external-force kernels
+# of nbr_fuse 0 have no row there)
 CLASS_KERNEL = {"pressure_force_predict": "upf::k_force<", "predicted_density": "upf::k_density<",
-                "neighbour_lists": "upf::k_nbr_build<", "external_force_predict": "upf::k_ext_static<",
                 "neighbour_lists_external_force": "upf::k_nbr_build_ext<",
                 "integrate": "upf::k_integrate", "gather": "upf::k_gather", "radix_sort": "upf::rs_scatter"}
 PROFILE = os.path.join(ROOT, "profiles", "pcisph_dam_break_1m_kernels.md")
