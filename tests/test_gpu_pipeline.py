@@ -30,7 +30,8 @@ def _close(a, b, what, rtol=RTOL, atol=0.0):
 
 
 # ------------------------------------------------------------------ sort
-@pytest.mark.parametrize("n", [1, 7, 1000, 2048, 2049, 100000, 1 << 20])
+# tiles of 1024, 2048 and 4096 keys (the sort picks by n), with ragged last tiles
+@pytest.mark.parametrize("n", [1, 7, 1000, 2048, 2049, 100000, (1 << 20) - 1, 1 << 20, (1 << 21) + 12345])
 def test_radix_sort_matches_stable_argsort(gpu, n):
     rng = np.random.RandomState(n)
     keys = rng.randint(0, 1 << 30, size=n).astype(np.uint32)

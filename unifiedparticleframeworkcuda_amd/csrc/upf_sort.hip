@@ -4,7 +4,9 @@
 // RadixAddOffsetsAndShuffle in src/gpu/radixsort_kernel.cu:64,216,339) with a
 // design of its own:
 //   * 8-bit digits, one pass per byte of the key range that is in use;
-//   * tile = 256 threads x 4 items; each wave ranks 64 consecutive items per
+//   * tile = 256 threads x IPT items (4, 8 or 16: the largest that still
+//     gives at least 512 tiles, so that long sorts write longer runs per
+//     digit and every CU still gets tiles); each wave ranks 64 consecutive items per
 //     round with 8 ballots (lanes with equal digits form a peer mask), so the
 //     rank of an item is the number of equal digits before it -> stable;
 //   * per-tile digit counts (digit-major) are scanned by one workgroup per
@@ -18,12 +20,13 @@
 namespace upf {
 
 constexpr int RS_THREADS = 256;
-constexpr int RS_IPT = 4;
-constexpr int RS_TILE = RS_THREADS * RS_IPT;
+constexpr int RS_MIN_IPT = 4;  // the smallest tile (most tiles) of any sort
 constexpr int RS_WAVES = RS_THREADS / 64;
 
+template <int RS_IPT>
 __global__ __launch_bounds__(RS_THREADS) void rs_hist(const uint32_t* __restrict__ keys, int n, int shift,
                                                       uint32_t* __restrict__ hist, int ntiles) {
+    constexpr int RS_TILE = RS_THREADS * RS_IPT;
     __shared__ uint32_t cnt[256];
     cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -66,11 +69,13 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scan(uint32_t* __restrict__ his
     if (threadIdx.x == RS_THREADS - 1) totals[blockIdx.x] = part[RS_THREADS - 1];
 }
 
+template <int RS_IPT>
 __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint32_t* __restrict__ kin,
                                                          const uint32_t* __restrict__ vin,
                                                          uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                          int n, int shift, const uint32_t* __restrict__ hist,
                                                          int ntiles, const uint32_t* __restrict__ totals) {
+    constexpr int RS_TILE = RS_THREADS * RS_IPT;
     __shared__ uint32_t digit_base[256];  // global start of each digit, then of this tile's run of it
     __shared__ uint32_t local_start[256]; // start of each digit within the tile
     __shared__ uint32_t wcnt[RS_WAVES][256];
@@ -166,8 +171,17 @@ __global__ __launch_bounds__(RS_THREADS) void rs_scatter(const uint32_t* __restr
     }
 }
 
+// items per thread of a sort of n keys: the largest of 16, 8, 4 that leaves
+// at least 512 tiles
+static int sort_ipt(int n) {
+    if (n >= 512 * RS_THREADS * 16) return 16;
+    if (n >= 512 * RS_THREADS * 8) return 8;
+    return RS_MIN_IPT;
+}
+
 size_t radix_sort_temp_bytes(int n) {
-    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    const int tile = RS_THREADS * sort_ipt(n);
+    const int ntiles = (n + tile - 1) / tile;
     return ((size_t)256 * ntiles + 256) * sizeof(uint32_t) + 2 * (size_t)n * sizeof(uint32_t);
 }
 
@@ -176,7 +190,8 @@ size_t radix_sort_temp_bytes(int n) {
 hipError_t radix_sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys_out, uint32_t* vals_out, int n,
                       int bits, void* temp, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    const int ntiles = (n + RS_TILE - 1) / RS_TILE;
+    const int ipt = sort_ipt(n);
+    const int ntiles = (n + RS_THREADS * ipt - 1) / (RS_THREADS * ipt);
     uint32_t* hist = (uint32_t*)temp;
     uint32_t* totals = hist + (size_t)256 * ntiles;
     uint32_t* kbuf = totals + 256;
@@ -192,10 +207,15 @@ hipError_t radix_sort(const uint32_t* keys, const uint32_t* vals, uint32_t* keys
         uint32_t* ko = to_out ? keys_out : kbuf;
         uint32_t* vo = to_out ? vals_out : vbuf;
         (void)last;
-        hipLaunchKernelGGL(rs_hist, dim3(ntiles), dim3(RS_THREADS), 0, st, kin, n, 8 * p, hist, ntiles);
-        hipLaunchKernelGGL(rs_scan, dim3(256), dim3(RS_THREADS), 0, st, hist, ntiles, totals);
-        hipLaunchKernelGGL(rs_scatter, dim3(ntiles), dim3(RS_THREADS), 0, st, kin, vin, ko, vo, n, 8 * p, hist,
-                           ntiles, totals);
+        auto pass = [&](auto hist_kernel, auto scatter_kernel) {
+            hipLaunchKernelGGL(hist_kernel, dim3(ntiles), dim3(RS_THREADS), 0, st, kin, n, 8 * p, hist, ntiles);
+            hipLaunchKernelGGL(rs_scan, dim3(256), dim3(RS_THREADS), 0, st, hist, ntiles, totals);
+            hipLaunchKernelGGL(scatter_kernel, dim3(ntiles), dim3(RS_THREADS), 0, st, kin, vin, ko, vo, n, 8 * p, hist,
+                               ntiles, totals);
+        };
+        if (ipt == 16) pass(rs_hist<16>, rs_scatter<16>);
+        else if (ipt == 8) pass(rs_hist<8>, rs_scatter<8>);
+        else pass(rs_hist<4>, rs_scatter<4>);
         kin = ko;
         vin = vo;
     }
