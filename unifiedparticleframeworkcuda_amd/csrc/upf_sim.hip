@@ -2167,8 +2167,12 @@ struct Prof {
         used += 2;
         return true;
     }
-    // after the stream has been synchronised
+    // Folds the recorded spans into the totals once the last of them has
+    // completed (stream order: then all have), and frees the events for
+    // reuse; otherwise leaves them for a later call.  A span is never read
+    // before its stop event reports completion.
     void collect() {
+        if (pending.empty() || hipEventQuery(pool[pending.back().second + 1]) != hipSuccess) return;
         for (auto& pr : pending) {
             float t = 0.0f;
             if (hipEventElapsedTime(&t, pool[pr.second], pool[pr.second + 1]) == hipSuccess) {
@@ -2981,6 +2985,8 @@ int upf_profile_classes(void) { return PC_COUNT; }
 const char* upf_profile_name(int cls) { return cls >= 0 && cls < PC_COUNT ? kProfNames[cls] : ""; }
 int upf_profile_read(upf_sim* s, int cls, double* total_ms, long long* launches) {
     if (!s || cls < 0 || cls >= PC_COUNT) return UPF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(s->st));
+    s->prof.collect();
     if (total_ms) *total_ms = s->prof.ms[cls];
     if (launches) *launches = s->prof.cnt[cls];
     return UPF_OK;
