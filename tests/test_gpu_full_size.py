@@ -4,7 +4,9 @@ fluid lattice, 1,048,576 fluid + 83,980 wall particles).
 Two PCISPH steps on the device against two steps of the oracle (its OpenMP
 build, a few seconds on the host): keys and slot order, positions,
 velocities, densities, pressures and the loop statistics agree bit for bit;
-and a restart file written after the two steps continues exactly.
+and a restart file written after the two steps continues exactly.  Then 100
+steps (about a minute of the OpenMP oracle), compared after 10, 25, 50 and
+100 steps, with the iteration count and largest density of every step.
 """
 import numpy as np
 import pytest
@@ -50,3 +52,25 @@ def test_two_steps_at_1m_bit_exact(gpu, full_scene, tmp_path):
     b.save_text(pb, upf.UPF_TEXT_RESTART)
     with open(pa, "rb") as fa, open(pb, "rb") as fb:
         assert fa.read() == fb.read()
+
+
+def test_hundred_steps_at_1m_bit_exact(gpu, full_scene):
+    p, pos, typ, nf = full_scene
+    f = typ == upf.UPF_LIQUID
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    o = ol.OracleSim(p, pos, None, typ, omp=True)
+    done = 0
+    for mark in (10, 25, 50, 100):
+        sim.step(mark - done, upf.UPF_PCISPH_STATIC)
+        o.step(mark - done, upf.UPF_PCISPH_STATIC)
+        done = mark
+        g, r = sim.state(), o.state()
+        kg, rg = sim.stats()
+        ko, ro = o.stats()
+        print("1M dam break, %d steps: iterations %d (device) / %d (oracle)" % (mark, kg.sum(), ko.sum()), flush=True)
+        assert np.array_equal(kg, ko) and np.array_equal(rg, ro), mark
+        for k in ("pos", "vel"):
+            assert np.array_equal(g[k][:, :3], r[k][:, :3]), (mark, k)
+        assert np.array_equal(g["rho"][f], r["rho"][f]), mark
+    sim.close()
