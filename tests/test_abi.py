@@ -148,3 +148,16 @@ def test_inconsistent_domains_are_rejected_before_the_device():
         cnt = (C.c_int * 2)(100, 100)
         assert L.upf_batch_create(2, arr, cnt, 4, C.byref(h)) == 2, field  # UPF_ERR_ARG
         assert L.upf_adjoint_create(C.byref(q), 1000, 4, 2, 2, C.byref(h)) == 2, field  # UPF_ERR_ARG
+
+
+def test_slot_counts_beyond_the_list_entry_range_are_rejected():
+    """A packed list entry holds a slot in 23 bits: a simulation or batch of
+    2^23 slots or more is rejected (UPF_ERR_ARG) before any device call."""
+    L = upf.lib()
+    p = upf.make_params(0.01, 0.001)
+    upf.dam_break(p, 8, 8, 8, (0.16, 0.08, 0.08))
+    h = C.c_void_p()
+    assert L.upf_create(C.byref(p), 1 << 23, C.byref(h)) == 2  # UPF_ERR_ARG
+    arr = (Params * 2)(p, p)
+    cnt = (C.c_int * 2)(1 << 22, 1 << 22)
+    assert L.upf_batch_create(2, arr, cnt, 4, C.byref(h)) == 2  # UPF_ERR_ARG
