@@ -74,3 +74,30 @@ def test_hundred_steps_at_1m_bit_exact(gpu, full_scene):
             assert np.array_equal(g[k][:, :3], r[k][:, :3]), (mark, k)
         assert np.array_equal(g["rho"][f], r["rho"][f]), mark
     sim.close()
+
+
+def test_half_spacing_walls_akinci_bit_exact(gpu):
+    """The same fluid with the configuration's wall layout: walls sampled at
+    half the spacing (335,400 boundary particles) and Akinci boundary volumes
+    (the bench's second 1M leg) -- twelve steps bit-equal to the oracle."""
+    p = upf.make_params(0.01, 0.001)
+    pos, typ, nf = upf.dam_break(p, 128, 64, 128, (2.56, 0.64, 1.28), wall_rel=0.5)
+    upf.density_error_factor(p)
+    p.boundary_pressure_mode = upf.UPF_BOUNDARY_AKINCI
+    print("half-spacing walls: %d fluid, %d boundary" % (nf, len(typ) - nf), flush=True)
+    assert nf == 1 << 20 and len(typ) - nf == 335400
+    f = typ == upf.UPF_LIQUID
+    sim = upf.Simulation(p, len(typ))
+    sim.upload(pos, typ)
+    sim.step(12, upf.UPF_PCISPH_STATIC)
+    o = ol.OracleSim(p, pos, None, typ, omp=True)
+    o.step(12, upf.UPF_PCISPH_STATIC)
+    g, r = sim.state(), o.state()
+    kg, rg = sim.stats()
+    ko, ro = o.stats()
+    print("half-spacing walls, 12 steps: iterations %s" % [int(k) for k in kg], flush=True)
+    assert np.array_equal(kg, ko) and np.array_equal(rg, ro)
+    for k in ("pos", "vel"):
+        assert np.array_equal(g[k][:, :3], r[k][:, :3]), k
+    assert np.array_equal(g["rho"][f], r["rho"][f])
+    sim.close()
