@@ -121,3 +121,39 @@ def test_union_lists_equal_delta_coded_lists(gpu, mode, lattice_rel):
     assert list(Kr) == list(Kg)
     for k in ("pos", "vel", "rho", "p", "force"):
         assert np.array_equal(ref[k], got[k]), k
+
+
+def test_fused_list_build_for_wcsph_and_coupling_equals_separate_kernels(gpu):
+    """The list build fused with the WCSPH / original SPH density pass, and
+    with the rigid-fluid coupling pass (both boundary modes), steps exactly
+    like the separate kernels."""
+    def run(method, make, fuse, steps):
+        p, pos, typ, extra = make()
+        sim = upf.Simulation(p, len(typ))
+        sim.set_tuning(dict(autotune.DEFAULT, nbr_fuse=fuse))
+        sim.upload(pos, typ)
+        if extra is not None:
+            sim.upload_bodies(*extra)
+        sim.step(steps, method)
+        st = sim.state()
+        sim.close()
+        return st
+
+    def wcsph():
+        p, pos, typ, nf = upf.standard_dam_break(16, 12, 16)
+        p.dt = 0.000253
+        return p, pos, typ, None
+
+    def rigid(mode):
+        def make():
+            p, pos, typ, bid, rel, bodies, counts = upf.standard_rigid_scene(16, 24, 16, n_bodies=4, mode=mode)
+            return p, pos, typ, (bodies, bid, rel)
+        return make
+
+    for method, make, steps in ((upf.UPF_WCSPH_STATIC, wcsph, 6), (upf.UPF_SPH_STATIC, wcsph, 4),
+                                (upf.UPF_PCISPH_RIGID, rigid(upf.UPF_BOUNDARY_AKINCI), 4),
+                                (upf.UPF_PCISPH_RIGID, rigid(upf.UPF_BOUNDARY_CPU_REF), 4)):
+        a, b = run(method, make, 1, steps), run(method, make, 0, steps)
+        print("method %d: fused vs separate kernels" % method, flush=True)
+        for k in ("pos", "vel", "rho"):
+            assert np.array_equal(a[k], b[k]), (method, k)

@@ -610,7 +610,7 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
 // which keeps the LDS free for more waves).
-template <int ROUNDS, int STAGE, int BIN, int SKIP>
+template <int ROUNDS, int STAGE, int BIN, int SKIP, int KEEP_RIGID = 0>
 __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __restrict__ pos,
                                            const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                            uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
@@ -814,7 +814,8 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
         __syncthreads();  // r_* are rewritten for the next cell
         s = pe;
     }
-    if (mine_valid) {
+    // KEEP_RIGID: rigid slots keep the counts of their own (box) lists
+    if (mine_valid && !(KEEP_RIGID && ptype(my_x) == UPF_RIGID)) {
         ncount[me] = my_cnt < cap ? my_cnt : cap;
         if (my_cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
     }
@@ -1547,15 +1548,12 @@ __global__ void k_integrate(StepArgs A, Slots S, Work W, int pcisph, int pure_fl
 // P.add_wall_weight, then the pressure of physicsType 'w' (TAIT,
 // CalculatePressureWCSPH :2491-2501) or 'o' (CalculatePressureSPH
 // :2479-2489), over the lists
+// The density work of slot i (one thread) over its packed list (entries
+// carry the table bin): the density pass and the fused list build share it.
 template <bool TAIT>
-__global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
-                                                         Ctl* ctl) {
-    __shared__ float lw[LUT_N];
-    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
-    __syncthreads();
+__device__ __forceinline__ void density_wcsph_slot(const StepArgs& A, const Slots& S, const Work& W, int i,
+                                                   const float* lw) {
     const upf_params& P = A.P;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n) return;
     const float4 xi = S.pos[i];
     if (ptype(xi) != UPF_LIQUID) {
         W.rho[i] = 0.0f;
@@ -1589,6 +1587,30 @@ __global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Wo
     const float4 vi = S.vel[i];
     W.xpd[i] = make_float4(xi.x, xi.y, xi.z, m / rho);
     W.vpg[i] = make_float4(vi.x, vi.y, vi.z, pres / (rho * rho));
+}
+
+template <bool TAIT>
+__global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                         Ctl* ctl) {
+    __shared__ float lw[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.n) density_wcsph_slot<TAIT>(A, S, W, i, lw);
+}
+
+// The list build (entries with bins) and the density pass in one kernel:
+// each lane sums the density of its own slot over the list its wave just
+// wrote (as k_nbr_build_ext does for the PCISPH external forces).
+template <int ROUNDS, bool TAIT>
+__global__ __launch_bounds__(64) void k_nbr_build_wcsph(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                        Ctl* ctl) {
+    __shared__ float lw[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
+    build_wave<ROUNDS, 0, 1, 0>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl);
+    __syncthreads();  // the block is the wave: its lists and the table are complete
+    const int i = xblk(A.xcd) * 64 + (int)threadIdx.x;
+    if (i < A.n) density_wcsph_slot<TAIT>(A, S, W, i, lw);
 }
 
 // CalculateLiquidParticleForces (:3009-3083) + gravity (:2822-2824)
@@ -1709,20 +1731,14 @@ __device__ inline void boundary_pressure(int mode, float dist, float h, const fl
 // for liquid and rigid particles, + the first prediction of the loop; every
 // slot resets its loop pressure.  Pressures and densities of neighbours are
 // the ones the previous step's loop ended with.
+// The coupling force work of slot i (one thread): shared by the coupling
+// pass and the fused list build.  lvl, lpg, lst, lad: the viscosity,
+// pressure-gradient, tension and adhesion tables in LDS.
 template <int MODE>
-__global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
-                                                        Ctl* ctl) {
-    __shared__ float lvl[LUT_N], lpg[LUT_N], lst[LUT_N], lad[LUT_N];
-    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
-        lpg[t] = luts[LUT_N + t];
-        lvl[t] = luts[2 * LUT_N + t];
-        lst[t] = luts[3 * LUT_N + t];
-        lad[t] = luts[4 * LUT_N + t];
-    }
-    __syncthreads();
+__device__ __forceinline__ void ext_coupling_slot(const StepArgs& A, const Slots& S, const Work& W, int i,
+                                                  const float* lvl, const float* lpg, const float* lst,
+                                                  const float* lad, Ctl* ctl) {
     const upf_params& P = A.P;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.n) return;
     const Dom d = dom_at(A, i);
     const float4 xi = S.pos[i];
     const int ti = ptype(xi);
@@ -1876,6 +1892,41 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
         W.force[i] = make_float4(fx, fy, fz, 0.0f);
         W.xs[i] = boundary_xs<MODE>(P.rigid_predicted_current ? xi : S.x0[i], S.psi[i], P.rest_density);
     }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                        Ctl* ctl) {
+    __shared__ float lvl[LUT_N], lpg[LUT_N], lst[LUT_N], lad[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
+        lpg[t] = luts[LUT_N + t];
+        lvl[t] = luts[2 * LUT_N + t];
+        lst[t] = luts[3 * LUT_N + t];
+        lad[t] = luts[4 * LUT_N + t];
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < A.n) ext_coupling_slot<MODE>(A, S, W, i, lvl, lpg, lst, lad, ctl);
+}
+
+// The liquid list build (entries with bins) and the coupling pass in one
+// kernel: the rigid particles' lists and boundary volumes come first (their
+// counts are kept), then a wave builds the lists of its liquid slots and each
+// lane runs the coupling work of its own slot.
+template <int ROUNDS, int MODE>
+__global__ __launch_bounds__(64) void k_nbr_build_coupling(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
+                                                           Ctl* ctl) {
+    __shared__ float lvl[LUT_N], lpg[LUT_N], lst[LUT_N], lad[LUT_N];
+    for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
+        lpg[t] = luts[LUT_N + t];
+        lvl[t] = luts[2 * LUT_N + t];
+        lst[t] = luts[3 * LUT_N + t];
+        lad[t] = luts[4 * LUT_N + t];
+    }
+    build_wave<ROUNDS, 0, 1, 0, 1>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl);
+    __syncthreads();  // the block is the wave: its lists and the tables are complete
+    const int i = xblk(A.xcd) * 64 + (int)threadIdx.x;
+    if (i < A.n) ext_coupling_slot<MODE>(A, S, W, i, lvl, lpg, lst, lad, ctl);
 }
 
 // key of a slot for grouping rigid particles by body (non-rigid last)
@@ -2119,11 +2170,13 @@ using namespace upf;
     } while (0)
 
 enum ProfClass { PC_SORT = 0, PC_GATHER, PC_CELLS, PC_PSI, PC_NBR_BUILD, PC_NBR_EXT, PC_DENSITY, PC_FORCE,
-                 PC_INTEGRATE, PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_UNION, PC_NBR_BUILD_EXT, PC_COUNT };
+                 PC_INTEGRATE, PC_WCSPH_DENSITY, PC_WCSPH_FORCE, PC_RIGID, PC_UNION, PC_NBR_BUILD_EXT,
+                 PC_NBR_BUILD_DENSITY, PC_COUNT };
 static const char* kProfNames[PC_COUNT] = {"radix_sort", "gather", "cell_table", "boundary_volume",
                                           "neighbour_lists", "external_force_predict", "predicted_density",
                                           "pressure_force_predict", "integrate", "wcsph_density", "wcsph_force",
-                                          "rigid_bodies", "neighbour_union", "neighbour_lists_external_force"};
+                                          "rigid_bodies", "neighbour_union", "neighbour_lists_external_force",
+                                          "neighbour_lists_wcsph_density"};
 
 struct Prof {
     bool on = false;
@@ -2594,12 +2647,16 @@ static int step_pcisph_rigid(upf_sim* s) {
     const StepArgs A = s->args();
     Slots& S = s->S[s->cur];
     const dim3 g = grid_for(s->n);
-    build_lists(s, A);  // liquid owners
+    const bool akinci = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI;
+    // fused: rigid lists and volumes first, then the liquid lists and the
+    // coupling pass as one kernel (the liquid build keeps the rigid counts)
+    const bool fuse = s->tune.nbr_fuse && s->tune.nbr_stage == 0 && s->tune.nbr_tile == 0;
+    if (!fuse) build_lists(s, A);  // liquid owners
     s->prof.begin(PC_NBR_BUILD, s->st);
     hipLaunchKernelGGL(k_nbr_build_box, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, (int)UPF_RIGID, s->W.nbr,
                        s->W.ncount, s->ctl);
     s->prof.end(s->st);
-    if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI) {
+    if (akinci) {
         s->prof.begin(PC_PSI, s->st);
         if (!s->psi_ready) {  // walls: once (they do not move)
             hipLaunchKernelGGL(k_psi, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, s->luts, S.psi, s->ctl);
@@ -2609,12 +2666,29 @@ static int step_pcisph_rigid(upf_sim* s) {
                            s->luts, S.psi, s->ctl);
         s->prof.end(s->st);
     }
-    s->prof.begin(PC_NBR_EXT, s->st);
-    if (s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI)
-        hipLaunchKernelGGL(k_ext_coupling<UPF_BOUNDARY_AKINCI>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
-    else
-        hipLaunchKernelGGL(k_ext_coupling<UPF_BOUNDARY_CPU_REF>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
-    s->prof.end(s->st);
+    if (fuse) {
+        s->prof.begin(PC_NBR_BUILD_EXT, s->st);
+        const dim3 gb((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
+        auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, s->st, A, S, s->W, s->luts, s->ctl); };
+        const int r = s->tune.nbr_rounds;
+        if (akinci) {
+            if (r == 2) launch(k_nbr_build_coupling<2, UPF_BOUNDARY_AKINCI>);
+            else if (r == 8) launch(k_nbr_build_coupling<8, UPF_BOUNDARY_AKINCI>);
+            else launch(k_nbr_build_coupling<4, UPF_BOUNDARY_AKINCI>);
+        } else {
+            if (r == 2) launch(k_nbr_build_coupling<2, UPF_BOUNDARY_CPU_REF>);
+            else if (r == 8) launch(k_nbr_build_coupling<8, UPF_BOUNDARY_CPU_REF>);
+            else launch(k_nbr_build_coupling<4, UPF_BOUNDARY_CPU_REF>);
+        }
+        s->prof.end(s->st);
+    } else {
+        s->prof.begin(PC_NBR_EXT, s->st);
+        if (akinci)
+            hipLaunchKernelGGL(k_ext_coupling<UPF_BOUNDARY_AKINCI>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+        else
+            hipLaunchKernelGGL(k_ext_coupling<UPF_BOUNDARY_CPU_REF>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+        s->prof.end(s->st);
+    }
     rc = s->P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? pcisph_loop<UPF_BOUNDARY_AKINCI>(s, A)
                                                             : pcisph_loop<UPF_BOUNDARY_CPU_REF>(s, A);
     if (rc) return rc;
@@ -2647,11 +2721,29 @@ static int step_wcsph_static(upf_sim* s, bool tait) {
     const StepArgs A = s->args();
     Slots& S = s->S[s->cur];
     const dim3 g = grid_for(s->n);
-    build_lists(s, A);
-    s->prof.begin(PC_WCSPH_DENSITY, s->st);
-    if (tait) hipLaunchKernelGGL(k_density_wcsph<true>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
-    else hipLaunchKernelGGL(k_density_wcsph<false>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
-    s->prof.end(s->st);
+    if (s->tune.nbr_fuse && s->tune.nbr_stage == 0 && s->tune.nbr_tile == 0) {
+        // the list build (with bins) and the density pass as one kernel
+        s->prof.begin(PC_NBR_BUILD_DENSITY, s->st);
+        const dim3 gb((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
+        auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, s->st, A, S, s->W, s->luts, s->ctl); };
+        const int r = s->tune.nbr_rounds;
+        if (tait) {
+            if (r == 2) launch(k_nbr_build_wcsph<2, true>);
+            else if (r == 8) launch(k_nbr_build_wcsph<8, true>);
+            else launch(k_nbr_build_wcsph<4, true>);
+        } else {
+            if (r == 2) launch(k_nbr_build_wcsph<2, false>);
+            else if (r == 8) launch(k_nbr_build_wcsph<8, false>);
+            else launch(k_nbr_build_wcsph<4, false>);
+        }
+        s->prof.end(s->st);
+    } else {
+        build_lists(s, A);
+        s->prof.begin(PC_WCSPH_DENSITY, s->st);
+        if (tait) hipLaunchKernelGGL(k_density_wcsph<true>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+        else hipLaunchKernelGGL(k_density_wcsph<false>, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
+        s->prof.end(s->st);
+    }
     s->prof.begin(PC_WCSPH_FORCE, s->st);
     hipLaunchKernelGGL(k_force_wcsph, g, dim3(BLOCK), 0, s->st, A, S, s->W, s->luts, s->ctl);
     s->prof.end(s->st);
