@@ -58,13 +58,33 @@ def test_tunings_give_identical_steps(gpu, tuning):
         assert np.array_equal(ref[k], got[k]), k
 
 
+def _wide_cell_scene():
+    """A dam break whose virtual box is chosen so that cells come out 1.8 h
+    wide (scaled h just above a power of two: the block doubles)."""
+    p, pos, typ, nf = upf.standard_dam_break(20, 14, 16)
+    h = p.support_radius
+    e_min = float(p.vbox_max[0])
+    k = 10
+    while 1024.0 * h / (2 ** k * 1.1) < e_min:
+        k -= 1
+    e = 1024.0 * h / (2 ** k * 1.1)
+    vmax = np.array([e, e, e], np.float32)
+    cmin = np.array(list(p.collision_min), np.float32)
+    cmax = np.array(list(p.collision_max), np.float32)
+    upf.check("upf_params_set_domain", upf.lib().upf_params_set_domain(
+        C.byref(p), vmax.ctypes.data_as(C.POINTER(C.c_float)), cmin.ctypes.data_as(C.POINTER(C.c_float)),
+        cmax.ctypes.data_as(C.POINTER(C.c_float))))
+    upf.density_error_factor(p)
+    return p, pos, typ, nf
+
+
 def test_round_skip_on_wide_cells_equals_the_oracle(gpu):
-    """A scene whose cells are much wider than h (cell side 1.69 h: about 40
+    """A scene whose cells are much wider than h (cell side 1.8 h: about 47
     particles per cell, most rounds of 64 candidates miss a particle's query
     box): the list build that passes over such rounds gives the steps of the
     one that tests every round, and both equal the oracle."""
     import oracle_lib as ol
-    p, pos, typ, nf = upf.batch_dam_breaks(4, seed=0)[2]
+    p, pos, typ, nf = _wide_cell_scene()
     ratio = (1 << p.block_shift) / p.scales[0] / p.support_radius
     print("wide cells: %d particles, cell side %.2f h" % (len(typ), ratio), flush=True)
     assert ratio > 1.5

@@ -27,7 +27,12 @@ uint64_t splitmix64(uint64_t& s) {
 double uniform01(uint64_t& s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
 
 // Domain for a container of extent box[3] whose wall layer starts at `pad`:
-// cubic virtual box (isotropic cells) grown until the block holds h.
+// cubic virtual box (isotropic cells) grown until the block holds h.  A cell
+// is a power-of-two block of the 1024-unit z-index grid, so that first extent
+// can leave cells up to 2 h wide, and the list build tests (cell / h)^3 times
+// the candidates it would with cells of h.  When cells come out wider than
+// 1.2 h, the extent grows further, to where the next smaller block holds h
+// with a small margin (cells of 1.0005 h, eight times as many).
 int choose_domain(upf_params* p, const float box[3], float pad) {
     float e = 0.0f;
     for (int a = 0; a < 3; ++a) e = std::max(e, box[a]);
@@ -38,13 +43,24 @@ int choose_domain(upf_params* p, const float box[3], float pad) {
         cmin[a] = pad + r;
         cmax[a] = pad + box[a] - r;  // container interior [pad, pad + box]
     }
-    for (int tries = 0; tries < 400; ++tries) {
-        float vmax[3] = {e, e, e};
-        upf_params q = *p;
-        if (upf_params_set_domain(&q, vmax, cmin, cmax) == UPF_OK) { *p = q; return UPF_OK; }
-        e *= 1.002f;
+    auto fit = [&](float e0, upf_params& out) {
+        for (int tries = 0; tries < 400; ++tries, e0 *= 1.002f) {
+            const float vmax[3] = {e0, e0, e0};
+            upf_params q = *p;
+            if (upf_params_set_domain(&q, vmax, cmin, cmax) == UPF_OK) { out = q; return true; }
+        }
+        return false;
+    };
+    upf_params q;
+    if (!fit(e, q)) return UPF_ERR_ARG;
+    const float wide = (float)q.block_size / (q.scales[0] * q.support_radius);
+    if (wide > 1.2f && q.block_size >= 8) {
+        upf_params f;
+        const float e1 = 1024.0f * p->support_radius * 1.0005f / (float)(q.block_size / 2);
+        if (fit(e1, f) && f.block_size == q.block_size / 2) q = f;
     }
-    return UPF_ERR_ARG;
+    *p = q;
+    return UPF_OK;
 }
 
 }  // namespace upf
