@@ -96,6 +96,16 @@ struct StepArgs {
     int slots_per_scene;
     int code_lists;      // the correction loop reads delta-coded lists (the external-force pass writes them)
     int xcd;             // remap blocks to XCDs by contiguous ranges (grids are multiples of 8)
+    // batched form: the scenes' slots in chunks of 64 (a wave; chunks never
+    // straddle scenes), and per PCISPH iteration parity the list of the
+    // chunks whose scene runs that iteration (act_n[p] < 0: all chunks, in order)
+    int n_chunks;
+    const int* chunk_begin;  // first slot of chunk c
+    const int* chunk_end;    // one past its last slot
+    const int* sc_chunk0;    // first chunk of scene s
+    const int* sc_nchunks;   // its number of chunks
+    int* act;                // [2][n_chunks] chunk lists
+    int* act_n;              // [2] list lengths
 };
 
 __device__ inline int scene_at(const StepArgs& A, int i) { return A.scene_of ? A.scene_of[i] : 0; }
@@ -1225,14 +1235,109 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     __shared__ float2 th[LUT_N];
     __shared__ float wmax[BLOCK / 64];  // blockDim.x <= BLOCK
     __shared__ float4 urec[CODED == 2 ? UNION_CAP : 1];  // the chunk's union: predicted positions
+    // batched: empty the next iteration's chunk list, which the force pass
+    // of this iteration fills (a different list from the one read here)
+    if (A.act && blockIdx.x == 0 && threadIdx.x == 0) A.act_n[(t + 1) & 1] = 0;
+    const upf_params& P = A.P;
+    const int lane = threadIdx.x & 63;
+    const int sps = A.slots_per_scene;
+    // the predicted density of slot i (this iteration's pressure, the
+    // force pass's record), or 0 with sc = -1 when the slot has none
+    auto density_of = [&](int i, int un, float& mine, int& sc) __attribute__((always_inline)) {
+        mine = 0.0f;
+        sc = scene_at(A, i);
+        // batched: a converged scene's slot stops before its record is
+        // read (one scene runs iteration t: the kernel's entry test decided it)
+        const bool runs = !A.scene_of || t < A.sctl[sc].want;
+        const float4 xi = runs ? W.xs[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if (!(runs && ptype(xi) == UPF_LIQUID)) {
+            sc = -1;
+            return;
+        }
+        const float m = P.initial_mass;
+        const float T = P.support_radius_sq_limit;
+        float rho = P.kernel_self * m;
+        if (P.add_wall_weight)  // the GPU path's order: walls before neighbours
+            rho = rho + wall_weight<false>(P, wall_distance(P, xi.x, xi.y, xi.z), A.lut_factor, lw);
+        const int cnt = W.ncount[i];
+        const size_t st = (size_t)A.stride;
+        // G neighbours per round: their loads are issued together,
+        // the sum still runs in list order
+        auto use = [&](int j, uint32_t, const float4& xj) __attribute__((always_inline)) {
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
+                if (d2 < T) {
+                    const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
+                    // a boundary neighbour's record carries -(rho0 psi)
+                    const float mj = ptype(xj) == UPF_LIQUID ? m : -xj.w;
+                    rho = rho + w * mj;
+                }
+            } else {
+                // without a branch: a pair at d2 >= T adds w = +0, and
+                // rho + (+0) * m is rho exactly (rho > 0)
+                const float wt = lw[bin_of_d2(d2, A.lut_factor, th)];
+                const float w = d2 < T ? wt : 0.0f;
+                rho = rho + w * m;
+            }
+        };
+        auto from_global = [&](int j) { return at_u32(W.xs, j); };
+        if constexpr (CODED == 2) {
+            if (un >= 0) for_each_listed<2, G>(W, i, st, cnt, [&](int u) { return urec[u]; }, use);
+            else for_each_listed<1, G>(W, i, st, cnt, from_global, use);
+        } else {
+            for_each_listed<CODED, G>(W, i, st, cnt, from_global, use);
+        }
+        const float err = fmax_ref(rho - P.rest_density, 0.0f);
+        const float delta = A.scene_of ? A.doms[sc].delta : P.density_error_factor;
+        const float p = W.p[i] + fmax_ref(err * delta, 0.0f);
+        W.p[i] = p;
+        // the force pass gathers current position and p / rho0^2 of a
+        // neighbour in one 16-byte load; the position part was written
+        // at the start of the step
+        const float pd = p / (P.rest_density * P.rest_density);
+        reinterpret_cast<float*>(W.xpd)[4 * (size_t)i + 3] = pd;
+        W.rho[i] = rho;
+        mine = rho;
+    };
+    if constexpr (CODED != 2) {
+        if (A.act) {
+            // batched: one wave per chunk of this iteration's list
+            int cnt = A.act_n[t & 1];
+            const bool all = cnt < 0;
+            if (all) cnt = -cnt;
+            // consecutive blocks go to different XCDs: a short list (late in
+            // the loop, few scenes left) spreads over the whole chip
+            const int wpb = blockDim.x >> 6;
+            const int w0 = (int)blockIdx.x * wpb;
+            if (w0 >= cnt) return;  // nothing for this block: no tables
+            for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
+                lw[k] = k < 300 ? luts[k] : 0.0f;  // bin 300 (dist >= h) weighs 0, as in lut_at
+                th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
+            }
+            __syncthreads();
+            const int* list = A.act + (size_t)(t & 1) * A.n_chunks;
+            for (int q = w0 + (int)(threadIdx.x >> 6); q < cnt; q += (int)gridDim.x * wpb) {
+                const int c = all ? q : list[q];
+                const int i = A.chunk_begin[c] + lane;
+                float mine = 0.0f;
+                int sc = -1;
+                if (i < A.chunk_end[c]) density_of(i, -1, mine, sc);
+                // a chunk holds one scene: one maximum per wave
+                const uint64_t act = __ballot(sc >= 0);
+                if (act == 0ull) continue;
+                const int first = __shfl(sc, __ffsll((long long)act) - 1, 64);
+                const float mx = wave_max(mine);
+                if (lane == 0) atomicMax(&A.max_slots[(first * sps + q % sps) * 32], __float_as_uint(mx));
+            }
+            return;
+        }
+    }
     for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
         lw[k] = k < 300 ? luts[k] : 0.0f;  // bin 300 (dist >= h) weighs 0, as in lut_at
         th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
     }
     __syncthreads();
-    const upf_params& P = A.P;
-    const int lane = threadIdx.x & 63;
-    const int sps = A.slots_per_scene;
     float run = 0.0f;  // single scene: running max over the thread's slots
     const int n_round = (A.n + gstride() - 1) / gstride();
     for (int r = 0; r < n_round; ++r) {
@@ -1242,62 +1347,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
         if constexpr (CODED == 2) un = stage_union(A, W, W.xs, urec, (i - (int)threadIdx.x) / UNION_CHUNK);
         float mine = 0.0f;
         int sc = -1;
-        if (i < A.n) {
-            sc = scene_at(A, i);
-            // batched: a converged scene's slot stops before its record is
-            // read (one scene runs iteration t: the kernel's entry test decided it)
-            const bool runs = !A.scene_of || t < A.sctl[sc].want;
-            const float4 xi = runs ? W.xs[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-            if (runs && ptype(xi) == UPF_LIQUID) {
-                const float m = P.initial_mass;
-                const float T = P.support_radius_sq_limit;
-                float rho = P.kernel_self * m;
-                if (P.add_wall_weight)  // the GPU path's order: walls before neighbours
-                    rho = rho + wall_weight<false>(P, wall_distance(P, xi.x, xi.y, xi.z), A.lut_factor, lw);
-                const int cnt = W.ncount[i];
-                const size_t st = (size_t)A.stride;
-                // G neighbours per round: their loads are issued together,
-                // the sum still runs in list order
-                auto use = [&](int j, uint32_t, const float4& xj) __attribute__((always_inline)) {
-                    const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
-                        if (d2 < T) {
-                            const float w = lw[bin_of_d2(d2, A.lut_factor, th)];
-                            // a boundary neighbour's record carries -(rho0 psi)
-                            const float mj = ptype(xj) == UPF_LIQUID ? m : -xj.w;
-                            rho = rho + w * mj;
-                        }
-                    } else {
-                        // without a branch: a pair at d2 >= T adds w = +0, and
-                        // rho + (+0) * m is rho exactly (rho > 0)
-                        const float wt = lw[bin_of_d2(d2, A.lut_factor, th)];
-                        const float w = d2 < T ? wt : 0.0f;
-                        rho = rho + w * m;
-                    }
-                };
-                auto from_global = [&](int j) { return at_u32(W.xs, j); };
-                if constexpr (CODED == 2) {
-                    if (un >= 0) for_each_listed<2, G>(W, i, st, cnt, [&](int u) { return urec[u]; }, use);
-                    else for_each_listed<1, G>(W, i, st, cnt, from_global, use);
-                } else {
-                    for_each_listed<CODED, G>(W, i, st, cnt, from_global, use);
-                }
-                const float err = fmax_ref(rho - P.rest_density, 0.0f);
-                const float delta = A.scene_of ? A.doms[sc].delta : P.density_error_factor;
-                const float p = W.p[i] + fmax_ref(err * delta, 0.0f);
-                W.p[i] = p;
-                // the force pass gathers current position and p / rho0^2 of a
-                // neighbour in one 16-byte load; the position part was written
-                // at the start of the step
-                const float pd = p / (P.rest_density * P.rest_density);
-                reinterpret_cast<float*>(W.xpd)[4 * (size_t)i + 3] = pd;
-                W.rho[i] = rho;
-                mine = rho;
-            } else {
-                sc = -1;
-            }
-        }
+        if (i < A.n) density_of(i, un, mine, sc);
         if (A.scene_of == nullptr) {
             run = fmaxf(run, mine);
             continue;
@@ -1345,6 +1395,7 @@ __device__ inline void loop_test(const StepArgs& A, int sc, int t, int lane, Ctl
         *slot = 0u;
     }
     for (int o = 32; o > 0; o >>= 1) b = max(b, (uint32_t)__shfl_xor((int)b, o, 64));
+    int want = 0;
     if (lane == 0) {
         const float mx = __uint_as_float(b);
         const float pct = fmax_ref(0.1f * mx - 100.0f, 0.0f);
@@ -1352,11 +1403,24 @@ __device__ inline void loop_test(const StepArgs& A, int sc, int t, int lane, Ctl
         if (pct < P.max_density_error_allowed) too_large = 0;
         const int it = t + 1;
         const bool next = it < P.min_loops || (too_large && it < P.max_loops);
-        const int want = next ? it + 1 : it;
+        want = next ? it + 1 : it;
         C.too_large = too_large;
         C.last_max = mx;
         C.want = want;
         atomicMax(&ctl->want_any, want);
+    }
+    // batched: a scene that runs iteration t + 1 puts its chunks on that
+    // iteration's list (emptied by the density pass of iteration t)
+    if (A.act) {
+        want = __shfl(want, 0, 64);
+        if (want > t + 1) {
+            const int nch = A.sc_nchunks[sc], c0 = A.sc_chunk0[sc];
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&A.act_n[(t + 1) & 1], nch);
+            base = __shfl(base, 0, 64);
+            int* list = A.act + (size_t)((t + 1) & 1) * A.n_chunks;
+            for (int k = lane; k < nch; k += 64) list[base + k] = c0 + k;
+        }
     }
 }
 
@@ -1370,7 +1434,10 @@ __global__ void k_loop_init(StepArgs A, Ctl* ctl) {
         A.sctl[sc].too_large = 1;
         A.sctl[sc].last_max = 0.0f;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->want_any = w;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->want_any = w;
+        if (A.act) A.act_n[0] = -A.n_chunks;  // iteration 0: every chunk
+    }
 }
 
 // Per-step record of every scene's iteration count and last max density
@@ -1426,67 +1493,92 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
     __shared__ float lpg[LUT_N];
     __shared__ float2 th[LUT_N];
     __shared__ float4 urec[CODED == 2 ? UNION_CAP : 1];  // the chunk's union: (position, p / rho0^2)
-    for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
-        lpg[k] = luts[LUT_N + k];
-        th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
-    }
-    __syncthreads();
     const upf_params& P = A.P;
+    // the correction pressure force of slot i and its next prediction
+    auto force_of = [&](int i, int un) __attribute__((always_inline)) {
+        // batched: a converged scene's slot stops before its record is read (one
+        // scene runs iteration t: the kernel's entry test decided it)
+        if (A.scene_of && t >= A.sctl[A.scene_of[i]].want) return;
+        const float4 xi = S.pos[i];
+        if (ptype(xi) != UPF_LIQUID) return;
+        const float m = P.initial_mass;
+        const float pi_d = W.p[i] / A.dens_sq;
+        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+        const int cnt = W.ncount[i];
+        const size_t st = (size_t)A.stride;
+        // every listed pair is closer than h at these (current) positions, so the
+        // reference's `dist < h` test holds; the table bin is the one the list
+        // build computed from the same positions (carried by the packed entry,
+        // recomputed by the same expression for the coded lists)
+        // (one 16-byte record per neighbour: current position and p / rho0^2,
+        // or for an Akinci boundary neighbour its mass rho0 psi, negated)
+        auto use = [&](int j, uint32_t e, const float4& xj) __attribute__((always_inline)) {
+            const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+            const uint32_t bin = CODED ? bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th) : (e & NB_BIN_MASK);
+            const float g = lpg[bin];
+            // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
+            // precomputed per particle by the density pass (same division)
+            float pd = xj.w, mj = m;
+            if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
+                if (__builtin_signbit(xj.w)) {  // boundary: p_j = 0, mass rho0 psi_j
+                    mj = -xj.w;
+                    pd = 0.0f;
+                }
+            }
+            const float grad = pi_d + pd;
+            fx = fx - (((dx * g) * grad) * m) * mj;
+            fy = fy - (((dy * g) * grad) * m) * mj;
+            fz = fz - (((dz * g) * grad) * m) * mj;
+        };
+        auto from_global = [&](int j) { return at_u32(W.xpd, j); };
+        if constexpr (CODED == 2) {
+            if (un >= 0)
+                for_each_listed<2, G>(W, i, st, cnt, [&](int u) { return urec[u]; }, use);
+            else
+                for_each_listed<1, G>(W, i, st, cnt, from_global, use);
+        } else {
+            for_each_listed<CODED, G>(W, i, st, cnt, from_global, use);
+        }
+        const float4 Fp = make_float4(fx, fy, fz, 0.0f);
+        W.fp[i] = Fp;
+        W.xs[i] = predict(xi, S.vel[i], W.force[i], Fp, A, dom_at(A, i), ctl);
+    };
+    auto load_tables = [&]() {
+        for (int k = threadIdx.x; k < LUT_N; k += blockDim.x) {
+            lpg[k] = luts[LUT_N + k];
+            th[k] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[k];
+        }
+        __syncthreads();
+    };
+    if constexpr (CODED != 2) {
+        if (A.act) {
+            // batched: one wave per chunk of this iteration's list
+            int cnt = A.act_n[t & 1];
+            const bool all = cnt < 0;
+            if (all) cnt = -cnt;
+            // consecutive blocks go to different XCDs: a short list (late in
+            // the loop, few scenes left) spreads over the whole chip
+            const int wpb = blockDim.x >> 6;
+            const int w0 = (int)blockIdx.x * wpb;
+            if (w0 >= cnt) return;  // nothing for this block: no tables
+            load_tables();
+            const int* list = A.act + (size_t)(t & 1) * A.n_chunks;
+            for (int q = w0 + (int)(threadIdx.x >> 6); q < cnt; q += (int)gridDim.x * wpb) {
+                const int c = all ? q : list[q];
+                const int i = A.chunk_begin[c] + (int)(threadIdx.x & 63);
+                if (i < A.chunk_end[c]) force_of(i, -1);
+            }
+            return;
+        }
+    }
+    load_tables();
     const int n_round = (A.n + gstride() - 1) / gstride();
     for (int r = 0; r < n_round; ++r) {
-    const int i = r * gstride() + xblk(A.xcd) * blockDim.x + threadIdx.x;
-    if (block_idle(A, i - (int)threadIdx.x, t)) continue;
-    int un = -1;
-    if constexpr (CODED == 2) un = stage_union(A, W, W.xpd, urec, (i - (int)threadIdx.x) / UNION_CHUNK);
-    if (i >= A.n) continue;
-    // batched: a converged scene's slot stops before its record is read (one
-    // scene runs iteration t: the kernel's entry test decided it)
-    if (A.scene_of && t >= A.sctl[A.scene_of[i]].want) continue;
-    const float4 xi = S.pos[i];
-    if (ptype(xi) != UPF_LIQUID) continue;
-    const float m = P.initial_mass;
-    const float T = P.support_radius_sq_limit;
-    const float rho0 = P.rest_density;
-    const float pi_d = W.p[i] / A.dens_sq;
-    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    const int cnt = W.ncount[i];
-    const size_t st = (size_t)A.stride;
-    // every listed pair is closer than h at these (current) positions, so the
-    // reference's `dist < h` test holds; the table bin is the one the list
-    // build computed from the same positions (carried by the packed entry,
-    // recomputed by the same expression for the coded lists)
-    // (one 16-byte record per neighbour: current position and p / rho0^2,
-    // or for an Akinci boundary neighbour its mass rho0 psi, negated)
-    auto use = [&](int j, uint32_t e, const float4& xj) __attribute__((always_inline)) {
-        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
-        const uint32_t bin = CODED ? bin_of_d2(dx * dx + dy * dy + dz * dz, A.lut_factor, th) : (e & NB_BIN_MASK);
-        const float g = lpg[bin];
-        // grad = p_i / densSq + p_j / (rho0 * rho0); the neighbour term is
-        // precomputed per particle by the density pass (same division)
-        float pd = xj.w, mj = m;
-        if constexpr (MODE == UPF_BOUNDARY_AKINCI) {
-            if (__builtin_signbit(xj.w)) {  // boundary: p_j = 0, mass rho0 psi_j
-                mj = -xj.w;
-                pd = 0.0f;
-            }
-        }
-        const float grad = pi_d + pd;
-        fx = fx - (((dx * g) * grad) * m) * mj;
-        fy = fy - (((dy * g) * grad) * m) * mj;
-        fz = fz - (((dz * g) * grad) * m) * mj;
-    };
-    auto from_global = [&](int j) { return at_u32(W.xpd, j); };
-    if constexpr (CODED == 2) {
-        if (un >= 0)
-            for_each_listed<2, G>(W, i, st, cnt, [&](int u) { return urec[u]; }, use);
-        else
-            for_each_listed<1, G>(W, i, st, cnt, from_global, use);
-    } else {
-        for_each_listed<CODED, G>(W, i, st, cnt, from_global, use);
-    }
-    const float4 Fp = make_float4(fx, fy, fz, 0.0f);
-    W.fp[i] = Fp;
-    W.xs[i] = predict(xi, S.vel[i], W.force[i], Fp, A, dom_at(A, i), ctl);
+        const int i = r * gstride() + xblk(A.xcd) * blockDim.x + threadIdx.x;
+        if (block_idle(A, i - (int)threadIdx.x, t)) continue;
+        int un = -1;
+        if constexpr (CODED == 2) un = stage_union(A, W, W.xpd, urec, (i - (int)threadIdx.x) / UNION_CHUNK);
+        if (i < A.n) force_of(i, un);
     }
 }
 
@@ -2268,6 +2360,14 @@ struct upf_sim {
     int sps = MAX_SLOTS;          // max-reduction slots per scene
     Dom* d_doms = nullptr;
     int* d_scene_of = nullptr;
+    // batched form: 64-slot chunks of the scenes and the per-iteration lists
+    int n_chunks = 0;
+    int* d_chunk_begin = nullptr;
+    int* d_chunk_end = nullptr;
+    int* d_sc_chunk0 = nullptr;
+    int* d_sc_nchunks = nullptr;
+    int* d_act = nullptr;
+    int* d_act_n = nullptr;
     uint32_t* b_keys1 = nullptr;  // batched sort: key-sorted keys, their permutation, scene keys
     uint32_t* b_perm1 = nullptr;
     uint32_t* b_sk = nullptr;
@@ -2323,6 +2423,13 @@ struct upf_sim {
         A.slots_per_scene = sps;
         A.code_lists = tune.loop_list;
         A.xcd = tune.xcd_remap;
+        A.n_chunks = n_chunks;
+        A.chunk_begin = d_chunk_begin;
+        A.chunk_end = d_chunk_end;
+        A.sc_chunk0 = d_sc_chunk0;
+        A.sc_nchunks = d_sc_nchunks;
+        A.act = n_scenes > 1 ? d_act : nullptr;
+        A.act_n = d_act_n;
         return A;
     }
     template <class T> hipError_t alloc(T** p, size_t count) {
@@ -3543,6 +3650,32 @@ int upf_batch_create(int n_scenes, const upf_params* scene_params, const int* co
         if (n_scenes > 1 &&
             hipMemcpy(s->d_doms, doms.data(), doms.size() * sizeof(Dom), hipMemcpyHostToDevice) != hipSuccess)
             rc = UPF_ERR_HIP;
+        // the scenes' 64-slot chunks (a chunk never straddles two scenes) for
+        // the correction loop's per-iteration lists of running scenes
+        if (!rc && n_scenes > 1) {
+            std::vector<int> cb, ce, c0(n_scenes), nc(n_scenes);
+            int off = 0;
+            for (int k = 0; k < n_scenes; ++k) {
+                c0[k] = (int)cb.size();
+                for (int b0 = 0; b0 < counts[k]; b0 += 64) {
+                    cb.push_back(off + b0);
+                    ce.push_back(off + std::min(b0 + 64, counts[k]));
+                }
+                nc[k] = (int)cb.size() - c0[k];
+                off += counts[k];
+            }
+            s->n_chunks = (int)cb.size();
+            if (s->alloc(&s->d_chunk_begin, cb.size()) != hipSuccess || s->alloc(&s->d_chunk_end, ce.size()) != hipSuccess ||
+                s->alloc(&s->d_sc_chunk0, (size_t)n_scenes) != hipSuccess ||
+                s->alloc(&s->d_sc_nchunks, (size_t)n_scenes) != hipSuccess ||
+                s->alloc(&s->d_act, 2 * cb.size()) != hipSuccess || s->alloc(&s->d_act_n, 2) != hipSuccess ||
+                hipMemcpy(s->d_chunk_begin, cb.data(), cb.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(s->d_chunk_end, ce.data(), ce.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(s->d_sc_chunk0, c0.data(), c0.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(s->d_sc_nchunks, nc.data(), nc.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemset(s->d_act_n, 0, 2 * sizeof(int)) != hipSuccess)
+                rc = UPF_ERR_HIP;
+        }
     }
     if (rc) {
         upf_batch_destroy(b);
