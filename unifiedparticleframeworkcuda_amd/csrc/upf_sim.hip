@@ -473,10 +473,23 @@ __device__ __forceinline__ void for_each_listed(const Work& W, int i, size_t st,
             lo = nlo;
             hi = nhi;
         }
-        // last, partial row
-        for (int u = 0; k0 + u < cnt; ++u) {
-            const int j = dec.next(k0, u, lo, hi);
-            use(j, 0u, load(j));
+        // last, partial row (1 .. 7 entries): its gathers are issued together
+        // as well, so that the row costs one round trip, not one per entry
+        if (k0 < cnt) {
+            const int rem = cnt - k0;
+            int j[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) j[u] = u < rem ? dec.next(k0, u, lo, hi) : i;
+#pragma unroll
+            for (int h = 0; h < 8; h += G) {
+                V v[G];
+#pragma unroll
+                for (int u = 0; u < G; ++u)
+                    if (h + u < rem) v[u] = load(j[h + u]);
+#pragma unroll
+                for (int u = 0; u < G; ++u)
+                    if (h + u < rem) use(j[h + u], 0u, v[u]);
+            }
         }
     } else {
         const uint32_t* L = W.nbr + i;
@@ -491,9 +504,20 @@ __device__ __forceinline__ void for_each_listed(const Work& W, int i, size_t st,
 #pragma unroll
             for (int u = 0; u < G; ++u) use((int)(e[u] >> NB_SLOT_SHIFT), e[u], v[u]);
         }
-        for (; k0 < cnt; ++k0) {
-            const uint32_t e = L[(size_t)k0 * st];
-            use((int)(e >> NB_SLOT_SHIFT), e, load((int)(e >> NB_SLOT_SHIFT)));
+        // the last cnt - k0 < G entries, their gathers issued together
+        if (k0 < cnt) {
+            const int rem = cnt - k0;
+            uint32_t e[G];
+            V v[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (u < rem) e[u] = L[(size_t)(k0 + u) * st];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (u < rem) v[u] = load((int)(e[u] >> NB_SLOT_SHIFT));
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+                if (u < rem) use((int)(e[u] >> NB_SLOT_SHIFT), e[u], v[u]);
         }
     }
 }
