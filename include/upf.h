@@ -515,7 +515,8 @@ typedef struct upf_tuning {
                          (static PCISPH; needs loop_block 256)                     */
     int loop_width;   /* correction loop: neighbours whose loads are issued together, 4 or 8 */
     int nbr_tile;     /* list build: 0 cell by cell (nbr_rounds, nbr_stage), or the tile
-                         form with 256 / 512 candidates staged in LDS at a time        */
+                         form with 256 / 512 candidates staged in LDS at a time (the
+                         coupling step's rigid lists: then one thread per rigid slot) */
     int spec_first;   /* host loop: first enqueue the previous step's iteration count,
                          at most this many past the minimum (0 .. 64)                  */
     int xcd_remap;    /* 1: each XCD (own L2) takes one contiguous eighth of the blocks

@@ -177,3 +177,23 @@ def test_fused_list_build_for_wcsph_and_coupling_equals_separate_kernels(gpu):
         print("method %d: fused vs separate kernels" % method, flush=True)
         for k in ("pos", "vel", "rho"):
             assert np.array_equal(a[k], b[k]), (method, k)
+
+
+def test_rigid_lists_by_the_wave_build_equal_the_per_particle_build(gpu):
+    """The rigid slots' lists of the coupling step come from the wave-cooperative
+    build (nbr_tile 0) or from one thread per rigid slot over its query box
+    (nbr_tile 256): both boundary modes step exactly alike."""
+    for mode in (upf.UPF_BOUNDARY_AKINCI, upf.UPF_BOUNDARY_CPU_REF):
+        out = []
+        for tile in (0, 256):
+            p, pos, typ, bid, rel, bodies, counts = upf.standard_rigid_scene(16, 24, 16, n_bodies=4, mode=mode)
+            sim = upf.Simulation(p, len(typ))
+            sim.set_tuning(dict(autotune.DEFAULT, nbr_tile=tile))
+            sim.upload(pos, typ)
+            sim.upload_bodies(bodies, bid, rel)
+            sim.step(4, upf.UPF_PCISPH_RIGID)
+            out.append(sim.state())
+            sim.close()
+        print("mode %d: rigid lists, wave build vs per-particle build" % mode, flush=True)
+        for k in ("pos", "vel", "rho", "p", "force"):
+            assert np.array_equal(out[0][k], out[1][k]), (mode, k)

@@ -644,7 +644,7 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
 // which keeps the LDS free for more waves).
-template <int ROUNDS, int STAGE, int BIN, int SKIP, int KEEP_RIGID = 0>
+template <int ROUNDS, int STAGE, int BIN, int SKIP, int KEEP_RIGID = 0, int OWNER = UPF_LIQUID>
 __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __restrict__ pos,
                                            const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                            uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
@@ -671,7 +671,9 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
     const float4 my_x = mine_valid ? pos[me] : make_float4(0.f, 0.f, 0.f, 0.f);
     const uint32_t my_key = mine_valid ? keys[me] : 0xFFFFFFFFu;
     const int my_scene = mine_valid ? scene_at(A, me) : -1;
-    const bool my_liquid = mine_valid && ptype(my_x) == UPF_LIQUID;
+    // owners: the slots whose lists this build writes (liquid, or the rigid
+    // slots of the coupling step)
+    const bool my_liquid = mine_valid && ptype(my_x) == OWNER;
     uint32_t my_blo = 0u, my_bhi = 0u;
     if (my_liquid) {
         // BoxQuery of the particle in cell coordinates, clipped to the grid
@@ -848,8 +850,9 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
         __syncthreads();  // r_* are rewritten for the next cell
         s = pe;
     }
-    // KEEP_RIGID: rigid slots keep the counts of their own (box) lists
-    if (mine_valid && !(KEEP_RIGID && ptype(my_x) == UPF_RIGID)) {
+    // KEEP_RIGID: rigid slots keep the counts of their own lists; a rigid-owner
+    // build writes only those
+    if (OWNER == UPF_LIQUID ? mine_valid && !(KEEP_RIGID && ptype(my_x) == UPF_RIGID) : my_liquid) {
         ncount[me] = my_cnt < cap ? my_cnt : cap;
         if (my_cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
     }
@@ -868,6 +871,17 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                                   uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
     build_wave<ROUNDS, STAGE, BIN, SKIP>(A, pos, keys, cells, nbr, ncount, ctl);
+}
+
+// The lists of the rigid slots for the coupling step (GetNeighbors of a rigid
+// particle: every particle type, entries with their table bins), by the same
+// wave-cooperative build; the other slots' counts are left as they are.
+template <int ROUNDS>
+__global__ __launch_bounds__(64) void k_nbr_build_rigid(StepArgs A, const float4* __restrict__ pos,
+                                                        const uint32_t* __restrict__ keys,
+                                                        const uint2* __restrict__ cells, uint32_t* __restrict__ nbr,
+                                                        int* __restrict__ ncount, Ctl* ctl) {
+    build_wave<ROUNDS, 0, 1, 0, 0, UPF_RIGID>(A, pos, keys, cells, nbr, ncount, ctl);
 }
 
 // GetNeighbors again, tile form: one wave per 64 consecutive slots of one
@@ -1156,12 +1170,17 @@ __device__ __forceinline__ void ext_static_slot(const StepArgs& A, const Slots& 
     };
     ListEncoder enc(W, i, st);
     const bool code = A.code_lists != 0;
+    // groups of four entries; the next group's entries are loaded before this
+    // group's gathers, and the last, partial group's gathers go out together
+    uint32_t e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) e[u] = u < cnt ? L[u * st] : 0u;
     int k = 0;
     for (; k + 4 <= cnt; k += 4) {
-        uint32_t e[4];
-        float4 xj[4], vj[4];
+        uint32_t en[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) e[u] = L[(k + u) * st];
+        for (int u = 0; u < 4; ++u) en[u] = k + 4 + u < cnt ? L[(k + 4 + u) * st] : 0u;
+        float4 xj[4], vj[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = (int)(e[u] >> NB_SLOT_SHIFT);
@@ -1171,12 +1190,24 @@ __device__ __forceinline__ void ext_static_slot(const StepArgs& A, const Slots& 
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) pair(e[u], xj[u], vj[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = en[u];
     }
-    for (; k < cnt; ++k) {
-        const uint32_t e = L[k * st];
-        const int j = (int)(e >> NB_SLOT_SHIFT);
-        if (code) enc.put(k, (uint32_t)j, ctl);
-        pair(e, S.pos[j], S.vel[j]);
+    if (k < cnt) {
+        const int rem = cnt - k;
+        float4 xj[4], vj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u < rem) {
+                const int j = (int)(e[u] >> NB_SLOT_SHIFT);
+                if (code) enc.put(k + u, (uint32_t)j, ctl);
+                xj[u] = S.pos[j];
+                vj[u] = S.vel[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u < rem) pair(e[u], xj[u], vj[u]);
     }
     if (code) enc.finish();
     fy = fy - P.initial_mass * P.gravity;
@@ -2784,8 +2815,20 @@ static int step_pcisph_rigid(upf_sim* s) {
     const bool fuse = s->tune.nbr_fuse && s->tune.nbr_stage == 0 && s->tune.nbr_tile == 0;
     if (!fuse) build_lists(s, A);  // liquid owners
     s->prof.begin(PC_NBR_BUILD, s->st);
-    hipLaunchKernelGGL(k_nbr_build_box, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, (int)UPF_RIGID, s->W.nbr,
-                       s->W.ncount, s->ctl);
+    if (s->tune.nbr_tile == 0) {
+        const dim3 gb((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
+        auto launch = [&](auto kern) {
+            hipLaunchKernelGGL(kern, gb, dim3(64), 0, s->st, A, S.pos, s->W.keys_sorted, s->W.cells, s->W.nbr,
+                               s->W.ncount, s->ctl);
+        };
+        const int r = s->tune.nbr_rounds;
+        if (r == 2) launch(k_nbr_build_rigid<2>);
+        else if (r == 8) launch(k_nbr_build_rigid<8>);
+        else launch(k_nbr_build_rigid<4>);
+    } else {
+        hipLaunchKernelGGL(k_nbr_build_box, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, (int)UPF_RIGID, s->W.nbr,
+                           s->W.ncount, s->ctl);
+    }
     s->prof.end(s->st);
     if (akinci) {
         s->prof.begin(PC_PSI, s->st);
