@@ -2105,24 +2105,47 @@ __device__ inline void mat3mul(const float* A, const float* B, float* C) {  // M
             C[r * 3 + c] = A[r * 3 + 0] * B[0 * 3 + c] + A[r * 3 + 1] * B[1 * 3 + c] + A[r * 3 + 2] * B[2 * 3 + c];
 }
 
-// CalculateRigidBodyForceTorqueMomenta (:3620-3703) with RigidBody::UpdateMomenta,
-// PerformStep, UpdateRotationMatrix (src/cpu/RigidBody.cpp:55-160): one block
-// per body.  The block loads RB_CHUNK members (in slot order) at a time and
-// computes each member's force, torque and weight terms in parallel; six
-// lanes then run the six sums -- F.x, F.y, F.z, T.x, T.y, T.z -- each in
-// member order with the reference's additions (F.x + f.x + 0, F.y + f.y +
-// gy, F.z + f.z + 0, T + t), so every sum has the serial loop's rounding.
-constexpr int RB_CHUNK = 256;
+// The force, torque and weight terms of every body member of
+// CalculateRigidBodyForceTorqueMomenta (:3620-3703), one thread per member
+// position k of the members grouped by body (ascending slots within a body):
+// terms[c * ld + k] for f.x f.y f.z t.x t.y t.z gy.
+__global__ void k_body_terms(int n, int nb, const uint32_t* __restrict__ keys, const uint32_t* __restrict__ members,
+                             const upf_body* __restrict__ bodies, const float4* __restrict__ pos,
+                             const float4* __restrict__ force, const float4* __restrict__ rel, upf_params P,
+                             float* __restrict__ terms, int ld) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t b = keys[k];
+    if (b >= (uint32_t)nb) return;
+    const int i = (int)members[k];
+    const float4 f = force[i];
+    const float4 x = pos[i];
+    const float r0 = x.x - bodies[b].pos[0], r1 = x.y - bodies[b].pos[1], r2 = x.z - bodies[b].pos[2];
+    terms[0 * (size_t)ld + k] = f.x;
+    terms[1 * (size_t)ld + k] = f.y;
+    terms[2 * (size_t)ld + k] = f.z;
+    terms[3 * (size_t)ld + k] = r1 * f.z - r2 * f.y;
+    terms[4 * (size_t)ld + k] = r2 * f.x - r0 * f.z;
+    terms[5 * (size_t)ld + k] = r0 * f.y - r1 * f.x;
+    terms[6 * (size_t)ld + k] = -rel[i].w * P.gravity;
+}
+
+// The sums of CalculateRigidBodyForceTorqueMomenta with RigidBody::UpdateMomenta,
+// PerformStep, UpdateRotationMatrix (src/cpu/RigidBody.cpp:55-160): one wave
+// per body.  The wave loads the terms of 64 members at a time (coalesced,
+// RB_DEPTH batches in flight) into LDS; six lanes then run the six sums --
+// F.x, F.y, F.z, T.x, T.y, T.z -- each in member order with the reference's
+// additions (F.x + f.x + 0, F.y + f.y + gy, F.z + f.z + 0, T + t), so every
+// sum has the serial loop's rounding.  The kernel takes as long as the
+// largest body's F.y chain (two dependent additions per member).
+constexpr int RB_CHUNK = 64;
+constexpr int RB_DEPTH = 4;
 __global__ __launch_bounds__(RB_CHUNK) void k_rigid_bodies(int nb, upf_body* __restrict__ bodies,
                                                            const int* __restrict__ start,
-                                                           const uint32_t* __restrict__ members,
-                                                           const float4* __restrict__ pos,
-                                                           const float4* __restrict__ force,
-                                                           const float4* __restrict__ rel, upf_params P) {
+                                                           const float* __restrict__ terms, int ld, upf_params P) {
     const int b = blockIdx.x;
     if (b >= nb) return;
-    // f.x f.y f.z t.x t.y t.z gy of a chunk's members, two chunks in flight
-    __shared__ float terms[2][7][RB_CHUNK];
+    __shared__ float tt[8][RB_CHUNK];  // the seven term rows and a row of +0
     __shared__ float sums[6];
     __shared__ int range[2];
     const int tid = threadIdx.x;
@@ -2137,55 +2160,50 @@ __global__ __launch_bounds__(RB_CHUNK) void k_rigid_bodies(int nb, upf_body* __r
     }
     __syncthreads();
     const int s0 = range[0], s1 = range[1];
-    const float bx = bodies[b].pos[0], by = bodies[b].pos[1], bz = bodies[b].pos[2];
-    // the terms of member k0 + tid; past the body's last member every term
-    // is +0, which leaves a sum unchanged (a sum from +0 is never -0)
-    auto load_chunk = [&](int k0, int buf) {
-        const int k = k0 + tid;
-        float t[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        if (k < s1) {
-            const int i = (int)members[k];
-            const float4 f = force[i];
-            const float4 x = pos[i];
-            const float r0 = x.x - bx, r1 = x.y - by, r2 = x.z - bz;
-            t[0] = f.x;
-            t[1] = f.y;
-            t[2] = f.z;
-            t[3] = r1 * f.z - r2 * f.y;
-            t[4] = r2 * f.x - r0 * f.z;
-            t[5] = r0 * f.y - r1 * f.x;
-            t[6] = -rel[i].w * P.gravity;
-        }
+    // member k0 + tid's seven terms (a clamped index past the body's end;
+    // those terms are replaced by +0 before they are added)
+    auto load = [&](int k0, float* out) __attribute__((always_inline)) {
+        const int k = min(k0 + tid, s1 - 1);
 #pragma unroll
-        for (int c = 0; c < 7; ++c) terms[buf][c][tid] = t[c];
+        for (int c = 0; c < 7; ++c) out[c] = terms[(size_t)c * ld + k];
     };
+    float r[RB_DEPTH][7];
+    tt[7][tid] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < RB_DEPTH; ++d)
+        if (s0 + d * RB_CHUNK < s1) load(s0 + d * RB_CHUNK, r[d]);
     float acc = 0.0f;  // lane a < 6: sum a
-    if (s0 < s1) load_chunk(s0, 0);
-    __syncthreads();
-    int buf = 0;
-    for (int k0 = s0; k0 < s1; k0 += RB_CHUNK, buf ^= 1) {
-        // the next chunk's loads go out before this chunk's sums
-        if (k0 + RB_CHUNK < s1) load_chunk(k0 + RB_CHUNK, buf ^ 1);
-        if (tid < 6) {
-            // sixteen members' terms at a time into registers (their LDS
-            // reads overlap), then the additions in member order
-            const int m = min(RB_CHUNK, s1 - k0);
-            const bool second = tid < 3;  // F sums add a second term per member
-            for (int u0 = 0; u0 < m; u0 += 16) {
-                float v[16], g[16];
+    const int gy_row = tid == 1 ? 6 : 7;  // row 7: zeros
+    for (int kb = s0; kb < s1; kb += RB_DEPTH * RB_CHUNK) {
 #pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    v[q] = terms[buf][tid][u0 + q];
-                    g[q] = tid == 1 ? terms[buf][6][u0 + q] : 0.0f;
-                }
+        for (int d = 0; d < RB_DEPTH; ++d) {
+            const int k0 = kb + d * RB_CHUNK;
+            if (k0 >= s1) break;
+            // past the body's last member a term is +0, which leaves a sum
+            // unchanged (a sum from +0 is never -0)
+            const bool in = k0 + tid < s1;
 #pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    acc = acc + v[q];
-                    if (second) acc = acc + g[q];
+            for (int c = 0; c < 7; ++c) tt[c][tid] = in ? r[d][c] : 0.0f;
+            if (k0 + RB_DEPTH * RB_CHUNK < s1) load(k0 + RB_DEPTH * RB_CHUNK, r[d]);
+            __syncthreads();
+            if (tid < 6) {
+                // every lane adds a second term per member: gy for F.y, +0
+                // for the others (a no-op on a sum that is never -0), which
+                // keeps the six dependency chains free of selects
+                const int m = min(RB_CHUNK, s1 - k0);
+                for (int u0 = 0; u0 < m; u0 += 16) {
+                    float v[16], g[16];
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        v[q] = tt[tid][u0 + q];
+                        g[q] = tt[gy_row][u0 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) acc = (acc + v[q]) + g[q];
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
     }
     if (tid < 6) sums[tid] = acc;
     __syncthreads();
@@ -2446,6 +2464,7 @@ struct upf_sim {
     uint32_t* d_body_keys = nullptr;   // scratch: body key per slot
     uint32_t* d_body_keys2 = nullptr;
     uint32_t* d_body_members = nullptr;  // slots grouped by body, ascending within a body
+    float* d_body_terms = nullptr;       // 7 rows of cap: the members' force, torque and weight terms
 
     StepArgs args() const {
         StepArgs A;
@@ -2878,8 +2897,10 @@ static int step_pcisph_rigid(upf_sim* s) {
     HIPCHK(radix_sort(s->d_body_keys, nullptr, s->d_body_keys2, s->d_body_members, s->n, body_bits, s->sort_tmp, s->st));
     HIPCHK(hipMemsetAsync(s->d_body_start, 0xFF, (size_t)(s->n_bodies + 1) * sizeof(int), s->st));
     hipLaunchKernelGGL(k_body_bounds, g, dim3(BLOCK), 0, s->st, s->n, s->d_body_keys2, s->n_bodies, s->d_body_start);
+    hipLaunchKernelGGL(k_body_terms, g, dim3(BLOCK), 0, s->st, s->n, s->n_bodies, s->d_body_keys2, s->d_body_members,
+                       s->d_bodies, S.pos, s->W.force, S.rel, s->P, s->d_body_terms, s->cap);
     hipLaunchKernelGGL(k_rigid_bodies, dim3(s->n_bodies), dim3(RB_CHUNK), 0, s->st, s->n_bodies, s->d_bodies,
-                       s->d_body_start, s->d_body_members, S.pos, s->W.force, S.rel, s->P);
+                       s->d_body_start, s->d_body_terms, s->cap, s->P);
     hipLaunchKernelGGL(k_rigid_sync, g, dim3(BLOCK), 0, s->st, s->n, S, s->d_bodies);
     s->prof.end(s->st);
     HIPCHK(hipGetLastError());
@@ -3400,7 +3421,8 @@ int upf_upload_bodies(upf_sim* s, int n_bodies, const upf_body* bodies, const in
               hipMalloc((void**)&d_start, (size_t)(n_bodies + 1) * sizeof(int)) == hipSuccess;
     if (ok && !s->d_body_keys)
         ok = s->alloc(&s->d_body_keys, s->cap) == hipSuccess && s->alloc(&s->d_body_keys2, s->cap) == hipSuccess &&
-             s->alloc(&s->d_body_members, s->cap) == hipSuccess;
+             s->alloc(&s->d_body_members, s->cap) == hipSuccess &&
+             s->alloc(&s->d_body_terms, (size_t)7 * s->cap) == hipSuccess;
     ok = ok && hipMemcpy(d_b, bodies, (size_t)n_bodies * sizeof(upf_body), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(S.body, bid.data(), (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(S.rel, rel.data(), (size_t)n * 16, hipMemcpyHostToDevice) == hipSuccess &&
