@@ -1912,11 +1912,9 @@ __device__ __forceinline__ void ext_coupling_slot(const StepArgs& A, const Slots
     const float own_psi = S.psi[i];
     ListEncoder enc(W, i, (size_t)A.stride);
     const bool code = A.code_lists != 0;
-    for (int k = 0; k < cnt; ++k) {
-        const uint32_t e = W.nbr[(size_t)k * A.stride + i];
+    // one neighbour (entry e, record xj)
+    auto pair = [&](uint32_t e, const float4& xj) __attribute__((always_inline)) {
         const int j = (int)(e >> NB_SLOT_SHIFT);
-        if (code) enc.put(k, (uint32_t)j, ctl);
-        const float4 xj = S.pos[j];
         const int tj = ptype(xj);
         const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
         const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
@@ -1985,6 +1983,31 @@ __device__ __forceinline__ void ext_coupling_slot(const StepArgs& A, const Slots
                 }
             }
         }
+    };
+    // list entries in groups of four, the next group's loaded ahead; the next
+    // neighbour's record is loaded before this neighbour's pair runs
+    const size_t st = (size_t)A.stride;
+    uint32_t e[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) e[u] = u < cnt ? W.nbr[(size_t)u * st + i] : 0u;
+    float4 xn = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (cnt > 0) xn = S.pos[e[0] >> NB_SLOT_SHIFT];
+    for (int k0 = 0; k0 < cnt; k0 += 4) {
+        uint32_t en[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) en[u] = k0 + 4 + u < cnt ? W.nbr[(size_t)(k0 + 4 + u) * st + i] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int k = k0 + u;
+            if (k < cnt) {
+                const float4 xj = xn;
+                if (k + 1 < cnt) xn = S.pos[(u < 3 ? e[u + 1] : en[0]) >> NB_SLOT_SHIFT];
+                if (code) enc.put(k, e[u] >> NB_SLOT_SHIFT, ctl);
+                pair(e[u], xj);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) e[u] = en[u];
     }
     if (code) enc.finish();
     if (ti == UPF_LIQUID) {
