@@ -644,10 +644,12 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
 // which keeps the LDS free for more waves).
+constexpr int CAND_CAP = 512;  // stencil candidates held in the LDS candidate table
 template <int ROUNDS, int STAGE, int BIN, int SKIP, int KEEP_RIGID = 0, int OWNER = UPF_LIQUID>
 __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __restrict__ pos,
                                            const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
-                                           uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
+                                           uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl,
+                                           uint32_t* __restrict__ cand) {
     constexpr int NB_CHUNK = 64 * ROUNDS;
     // cell coordinates and query boxes are packed in 10-bit fields (x, y, z)
     __shared__ uint32_t r_start[32], r_pref[33], r_cell[32];
@@ -754,6 +756,18 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
         if (lane == 0) r_pref[0] = 0;
         __syncthreads();
         const int M = (int)r_pref[nr];
+        // candidate table (cand, CAND_CAP entries of LDS): slot << 5 | cell rank
+        // of every candidate, written by the lanes of the occupied cells, so
+        // that each round reads its candidates at once; a stencil with more
+        // candidates finds their cells by a walk over the prefix sums
+        const bool table = M <= CAND_CAP;
+        if (table) {
+            const uint32_t len = valid ? rl : 0u;
+            const int beg = (int)(incl - len);  // r_pref[rank] of a valid lane
+            for (uint32_t t = 0; __ballot(t < len) != 0ull; ++t)
+                if (t < len) cand[beg + (int)t] = ((rs + t) << 5) | (uint32_t)rank;
+            __syncthreads();
+        }
         // Which occupied stencil cells (bit k: rank k) lie in the query box of
         // this lane's particle -- the box test of every candidate of the cell,
         // done once per cell: lane k < nr holds rank k's cell; per axis, the
@@ -791,8 +805,14 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
                 cbit[r] = 0u;
                 cx[r] = make_float4(__int_as_float(0x7f800000), 0.f, 0.f, 0.f);
                 if (ci < M) {
-                    while (k + 1 < nr && (int)r_pref[k + 1] <= ci) ++k;
-                    cj[r] = (int)r_start[k] + (ci - (int)r_pref[k]);
+                    if (table) {
+                        const uint32_t v = cand[ci];
+                        cj[r] = (int)(v >> 5);
+                        k = (int)(v & 31u);
+                    } else {
+                        while (k + 1 < nr && (int)r_pref[k + 1] <= ci) ++k;
+                        cj[r] = (int)r_start[k] + (ci - (int)r_pref[k]);
+                    }
                     cbit[r] = 1u << k;
                     cx[r] = pos[cj[r]];
                 }
@@ -870,7 +890,8 @@ template <int ROUNDS, int STAGE, int BIN, int SKIP = 0>
 __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __restrict__ pos,
                                                   const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                                   uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl) {
-    build_wave<ROUNDS, STAGE, BIN, SKIP>(A, pos, keys, cells, nbr, ncount, ctl);
+    __shared__ uint32_t cand[CAND_CAP];
+    build_wave<ROUNDS, STAGE, BIN, SKIP>(A, pos, keys, cells, nbr, ncount, ctl, cand);
 }
 
 // The lists of the rigid slots for the coupling step (GetNeighbors of a rigid
@@ -881,7 +902,8 @@ __global__ __launch_bounds__(64) void k_nbr_build_rigid(StepArgs A, const float4
                                                         const uint32_t* __restrict__ keys,
                                                         const uint2* __restrict__ cells, uint32_t* __restrict__ nbr,
                                                         int* __restrict__ ncount, Ctl* ctl) {
-    build_wave<ROUNDS, 0, 1, 0, 0, UPF_RIGID>(A, pos, keys, cells, nbr, ncount, ctl);
+    __shared__ uint32_t cand[CAND_CAP];
+    build_wave<ROUNDS, 0, 1, 0, 0, UPF_RIGID>(A, pos, keys, cells, nbr, ncount, ctl, cand);
 }
 
 // GetNeighbors again, tile form: one wave per 64 consecutive slots of one
@@ -1242,16 +1264,22 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
 template <int ROUNDS, int SKIP, int MODE>
 __global__ __launch_bounds__(64) void k_nbr_build_ext(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                       Ctl* ctl) {
-    __shared__ float lvl[LUT_N], lpg[LUT_N];
-    __shared__ float2 th[LUT_N];
+    // the build's candidate table, then the tables of the force pass
+    __shared__ float4 tabs[LUT_N];
+    static_assert(4 * LUT_N >= CAND_CAP, "candidate table");
+    build_wave<ROUNDS, 0, 0, SKIP>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl,
+                                   reinterpret_cast<uint32_t*>(tabs));
+    // the block is the wave: its lists (written by all its lanes) are complete
+    // and visible to every lane after the barrier, and so are the tables
+    __syncthreads();
+    float* lvl = reinterpret_cast<float*>(tabs);
+    float* lpg = lvl + LUT_N;
+    float2* th = reinterpret_cast<float2*>(lvl + 2 * LUT_N);
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
         lvl[t] = luts[2 * LUT_N + t];
         lpg[t] = luts[LUT_N + t];
         th[t] = reinterpret_cast<const float2*>(luts + LUT_TH_OFF)[t];
     }
-    build_wave<ROUNDS, 0, 0, SKIP>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl);
-    // the block is the wave: its lists (written by all its lanes) and the
-    // tables are complete and visible to every lane after the barrier
     __syncthreads();
     const int i = xblk(A.xcd) * 64 + (int)threadIdx.x;
     if (i < A.n) ext_static_slot<MODE>(A, S, W, i, lvl, lpg, th, ctl);
@@ -1752,10 +1780,13 @@ __global__ __launch_bounds__(BLOCK) void k_density_wcsph(StepArgs A, Slots S, Wo
 template <int ROUNDS, bool TAIT>
 __global__ __launch_bounds__(64) void k_nbr_build_wcsph(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                         Ctl* ctl) {
-    __shared__ float lw[LUT_N];
+    // the build's candidate table, then the density table
+    __shared__ float lw[LUT_N > CAND_CAP ? LUT_N : CAND_CAP];
+    build_wave<ROUNDS, 0, 1, 0>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl,
+                                reinterpret_cast<uint32_t*>(lw));
+    __syncthreads();  // the block is the wave: its lists are complete
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
-    build_wave<ROUNDS, 0, 1, 0>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl);
-    __syncthreads();  // the block is the wave: its lists and the table are complete
+    __syncthreads();
     const int i = xblk(A.xcd) * 64 + (int)threadIdx.x;
     if (i < A.n) density_wcsph_slot<TAIT>(A, S, W, i, lw);
 }
@@ -2086,15 +2117,23 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
 template <int ROUNDS, int MODE>
 __global__ __launch_bounds__(64) void k_nbr_build_coupling(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                            Ctl* ctl) {
-    __shared__ float lvl[LUT_N], lpg[LUT_N], lst[LUT_N], lad[LUT_N];
+    // the build's candidate table, then the tables of the coupling pass
+    __shared__ float tabs[4 * LUT_N];
+    static_assert(4 * LUT_N >= CAND_CAP, "candidate table");
+    build_wave<ROUNDS, 0, 1, 0, 1>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl,
+                                   reinterpret_cast<uint32_t*>(tabs));
+    __syncthreads();  // the block is the wave: its lists are complete
+    float* lvl = tabs;
+    float* lpg = tabs + LUT_N;
+    float* lst = tabs + 2 * LUT_N;
+    float* lad = tabs + 3 * LUT_N;
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) {
         lpg[t] = luts[LUT_N + t];
         lvl[t] = luts[2 * LUT_N + t];
         lst[t] = luts[3 * LUT_N + t];
         lad[t] = luts[4 * LUT_N + t];
     }
-    build_wave<ROUNDS, 0, 1, 0, 1>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl);
-    __syncthreads();  // the block is the wave: its lists and the tables are complete
+    __syncthreads();
     const int i = xblk(A.xcd) * 64 + (int)threadIdx.x;
     if (i < A.n) ext_coupling_slot<MODE>(A, S, W, i, lvl, lpg, lst, lad, ctl);
 }
