@@ -76,6 +76,11 @@ __device__ inline float wall_weight(const upf_params& P, float dist, float inv_h
 
 __device__ inline int ptype(const float4& x) { return __float_as_int(x.w); }
 
+// The wave's lanes (among the active ones) whose predicate holds.  The
+// builtin takes the predicate as a condition, so a compare result is the mask
+// itself (no select to an integer and compare back).
+__device__ __forceinline__ uint64_t wballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 __device__ inline void set_err(Ctl* ctl, int code) { atomicOr(&ctl->err, 1u << code); }
 
 struct StepArgs {
@@ -694,7 +699,7 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
             my_bhi |= u << (10 * a);
         }
     }
-    const uint64_t liquid = __ballot(my_liquid);
+    const uint64_t liquid = wballot(my_liquid);
     int my_cnt = 0;  // this lane's slot: list length so far
     int s = base;
     while (s < wend) {
@@ -706,7 +711,7 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
         const uint2* T3 = cells + d.cell_off;
         const uint32_t c = (uint32_t)__shfl((int)my_key, sl, 64) >> shift3;
         // end of the cell within the wave: first later slot of another cell or scene
-        const uint64_t other = __ballot(mine_valid && me > s && ((my_key >> shift3) != c || my_scene != sc));
+        const uint64_t other = wballot(mine_valid && me > s && ((my_key >> shift3) != c || my_scene != sc));
         const int pe = other ? base + __ffsll((long long)other) - 1 : wend;
         // the cell's liquid slots (bits sl .. pe - base - 1)
         const uint64_t upto = (pe - base) >= 64 ? ~0ull : ((1ull << (pe - base)) - 1ull);
@@ -743,7 +748,7 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
             }
         }
         const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const uint64_t vm = __ballot(valid);
+        const uint64_t vm = wballot(valid);
         const int nr = __popcll(vm);
         const int rank = __popcll(vm & lt);
         // prefix of the occupied cells' lengths in rank (= lane) order
@@ -764,7 +769,7 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
         if (table) {
             const uint32_t len = valid ? rl : 0u;
             const int beg = (int)(incl - len);  // r_pref[rank] of a valid lane
-            for (uint32_t t = 0; __ballot(t < len) != 0ull; ++t)
+            for (uint32_t t = 0; wballot(t < len) != 0ull; ++t)
                 if (t < len) cand[beg + (int)t] = ((rs + t) << 5) | (uint32_t)rank;
             __syncthreads();
         }
@@ -780,9 +785,9 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const uint32_t qa = (qk >> (10 * a)) & 0x3FFu;
-                const uint32_t below = (uint32_t)__ballot(lane < nr && qa < cc[a]);
-                const uint32_t above = (uint32_t)__ballot(lane < nr && qa > cc[a]);
-                const uint32_t at = (uint32_t)__ballot(lane < nr && qa == cc[a]);
+                const uint32_t below = (uint32_t)wballot(lane < nr && qa < cc[a]);
+                const uint32_t above = (uint32_t)wballot(lane < nr && qa > cc[a]);
+                const uint32_t at = (uint32_t)wballot(lane < nr && qa == cc[a]);
                 const uint32_t l = (my_blo >> (10 * a)) & 0x3FFu, u = (my_bhi >> (10 * a)) & 0x3FFu;
                 // stencil coordinates are cc - 1, cc, cc + 1 (cc - 1 only when cc > 0)
                 if (!(cc[a] > 0 && l <= cc[a] - 1 && cc[a] - 1 <= u)) in_box &= ~below;
@@ -843,8 +848,10 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
                     const bool inbox = (boxm & cbit[r]) != 0u;
                     const float dx = px - cx[r].x, dy = py - cx[r].y, dz = pz - cx[r].z;
                     const float d2 = dx * dx + dy * dy + dz * dz;
-                    const bool pass = inbox & (d2 < T) & (cj[r] != p);
-                    const uint64_t m = __ballot(pass);
+                    const bool near = d2 < T, other = cj[r] != p;
+                    const bool pass = inbox & near & other;
+                    // the AND of three compare masks (each ballot is its compare)
+                    const uint64_t m = wballot(inbox) & wballot(near) & wballot(other);
                     if (m == 0ull) continue;
                     if (pass) {
                         const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -956,7 +963,7 @@ __global__ __launch_bounds__(64) void k_nbr_build_tile(StepArgs A, const float4*
         const Dom d = dom_at(A, s);
         const uint32_t shift3 = 3u * d.block_shift, G = d.grid_dim;
         const uint2* T3 = cells + d.cell_off;
-        const uint64_t other = __ballot(valid && p >= s && my_scene != sc);
+        const uint64_t other = wballot(valid && p >= s && my_scene != sc);
         int se = other ? base + __ffsll((long long)other) - 1 : wend;
         // this lane's query box in cell coordinates (its own scene's grid)
         uint32_t blo[3] = {0u, 0u, 0u}, bhi[3] = {0u, 0u, 0u};
@@ -1011,7 +1018,7 @@ __global__ __launch_bounds__(64) void k_nbr_build_tile(StepArgs A, const float4*
                     occ = rg.y > rg.x;
                     pk = qx | (qy << 10) | (qz << 20);
                 }
-                const uint64_t m = __ballot(occ);
+                const uint64_t m = wballot(occ);
                 if (occ) {
                     const int k = n_occ + __popcll(m & lt);
                     o_start[k] = rg.x;
@@ -1407,7 +1414,7 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
                 int sc = -1;
                 if (i < A.chunk_end[c]) density_of(i, -1, mine, sc);
                 // a chunk holds one scene: one maximum per wave
-                const uint64_t act = __ballot(sc >= 0);
+                const uint64_t act = wballot(sc >= 0);
                 if (act == 0ull) continue;
                 const int first = __shfl(sc, __ffsll((long long)act) - 1, 64);
                 const float mx = wave_max(mine);
@@ -1436,10 +1443,10 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
             continue;
         }
         // several scenes: reduce per wave when the wave holds one scene
-        const uint64_t act = __ballot(sc >= 0);
+        const uint64_t act = wballot(sc >= 0);
         if (act == 0ull) continue;
         const int first = __shfl(sc, __ffsll((long long)act) - 1, 64);
-        const bool uniform = __ballot(sc >= 0 && sc != first) == 0ull;
+        const bool uniform = wballot(sc >= 0 && sc != first) == 0ull;
         if (uniform) {
             const float m = wave_max(mine);
             if (lane == 0)
