@@ -704,12 +704,14 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
     int s = base;
     while (s < wend) {
         const int sl = s - base;
-        const int sc = __shfl(my_scene, sl, 64);
+        // the cell's values are the same on every lane: read from lane sl into
+        // scalar registers, so that the cell's set-up runs on the scalar unit
+        const int sc = __builtin_amdgcn_readlane(my_scene, sl);
         const Dom d = A.scene_of ? A.doms[sc] : A.dom0;  // the cell's scene
         const uint32_t shift3 = 3u * d.block_shift;
         const uint32_t G = d.grid_dim;
         const uint2* T3 = cells + d.cell_off;
-        const uint32_t c = (uint32_t)__shfl((int)my_key, sl, 64) >> shift3;
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_key, sl) >> shift3;
         // end of the cell within the wave: first later slot of another cell or scene
         const uint64_t other = wballot(mine_valid && me > s && ((my_key >> shift3) != c || my_scene != sc));
         const int pe = other ? base + __ffsll((long long)other) - 1 : wend;
