@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "upf.h"
@@ -92,6 +93,7 @@ struct StepArgs {
     float dens_sq;       // rho0 * rho0
     int n;
     int stride;          // column stride of the neighbour lists
+    int nbr_ld;          // row pitch of the slot-major packed lists (max_neighbors rounded up to 4)
     Dom dom0;            // the domain when there is one scene
     const Dom* doms;     // per-scene domains (batched form), else null
     const int* scene_of; // scene of every slot (batched form), else null
@@ -648,9 +650,12 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // slots are staged in LDS (row k, one column per slot, padded against bank
 // conflicts) and leave in full 256-byte rows of the column-major layout;
 // entries past the first STAGE of a list are written directly (STAGE 0: all,
-// which keeps the LDS free for more waves).
+// which keeps the LDS free for more waves).  SLOT_MAJOR (STAGE 0): each list
+// in its own row of A.nbr_ld words instead, where a round's entries are
+// consecutive words (for a reader in the same kernel).
 constexpr int CAND_CAP = 512;  // stencil candidates held in the LDS candidate table
-template <int ROUNDS, int STAGE, int BIN, int SKIP, int KEEP_RIGID = 0, int OWNER = UPF_LIQUID>
+template <int ROUNDS, int STAGE, int BIN, int SKIP, int KEEP_RIGID = 0, int OWNER = UPF_LIQUID,
+          bool SLOT_MAJOR = false>
 __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __restrict__ pos,
                                            const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                            uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl,
@@ -865,6 +870,10 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
                         if constexpr (STAGE > 0) {
                             if (slot < STAGE) stage[slot * 65 + pl] = e;
                             else if (slot < cap) nbr[(size_t)slot * stride + p] = e;
+                        } else if constexpr (SLOT_MAJOR) {
+                            // the particle's own row: the round's entries are
+                            // consecutive words
+                            if (slot < cap) nbr[(size_t)p * A.nbr_ld + slot] = e;
                         } else {
                             // row cnt of the list column, then `below` rows on
                             // (slots < 2^23: the offset fits 24-bit multiplies)
@@ -1134,7 +1143,7 @@ __device__ __forceinline__ float4 boundary_xs(float4 x, float psi, float rho0) {
 // and for a liquid slot the forces over its list, the first predicted
 // position and the delta-coded copy of the list.  lvl, lpg, th: the
 // viscosity and pressure-gradient tables and the bin thresholds in LDS.
-template <int MODE>
+template <int MODE, bool SLOT_MAJOR = false>
 __device__ __forceinline__ void ext_static_slot(const StepArgs& A, const Slots& S, const Work& W, int i,
                                                 const float* lvl, const float* lpg, const float2* th, Ctl* ctl) {
     const upf_params& P = A.P;
@@ -1159,9 +1168,10 @@ __device__ __forceinline__ void ext_static_slot(const StepArgs& A, const Slots& 
     const float own_psi = S.psi[i];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     const int cnt = W.ncount[i];
-    const uint32_t* L = W.nbr + i;
+    // the packed list: column-major, or (SLOT_MAJOR) the slot's own row
+    const uint32_t* L = SLOT_MAJOR ? W.nbr + (size_t)i * A.nbr_ld : W.nbr + i;
     const bool bin_free = A.code_lists != 0;  // the lists of this step carry no bins
-    const size_t st = (size_t)A.stride;
+    const size_t st = (size_t)A.stride;  // column stride (the coded lists; column-major packed lists)
     auto pair = [&](uint32_t e, const float4& xj, const float4& vj) __attribute__((always_inline)) {
         const int tj = ptype(xj);
         if (tj == UPF_LIQUID) {
@@ -1203,14 +1213,27 @@ __device__ __forceinline__ void ext_static_slot(const StepArgs& A, const Slots& 
     const bool code = A.code_lists != 0;
     // groups of four entries; the next group's entries are loaded before this
     // group's gathers, and the last, partial group's gathers go out together
-    uint32_t e[4];
+    // (a slot-major row holds its groups as aligned 16-byte words; entries
+    // past the list's end are loaded but never used)
+    auto group = [&](int k0, uint32_t* out) __attribute__((always_inline)) {
+        if constexpr (SLOT_MAJOR) {
+            uint4 q = make_uint4(0u, 0u, 0u, 0u);
+            if (k0 < cnt) q = *reinterpret_cast<const uint4*>(L + k0);
+            out[0] = q.x;
+            out[1] = q.y;
+            out[2] = q.z;
+            out[3] = q.w;
+        } else {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) e[u] = u < cnt ? L[u * st] : 0u;
+            for (int u = 0; u < 4; ++u) out[u] = k0 + u < cnt ? L[(size_t)(k0 + u) * st] : 0u;
+        }
+    };
+    uint32_t e[4];
+    group(0, e);
     int k = 0;
     for (; k + 4 <= cnt; k += 4) {
         uint32_t en[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) en[u] = k + 4 + u < cnt ? L[(k + 4 + u) * st] : 0u;
+        group(k + 4, en);
         float4 xj[4], vj[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -1270,14 +1293,14 @@ __global__ __launch_bounds__(BLOCK) void k_ext_static(StepArgs A, Slots S, Work 
 // wave just wrote.  Nothing crosses waves between the two parts, so the
 // memory-bound force work of some waves overlaps the VALU-bound list work of
 // others, and the lists are read back while they are still in the cache.
-template <int ROUNDS, int SKIP, int MODE>
+template <int ROUNDS, int SKIP, int MODE, bool SLOT_MAJOR>
 __global__ __launch_bounds__(64) void k_nbr_build_ext(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                       Ctl* ctl) {
     // the build's candidate table, then the tables of the force pass
     __shared__ float4 tabs[LUT_N];
     static_assert(4 * LUT_N >= CAND_CAP, "candidate table");
-    build_wave<ROUNDS, 0, 0, SKIP>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl,
-                                   reinterpret_cast<uint32_t*>(tabs));
+    build_wave<ROUNDS, 0, 0, SKIP, 0, UPF_LIQUID, SLOT_MAJOR>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl,
+                                                             reinterpret_cast<uint32_t*>(tabs));
     // the block is the wave: its lists (written by all its lanes) are complete
     // and visible to every lane after the barrier, and so are the tables
     __syncthreads();
@@ -1291,7 +1314,7 @@ __global__ __launch_bounds__(64) void k_nbr_build_ext(StepArgs A, Slots S, Work 
     }
     __syncthreads();
     const int i = xblk(A.xcd) * 64 + (int)threadIdx.x;
-    if (i < A.n) ext_static_slot<MODE>(A, S, W, i, lvl, lpg, th, ctl);
+    if (i < A.n) ext_static_slot<MODE, SLOT_MAJOR>(A, S, W, i, lvl, lpg, th, ctl);
 }
 
 // ------------------------------------------------------ PCISPH iteration
@@ -2547,6 +2570,7 @@ struct upf_sim {
         A.dens_sq = P.rest_density * P.rest_density;
         A.n = n;
         A.stride = cap;
+        A.nbr_ld = (P.max_neighbors + 3) & ~3;
         A.dom0.scales[0] = P.scales[0];
         A.dom0.scales[1] = P.scales[1];
         A.dom0.scales[2] = P.scales[2];
@@ -2818,15 +2842,22 @@ static int step_static_head(upf_sim* s) {
         const dim3 gb((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
         auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, s->st, A, S, s->W, s->luts, s->ctl); };
         const int r = s->tune.nbr_rounds, k = s->tune.nbr_skip;
-        if (akinci) {
-            if (r == 2) k ? launch(k_nbr_build_ext<2, 1, UPF_BOUNDARY_AKINCI>) : launch(k_nbr_build_ext<2, 0, UPF_BOUNDARY_AKINCI>);
-            else if (r == 8) k ? launch(k_nbr_build_ext<8, 1, UPF_BOUNDARY_AKINCI>) : launch(k_nbr_build_ext<8, 0, UPF_BOUNDARY_AKINCI>);
-            else k ? launch(k_nbr_build_ext<4, 1, UPF_BOUNDARY_AKINCI>) : launch(k_nbr_build_ext<4, 0, UPF_BOUNDARY_AKINCI>);
-        } else {
-            if (r == 2) k ? launch(k_nbr_build_ext<2, 1, UPF_BOUNDARY_CPU_REF>) : launch(k_nbr_build_ext<2, 0, UPF_BOUNDARY_CPU_REF>);
-            else if (r == 8) k ? launch(k_nbr_build_ext<8, 1, UPF_BOUNDARY_CPU_REF>) : launch(k_nbr_build_ext<8, 0, UPF_BOUNDARY_CPU_REF>);
-            else k ? launch(k_nbr_build_ext<4, 1, UPF_BOUNDARY_CPU_REF>) : launch(k_nbr_build_ext<4, 0, UPF_BOUNDARY_CPU_REF>);
-        }
+        // slot-major packed lists when only this kernel reads them (the delta-
+        // coded loop lists); the chunk unions read the column-major ones
+        auto go = [&](auto sm) {
+            constexpr bool SM = decltype(sm)::value;
+            if (akinci) {
+                if (r == 2) k ? launch(k_nbr_build_ext<2, 1, UPF_BOUNDARY_AKINCI, SM>) : launch(k_nbr_build_ext<2, 0, UPF_BOUNDARY_AKINCI, SM>);
+                else if (r == 8) k ? launch(k_nbr_build_ext<8, 1, UPF_BOUNDARY_AKINCI, SM>) : launch(k_nbr_build_ext<8, 0, UPF_BOUNDARY_AKINCI, SM>);
+                else k ? launch(k_nbr_build_ext<4, 1, UPF_BOUNDARY_AKINCI, SM>) : launch(k_nbr_build_ext<4, 0, UPF_BOUNDARY_AKINCI, SM>);
+            } else {
+                if (r == 2) k ? launch(k_nbr_build_ext<2, 1, UPF_BOUNDARY_CPU_REF, SM>) : launch(k_nbr_build_ext<2, 0, UPF_BOUNDARY_CPU_REF, SM>);
+                else if (r == 8) k ? launch(k_nbr_build_ext<8, 1, UPF_BOUNDARY_CPU_REF, SM>) : launch(k_nbr_build_ext<8, 0, UPF_BOUNDARY_CPU_REF, SM>);
+                else k ? launch(k_nbr_build_ext<4, 1, UPF_BOUNDARY_CPU_REF, SM>) : launch(k_nbr_build_ext<4, 0, UPF_BOUNDARY_CPU_REF, SM>);
+            }
+        };
+        if (s->tune.loop_list == 1) go(std::true_type{});
+        else go(std::false_type{});
         s->prof.end(s->st);
     } else {
         s->static_lists = true;
@@ -3086,7 +3117,9 @@ static int sim_alloc(upf_sim* s, int capacity, uint32_t ncells, int n_scenes) {
     HIPCHK(s->alloc(&W.vpg, capacity));
     HIPCHK(s->alloc(&W.max_slots, nslots));
     HIPCHK(s->alloc(&W.ncount, capacity));
-    HIPCHK(s->alloc(&W.nbr, (size_t)capacity * p->max_neighbors));
+    // column-major (k * capacity + slot) or, for the fused build, slot-major
+    // (slot * pitch + k, the pitch rounded up to four entries)
+    HIPCHK(s->alloc(&W.nbr, (size_t)capacity * ((p->max_neighbors + 3) & ~3)));
     HIPCHK(s->alloc(&W.nbw, (size_t)capacity * ((p->max_neighbors + 7) / 8)));
     HIPCHK(s->alloc(&W.nbf, capacity));
     HIPCHK(s->alloc(&W.nbe, (size_t)capacity * NB_ESC));
