@@ -182,16 +182,24 @@ def test_fused_list_build_for_wcsph_and_coupling_equals_separate_kernels(gpu):
 def test_rigid_lists_by_the_wave_build_equal_the_per_particle_build(gpu):
     """The rigid slots' lists of the coupling step come from the wave-cooperative
     build (nbr_tile 0) or from one thread per rigid slot over its query box
-    (nbr_tile 256): both boundary modes step exactly alike."""
+    (nbr_tile 256): both boundary modes step exactly alike.  A rigid particle's
+    sort key is that of its position before the last rigid update; with the
+    bodies thrown sideways at 3 m/s (3 mm per step, a seventh of a cell) into
+    the container's walls, rigid particles' query boxes reach cells two away
+    from their key's, where wall particles lie within h, so the build must
+    centre their stencils on their current cells."""
     for mode in (upf.UPF_BOUNDARY_AKINCI, upf.UPF_BOUNDARY_CPU_REF):
         out = []
         for tile in (0, 256):
             p, pos, typ, bid, rel, bodies, counts = upf.standard_rigid_scene(16, 24, 16, n_bodies=4, mode=mode)
+            for k, b in enumerate(bodies):
+                b.vel[0] = 3.0 if k % 2 == 0 else -3.0
+                b.linear_momentum[0] = b.vel[0] * b.mass
             sim = upf.Simulation(p, len(typ))
             sim.set_tuning(dict(autotune.DEFAULT, nbr_tile=tile))
             sim.upload(pos, typ)
             sim.upload_bodies(bodies, bid, rel)
-            sim.step(4, upf.UPF_PCISPH_RIGID)
+            sim.step(60, upf.UPF_PCISPH_RIGID)
             out.append(sim.state())
             sim.close()
         print("mode %d: rigid lists, wave build vs per-particle build" % mode, flush=True)

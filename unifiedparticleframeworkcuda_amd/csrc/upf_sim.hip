@@ -704,6 +704,17 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
             my_bhi |= u << (10 * a);
         }
     }
+    // the key whose cell centres a slot's stencil: its sort key, or for a
+    // rigid owner -- keyed at its position before the last rigid update -- the
+    // key of its current position, whose cell's stencil holds its query box
+    uint32_t my_seg = my_key;
+    if constexpr (OWNER == UPF_RIGID) {
+        if (my_liquid) {
+            const Dom dm = A.scene_of ? A.doms[my_scene] : A.dom0;
+            bool ok;
+            my_seg = dkey(my_x.x, my_x.y, my_x.z, dm.scales, ok);
+        }
+    }
     const uint64_t liquid = wballot(my_liquid);
     int my_cnt = 0;  // this lane's slot: list length so far
     int s = base;
@@ -716,9 +727,9 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
         const uint32_t shift3 = 3u * d.block_shift;
         const uint32_t G = d.grid_dim;
         const uint2* T3 = cells + d.cell_off;
-        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_key, sl) >> shift3;
+        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)my_seg, sl) >> shift3;
         // end of the cell within the wave: first later slot of another cell or scene
-        const uint64_t other = wballot(mine_valid && me > s && ((my_key >> shift3) != c || my_scene != sc));
+        const uint64_t other = wballot(mine_valid && me > s && ((my_seg >> shift3) != c || my_scene != sc));
         const int pe = other ? base + __ffsll((long long)other) - 1 : wend;
         // the cell's liquid slots (bits sl .. pe - base - 1)
         const uint64_t upto = (pe - base) >= 64 ? ~0ull : ((1ull << (pe - base)) - 1ull);
@@ -915,13 +926,15 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
 // The lists of the rigid slots for the coupling step (GetNeighbors of a rigid
 // particle: every particle type, entries with their table bins), by the same
 // wave-cooperative build; the other slots' counts are left as they are.
-template <int ROUNDS>
+// SLOT_MAJOR: the lists in slot-major rows (as the fused coupling kernel
+// writes and reads the liquid ones).
+template <int ROUNDS, bool SLOT_MAJOR>
 __global__ __launch_bounds__(64) void k_nbr_build_rigid(StepArgs A, const float4* __restrict__ pos,
                                                         const uint32_t* __restrict__ keys,
                                                         const uint2* __restrict__ cells, uint32_t* __restrict__ nbr,
                                                         int* __restrict__ ncount, Ctl* ctl) {
     __shared__ uint32_t cand[CAND_CAP];
-    build_wave<ROUNDS, 0, 1, 0, 0, UPF_RIGID>(A, pos, keys, cells, nbr, ncount, ctl, cand);
+    build_wave<ROUNDS, 0, 1, 0, 0, UPF_RIGID, SLOT_MAJOR>(A, pos, keys, cells, nbr, ncount, ctl, cand);
 }
 
 // GetNeighbors again, tile form: one wave per 64 consecutive slots of one
@@ -1901,8 +1914,9 @@ __global__ __launch_bounds__(BLOCK) void k_nbr_build_box(StepArgs A, const float
 }
 
 // Boundary volumes of the rigid particles only (walls keep theirs).
-__global__ void k_psi_rigid(int n, const float4* __restrict__ pos, const uint32_t* __restrict__ nbr, int stride,
-                            const int* __restrict__ ncount, upf_params P, const float* __restrict__ luts,
+// (entry k of slot i at nbr[k * kstride + i * istride]: column-major or slot-major lists)
+__global__ void k_psi_rigid(int n, const float4* __restrict__ pos, const uint32_t* __restrict__ nbr, int kstride,
+                            int istride, const int* __restrict__ ncount, upf_params P, const float* __restrict__ luts,
                             float* __restrict__ psi, Ctl* ctl) {
     __shared__ float lw[LUT_N];
     for (int t = threadIdx.x; t < LUT_N; t += blockDim.x) lw[t] = luts[t];
@@ -1912,10 +1926,22 @@ __global__ void k_psi_rigid(int n, const float4* __restrict__ pos, const uint32_
     if (ptype(pos[i]) != UPF_RIGID) return;
     float inv = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? lw[0] : 0.0f;
     const int cnt = ncount[i];
-    for (int k = 0; k < cnt; ++k) {
-        const uint32_t e = nbr[(size_t)k * stride + i];
+    auto add = [&](uint32_t e) __attribute__((always_inline)) {
         const int tj = ptype(pos[e >> NB_SLOT_SHIFT]);
         if (tj == UPF_FROZEN || tj == UPF_RIGID) inv += lw[e & NB_BIN_MASK];
+    };
+    if (kstride == 1) {
+        // a slot-major row (its pitch a multiple of four): 16-byte groups
+        const uint32_t* L = nbr + (size_t)i * istride;
+        for (int k = 0; k < cnt; k += 4) {
+            const uint4 q = *reinterpret_cast<const uint4*>(L + k);
+            const uint32_t e[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (k + u < cnt) add(e[u]);
+        }
+    } else {
+        for (int k = 0; k < cnt; ++k) add(nbr[(size_t)k * kstride + (size_t)i * istride]);
     }
     if (inv != 0.0f) psi[i] = P.boundary_pressure_mode == UPF_BOUNDARY_AKINCI ? (float)(0.7 / (double)inv) : (float)(1.0 / (double)inv);
     else set_err(ctl, UPF_ERR_ISOLATED);
@@ -1944,7 +1970,7 @@ __device__ inline void boundary_pressure(int mode, float dist, float h, const fl
 // The coupling force work of slot i (one thread): shared by the coupling
 // pass and the fused list build.  lvl, lpg, lst, lad: the viscosity,
 // pressure-gradient, tension and adhesion tables in LDS.
-template <int MODE>
+template <int MODE, bool SLOT_MAJOR = false>
 __device__ __forceinline__ void ext_coupling_slot(const StepArgs& A, const Slots& S, const Work& W, int i,
                                                   const float* lvl, const float* lpg, const float* lst,
                                                   const float* lad, Ctl* ctl) {
@@ -2048,17 +2074,31 @@ __device__ __forceinline__ void ext_coupling_slot(const StepArgs& A, const Slots
         }
     };
     // list entries in groups of four, the next group's loaded ahead; the next
-    // neighbour's record is loaded before this neighbour's pair runs
+    // neighbour's record is loaded before this neighbour's pair runs.  The
+    // packed list: column-major, or (SLOT_MAJOR) the slot's own row, whose
+    // groups are aligned 16-byte words (entries past the end are not used)
     const size_t st = (size_t)A.stride;
-    uint32_t e[4];
+    const uint32_t* L = SLOT_MAJOR ? W.nbr + (size_t)i * A.nbr_ld : W.nbr + i;
+    auto group = [&](int k0, uint32_t* out) __attribute__((always_inline)) {
+        if constexpr (SLOT_MAJOR) {
+            uint4 q = make_uint4(0u, 0u, 0u, 0u);
+            if (k0 < cnt) q = *reinterpret_cast<const uint4*>(L + k0);
+            out[0] = q.x;
+            out[1] = q.y;
+            out[2] = q.z;
+            out[3] = q.w;
+        } else {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) e[u] = u < cnt ? W.nbr[(size_t)u * st + i] : 0u;
+            for (int u = 0; u < 4; ++u) out[u] = k0 + u < cnt ? L[(size_t)(k0 + u) * st] : 0u;
+        }
+    };
+    uint32_t e[4];
+    group(0, e);
     float4 xn = make_float4(0.f, 0.f, 0.f, 0.f);
     if (cnt > 0) xn = S.pos[e[0] >> NB_SLOT_SHIFT];
     for (int k0 = 0; k0 < cnt; k0 += 4) {
         uint32_t en[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) en[u] = k0 + 4 + u < cnt ? W.nbr[(size_t)(k0 + 4 + u) * st + i] : 0u;
+        group(k0 + 4, en);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int k = k0 + u;
@@ -2146,14 +2186,14 @@ __global__ __launch_bounds__(BLOCK) void k_ext_coupling(StepArgs A, Slots S, Wor
 // kernel: the rigid particles' lists and boundary volumes come first (their
 // counts are kept), then a wave builds the lists of its liquid slots and each
 // lane runs the coupling work of its own slot.
-template <int ROUNDS, int MODE>
+template <int ROUNDS, int MODE, bool SLOT_MAJOR>
 __global__ __launch_bounds__(64) void k_nbr_build_coupling(StepArgs A, Slots S, Work W, const float* __restrict__ luts,
                                                            Ctl* ctl) {
     // the build's candidate table, then the tables of the coupling pass
     __shared__ float tabs[4 * LUT_N];
     static_assert(4 * LUT_N >= CAND_CAP, "candidate table");
-    build_wave<ROUNDS, 0, 1, 0, 1>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl,
-                                   reinterpret_cast<uint32_t*>(tabs));
+    build_wave<ROUNDS, 0, 1, 0, 1, UPF_LIQUID, SLOT_MAJOR>(A, S.pos, W.keys_sorted, W.cells, W.nbr, W.ncount, ctl,
+                                                          reinterpret_cast<uint32_t*>(tabs));
     __syncthreads();  // the block is the wave: its lists are complete
     float* lvl = tabs;
     float* lpg = tabs + LUT_N;
@@ -2167,7 +2207,7 @@ __global__ __launch_bounds__(64) void k_nbr_build_coupling(StepArgs A, Slots S, 
     }
     __syncthreads();
     const int i = xblk(A.xcd) * 64 + (int)threadIdx.x;
-    if (i < A.n) ext_coupling_slot<MODE>(A, S, W, i, lvl, lpg, lst, lad, ctl);
+    if (i < A.n) ext_coupling_slot<MODE, SLOT_MAJOR>(A, S, W, i, lvl, lpg, lst, lad, ctl);
 }
 
 // key of a slot for grouping rigid particles by body (non-rigid last)
@@ -2943,9 +2983,16 @@ static int step_pcisph_rigid(upf_sim* s) {
                                s->W.ncount, s->ctl);
         };
         const int r = s->tune.nbr_rounds;
-        if (r == 2) launch(k_nbr_build_rigid<2>);
-        else if (r == 8) launch(k_nbr_build_rigid<8>);
-        else launch(k_nbr_build_rigid<4>);
+        // slot-major when the fused coupling kernel reads them
+        if (fuse) {
+            if (r == 2) launch(k_nbr_build_rigid<2, true>);
+            else if (r == 8) launch(k_nbr_build_rigid<8, true>);
+            else launch(k_nbr_build_rigid<4, true>);
+        } else {
+            if (r == 2) launch(k_nbr_build_rigid<2, false>);
+            else if (r == 8) launch(k_nbr_build_rigid<8, false>);
+            else launch(k_nbr_build_rigid<4, false>);
+        }
     } else {
         hipLaunchKernelGGL(k_nbr_build_box, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, (int)UPF_RIGID, s->W.nbr,
                            s->W.ncount, s->ctl);
@@ -2957,7 +3004,8 @@ static int step_pcisph_rigid(upf_sim* s) {
             hipLaunchKernelGGL(k_psi, g, dim3(BLOCK), 0, s->st, A, S.pos, s->W.cells, s->luts, S.psi, s->ctl);
             s->psi_ready = true;
         }
-        hipLaunchKernelGGL(k_psi_rigid, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.nbr, A.stride, s->W.ncount, s->P,
+        hipLaunchKernelGGL(k_psi_rigid, g, dim3(BLOCK), 0, s->st, s->n, S.pos, s->W.nbr, fuse ? 1 : A.stride,
+                           fuse ? A.nbr_ld : 1, s->W.ncount, s->P,
                            s->luts, S.psi, s->ctl);
         s->prof.end(s->st);
     }
@@ -2967,13 +3015,13 @@ static int step_pcisph_rigid(upf_sim* s) {
         auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, gb, dim3(64), 0, s->st, A, S, s->W, s->luts, s->ctl); };
         const int r = s->tune.nbr_rounds;
         if (akinci) {
-            if (r == 2) launch(k_nbr_build_coupling<2, UPF_BOUNDARY_AKINCI>);
-            else if (r == 8) launch(k_nbr_build_coupling<8, UPF_BOUNDARY_AKINCI>);
-            else launch(k_nbr_build_coupling<4, UPF_BOUNDARY_AKINCI>);
+            if (r == 2) launch(k_nbr_build_coupling<2, UPF_BOUNDARY_AKINCI, true>);
+            else if (r == 8) launch(k_nbr_build_coupling<8, UPF_BOUNDARY_AKINCI, true>);
+            else launch(k_nbr_build_coupling<4, UPF_BOUNDARY_AKINCI, true>);
         } else {
-            if (r == 2) launch(k_nbr_build_coupling<2, UPF_BOUNDARY_CPU_REF>);
-            else if (r == 8) launch(k_nbr_build_coupling<8, UPF_BOUNDARY_CPU_REF>);
-            else launch(k_nbr_build_coupling<4, UPF_BOUNDARY_CPU_REF>);
+            if (r == 2) launch(k_nbr_build_coupling<2, UPF_BOUNDARY_CPU_REF, true>);
+            else if (r == 8) launch(k_nbr_build_coupling<8, UPF_BOUNDARY_CPU_REF, true>);
+            else launch(k_nbr_build_coupling<4, UPF_BOUNDARY_CPU_REF, true>);
         }
         s->prof.end(s->st);
     } else {
