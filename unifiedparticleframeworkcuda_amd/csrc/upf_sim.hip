@@ -142,6 +142,15 @@ __device__ inline int xblk(int remap) {
     return (blockIdx.x & 7) * per + (blockIdx.x >> 3);
 }
 
+// The same blocks in the opposite order: each XCD walks its eighth from the
+// top down.  A pass that follows one walked upwards starts on the slots whose
+// lists and records the previous pass left in that XCD's L2.
+__device__ inline int xblk_down(int remap) {
+    if (!remap) return gridDim.x - 1 - blockIdx.x;
+    const int per = gridDim.x >> 3;
+    return (blockIdx.x & 7) * per + (per - 1 - (blockIdx.x >> 3));
+}
+
 // ------------------------------------------------------------------ keys
 __global__ void k_keys(StepArgs A, const float4* __restrict__ pos, uint32_t* __restrict__ keys, Ctl* ctl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1467,9 +1476,11 @@ __global__ __launch_bounds__(BLOCK) void k_density(int t, StepArgs A, Slots S, W
     }
     __syncthreads();
     float run = 0.0f;  // single scene: running max over the thread's slots
+    // top down (the list build and the force pass walk upwards), so that the
+    // previous kernel's last slots come first, while they are in L2
     const int n_round = (A.n + gstride() - 1) / gstride();
-    for (int r = 0; r < n_round; ++r) {
-        const int i = r * gstride() + xblk(A.xcd) * blockDim.x + threadIdx.x;
+    for (int r = n_round - 1; r >= 0; --r) {
+        const int i = r * gstride() + xblk_down(A.xcd) * blockDim.x + threadIdx.x;
         if (block_idle(A, i - (int)threadIdx.x, t)) continue;
         int un = -1;
         if constexpr (CODED == 2) un = stage_union(A, W, W.xs, urec, (i - (int)threadIdx.x) / UNION_CHUNK);
