@@ -1147,7 +1147,10 @@ __global__ __launch_bounds__(64) void k_nbr_build_tile(StepArgs A, const float4*
 template <int MODE>
 __device__ __forceinline__ float pressure_record_w(int type, float psi, float rho0) {
     if (MODE == UPF_BOUNDARY_AKINCI && type != UPF_LIQUID) return -(rho0 * psi);
-    return 0.0f;
+    // a liquid particle's w is p / rho0^2 >= +0 (+0 until the first density
+    // pass); any other particle's has its sign bit set, which is how the
+    // pressure-force pass tells them apart (-0 adds as +0 to a pressure term)
+    return type == UPF_LIQUID ? 0.0f : -0.0f;
 }
 
 // The predicted-position record of a non-liquid particle: with Akinci
@@ -1638,10 +1641,14 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
         // batched: a converged scene's slot stops before its record is read (one
         // scene runs iteration t: the kernel's entry test decided it)
         if (A.scene_of && t >= A.sctl[A.scene_of[i]].want) return;
-        const float4 xi = S.pos[i];
-        if (ptype(xi) != UPF_LIQUID) return;
+        // the slot's own record: current position and p / rho0^2 -- the density
+        // pass's p / (rho0 rho0), the same division as W.p[i] / dens_sq -- with
+        // the sign bit set for a non-liquid particle
+        const float4 xr = W.xpd[i];
+        if (__builtin_signbit(xr.w)) return;
+        const float4 xi = make_float4(xr.x, xr.y, xr.z, __int_as_float(UPF_LIQUID));
         const float m = P.initial_mass;
-        const float pi_d = W.p[i] / A.dens_sq;
+        const float pi_d = xr.w;
         float fx = 0.0f, fy = 0.0f, fz = 0.0f;
         const int cnt = W.ncount[i];
         const size_t st = (size_t)A.stride;
