@@ -645,10 +645,12 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
     if (tid == 0) W.un[c] = U;
 }
 
-// GetNeighbors (src/cpu/UnifiedPhysics.cpp:2242-2295) for every liquid slot,
-// one wave per 64 consecutive slots.  For each cell met in its slots the wave
-// lists the occupied cells of the 3x3x3 stencil in Morton order (so that the
-// candidates come in ascending slot order), then tests the candidates 64 at
+// GetNeighbors (src/cpu/UnifiedPhysics.cpp:2242-2295) for every slot of type
+// OWNER (liquid; rigid for the coupling step's rigid lists), one wave per 64
+// consecutive slots.  For each cell met in its slots the wave lists the
+// occupied cells of the 3x3x3 stencil in Morton order (so that the candidates
+// come in ascending slot order) and their candidates in an LDS table (slot,
+// cell rank), then tests the candidates 64 at
 // a time against each particle of the cell: the candidate's cell must lie in
 // the particle's query box (the reference's BoxQuery; decided once per cell
 // and particle for the occupied stencil cells; SKIP: a round of 64 candidates
@@ -694,9 +696,9 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
     const int my_scene = mine_valid ? scene_at(A, me) : -1;
     // owners: the slots whose lists this build writes (liquid, or the rigid
     // slots of the coupling step)
-    const bool my_liquid = mine_valid && ptype(my_x) == OWNER;
+    const bool my_owner = mine_valid && ptype(my_x) == OWNER;
     uint32_t my_blo = 0u, my_bhi = 0u;
-    if (my_liquid) {
+    if (my_owner) {
         // BoxQuery of the particle in cell coordinates, clipped to the grid
         const Dom dm = A.scene_of ? A.doms[my_scene] : A.dom0;
         const float xv[3] = {my_x.x, my_x.y, my_x.z};
@@ -718,13 +720,13 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
     // key of its current position, whose cell's stencil holds its query box
     uint32_t my_seg = my_key;
     if constexpr (OWNER == UPF_RIGID) {
-        if (my_liquid) {
+        if (my_owner) {
             const Dom dm = A.scene_of ? A.doms[my_scene] : A.dom0;
             bool ok;
             my_seg = dkey(my_x.x, my_x.y, my_x.z, dm.scales, ok);
         }
     }
-    const uint64_t liquid = wballot(my_liquid);
+    const uint64_t owners = wballot(my_owner);
     int my_cnt = 0;  // this lane's slot: list length so far
     int s = base;
     while (s < wend) {
@@ -740,10 +742,10 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
         // end of the cell within the wave: first later slot of another cell or scene
         const uint64_t other = wballot(mine_valid && me > s && ((my_seg >> shift3) != c || my_scene != sc));
         const int pe = other ? base + __ffsll((long long)other) - 1 : wend;
-        // the cell's liquid slots (bits sl .. pe - base - 1)
+        // the cell's owner slots (bits sl .. pe - base - 1)
         const uint64_t upto = (pe - base) >= 64 ? ~0ull : ((1ull << (pe - base)) - 1ull);
-        const uint64_t cell_liquid = liquid & upto & ~((1ull << sl) - 1ull);
-        if (cell_liquid == 0ull) {  // no list to build in this cell (walls, rigid particles)
+        const uint64_t cell_owners = owners & upto & ~((1ull << sl) - 1ull);
+        if (cell_owners == 0ull) {  // no list to build in this cell (e.g. walls)
             s = pe;
             continue;
         }
@@ -857,7 +859,7 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
                 }
             }
             const int rounds = min(ROUNDS, (M - c0 + 63) >> 6);
-            uint64_t pm = cell_liquid;
+            uint64_t pm = cell_owners;
             while (pm) {
                 const int pl = __builtin_ctzll(pm);
                 pm &= pm - 1ull;
@@ -910,7 +912,7 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
     }
     // KEEP_RIGID: rigid slots keep the counts of their own lists; a rigid-owner
     // build writes only those
-    if (OWNER == UPF_LIQUID ? mine_valid && !(KEEP_RIGID && ptype(my_x) == UPF_RIGID) : my_liquid) {
+    if (OWNER == UPF_LIQUID ? mine_valid && !(KEEP_RIGID && ptype(my_x) == UPF_RIGID) : my_owner) {
         ncount[me] = my_cnt < cap ? my_cnt : cap;
         if (my_cnt > cap) set_err(ctl, UPF_ERR_NBR_OVERFLOW);
     }
