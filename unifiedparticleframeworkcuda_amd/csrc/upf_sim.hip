@@ -666,7 +666,7 @@ __global__ __launch_bounds__(UNION_CHUNK) void k_union(StepArgs A, Work W) {
 // consecutive words (for a reader in the same kernel).
 constexpr int CAND_CAP = 512;  // stencil candidates held in the LDS candidate table
 template <int ROUNDS, int STAGE, int BIN, int SKIP, int KEEP_RIGID = 0, int OWNER = UPF_LIQUID,
-          bool SLOT_MAJOR = false>
+          bool SLOT_MAJOR = false, int SPLIT = 1>
 __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __restrict__ pos,
                                            const uint32_t* __restrict__ keys, const uint2* __restrict__ cells,
                                            uint32_t* __restrict__ nbr, int* __restrict__ ncount, Ctl* ctl,
@@ -680,7 +680,12 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
     const int n = A.n;
     const int stride = A.stride;
     const int lane = threadIdx.x;
-    const int base = xblk(A.xcd) * 64;
+    // SPLIT waves share a window of 64 slots, each building the lists of the
+    // owners in its 64 / SPLIT lanes (more, shorter waves where owners are
+    // few and dense, as the rigid particles of the coupling step)
+    const int vb = xblk(A.xcd);
+    const int base = (vb / SPLIT) * 64;
+    const int part = vb % SPLIT;
     if (base >= n) return;
     const int wend = min(base + 64, n);
     const float T = P.support_radius_sq_limit;
@@ -696,7 +701,7 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
     const int my_scene = mine_valid ? scene_at(A, me) : -1;
     // owners: the slots whose lists this build writes (liquid, or the rigid
     // slots of the coupling step)
-    const bool my_owner = mine_valid && ptype(my_x) == OWNER;
+    const bool my_owner = mine_valid && ptype(my_x) == OWNER && (SPLIT == 1 || lane / (64 / SPLIT) == part);
     uint32_t my_blo = 0u, my_bhi = 0u;
     if (my_owner) {
         // BoxQuery of the particle in cell coordinates, clipped to the grid
@@ -939,13 +944,17 @@ __global__ __launch_bounds__(64) void k_nbr_build(StepArgs A, const float4* __re
 // wave-cooperative build; the other slots' counts are left as they are.
 // SLOT_MAJOR: the lists in slot-major rows (as the fused coupling kernel
 // writes and reads the liquid ones).
+// The rigid particles of a body are packed densely in few windows of 64
+// slots: four waves share each window (RIGID_SPLIT), so that many more waves
+// run the build at once.
+constexpr int RIGID_SPLIT = 4;
 template <int ROUNDS, bool SLOT_MAJOR>
 __global__ __launch_bounds__(64) void k_nbr_build_rigid(StepArgs A, const float4* __restrict__ pos,
                                                         const uint32_t* __restrict__ keys,
                                                         const uint2* __restrict__ cells, uint32_t* __restrict__ nbr,
                                                         int* __restrict__ ncount, Ctl* ctl) {
     __shared__ uint32_t cand[CAND_CAP];
-    build_wave<ROUNDS, 0, 1, 0, 0, UPF_RIGID, SLOT_MAJOR>(A, pos, keys, cells, nbr, ncount, ctl, cand);
+    build_wave<ROUNDS, 0, 1, 0, 0, UPF_RIGID, SLOT_MAJOR, RIGID_SPLIT>(A, pos, keys, cells, nbr, ncount, ctl, cand);
 }
 
 // GetNeighbors again, tile form: one wave per 64 consecutive slots of one
@@ -2997,7 +3006,8 @@ static int step_pcisph_rigid(upf_sim* s) {
     if (!fuse) build_lists(s, A);  // liquid owners
     s->prof.begin(PC_NBR_BUILD, s->st);
     if (s->tune.nbr_tile == 0) {
-        const dim3 gb((unsigned)(((s->n + 63) / 64 + 7) / 8 * 8));  // a multiple of 8 (XCD remap)
+        // RIGID_SPLIT waves per window of 64 slots, a multiple of 8 (XCD remap)
+        const dim3 gb((unsigned)(((s->n + 63) / 64 * RIGID_SPLIT + 7) / 8 * 8));
         auto launch = [&](auto kern) {
             hipLaunchKernelGGL(kern, gb, dim3(64), 0, s->st, A, S.pos, s->W.keys_sorted, s->W.cells, s->W.nbr,
                                s->W.ncount, s->ctl);
