@@ -1697,7 +1697,12 @@ __global__ __launch_bounds__(BLOCK) void k_force(int t, StepArgs A, Slots S, Wor
             for_each_listed<CODED, G>(W, i, st, cnt, from_global, use);
         }
         const float4 Fp = make_float4(fx, fy, fz, 0.0f);
-        W.fp[i] = Fp;
+        // only the last iteration's pressure force is read (by the integration):
+        // it is written when this iteration may be the scene's last -- the scene's
+        // loop count is t + 1 until this iteration's test (at the start of this
+        // kernel) raises it, so a slot that reads it before the test writes too
+        const int want = __atomic_load_n(&A.sctl[A.scene_of ? A.scene_of[i] : 0].want, __ATOMIC_RELAXED);
+        if (t + 1 >= want) W.fp[i] = Fp;
         W.xs[i] = predict(xi, S.vel[i], W.force[i], Fp, A, dom_at(A, i), ctl);
     };
     auto load_tables = [&]() {
