@@ -888,9 +888,10 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
                     const uint64_t m = wballot(inbox) & wballot(near) & wballot(other);
                     if (m == 0ull) continue;
                     if (pass) {
-                        const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                                         __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-                        const int slot = cnt + (int)below;
+                        // place in the list: cnt + the passing lanes below this one
+                        const int slot = (int)__builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, (uint32_t)cnt));
+                        const uint32_t below = (uint32_t)(slot - cnt);
                         // BIN == 0: the table bin is left to the readers (the
                         // external-force pass computes it from the same distance)
                         const uint32_t e = ((uint32_t)cj[r] << NB_SLOT_SHIFT) | (BIN ? lut_bin(sqrtf(d2), lut_factor) : 0u);
@@ -898,9 +899,11 @@ __device__ __forceinline__ void build_wave(const StepArgs& A, const float4* __re
                             if (slot < STAGE) stage[slot * 65 + pl] = e;
                             else if (slot < cap) nbr[(size_t)slot * stride + p] = e;
                         } else if constexpr (SLOT_MAJOR) {
-                            // the particle's own row: the round's entries are
-                            // consecutive words
-                            if (slot < cap) nbr[(size_t)p * A.nbr_ld + slot] = e;
+                            // the particle's own row (a uniform base): the
+                            // round's entries are consecutive words
+                            uint32_t* row = nbr + (size_t)p * A.nbr_ld;
+                            if (slot < cap)
+                                *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(row) + (uint32_t)slot * 4u) = e;
                         } else {
                             // row cnt of the list column, then `below` rows on
                             // (slots < 2^23: the offset fits 24-bit multiplies)
